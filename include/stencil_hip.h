@@ -1,0 +1,174 @@
+/*
+ * stencil_hip.h -- C ABI of libstencil_hip, the MI355X (gfx950) sweep library
+ * that replaces the reference's time-stepping hot path.
+ *
+ * Reference interface replaced (files of mathias-puetz/stencil3d):
+ *   ITERATE(v,s,u,grid,prefetch,dups,iterations,cacheiter)
+ *       iterators.h:6-29, iterate_3d iterators.cpp:130-157,
+ *       selected by config.h:55-65                      -> sth_iterate
+ *   per-point kernels stencil_{7,13,25}_ref_1
+ *       stencil_7.h:31-39, stencil_13.h:50-61, stencil_25.h:75-83
+ *                                                       -> device code behind sth_iterate
+ *   s_grid geometry (grid.h:7-71, grid.cpp:9-48)        -> sth_geom / sth_layout
+ *   array allocation (main.cpp:244-275)                 -> sth_create (hipMalloc)
+ *   init_arrays (iterators.h:32, iterators.cpp:292-301) -> sth_init_const
+ *   check (iterators.h:35, iterators.cpp:304-325)       -> sth_check
+ *   the leapfrog update of README.md:10-24 (no reference code) -> sth_leapfrog
+ *
+ * Conventions: every call returns an sth_status (0 = OK) and never exits the
+ * process.  A context owns its device memory and one HIP stream; calls on a
+ * context are stream-ordered and asynchronous until sth_sync / a download /
+ * a check.  One host thread per context; contexts are independent.  Axis
+ * naming follows the reference: i (x, slowest), j (y), k (z, unit stride).
+ */
+#ifndef STENCIL_HIP_H
+#define STENCIL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STH_VERSION 1
+
+typedef enum {
+  STH_OK = 0,
+  STH_ERR_ARG = 1,    /* invalid argument (the reference exits with 2, main.cpp:149-150) */
+  STH_ERR_HIP = 2,    /* HIP runtime error */
+  STH_ERR_NOMEM = 3,  /* device allocation failed */
+  STH_ERR_STATE = 4,  /* call not valid in the context's current state */
+  STH_ERR_NODEV = 5,  /* no usable GPU */
+  STH_ERR_UNSUPPORTED = 6
+} sth_status;
+
+typedef enum { STH_FP32 = 0, STH_FP16 = 1, STH_BF16 = 2 } sth_dtype;
+
+/* Logical fields.  In REF mode v is updated in place and u, s are read-only.
+ * In LEAPFROG mode u is the newest field and v the previous one; each step
+ * writes the new field over v and swaps the two names. */
+typedef enum { STH_FIELD_V = 0, STH_FIELD_S = 1, STH_FIELD_U = 2 } sth_field;
+
+/* Geometry of a run (s_grid, grid.h:7-71).  The device keeps a halo of
+ * exactly the stencil radius; the caller's halo widths describe its own host
+ * buffers (sth_host_layout).  zs, ys, xs: 1..65535. */
+typedef struct {
+  uint32_t xs, ys, zs; /* interior extents (--xs --ys --zs) */
+  uint32_t npts;       /* 7, 13 or 25 (STENCIL_SIZE, config.h:37-39) */
+  uint32_t dtype;      /* sth_dtype: storage type of u, v, s */
+} sth_geom;
+
+/* Device layout (host-side computation, no GPU needed).  Element (i,j,k),
+ * -R <= i < xs+R etc., lives at element offset
+ *   origin + k + j*pitch + i*plane
+ * from the start of the field; pitch is a multiple of 256 bytes and
+ * origin*elem_bytes is a multiple of 256, so every interior row starts on a
+ * 256-byte boundary. */
+typedef struct {
+  uint32_t radius;     /* R = 1, 2, 4 */
+  uint32_t elem_bytes; /* 4 or 2 */
+  uint64_t pitch;      /* elements between consecutive j rows */
+  uint64_t plane;      /* elements between consecutive i planes */
+  uint64_t origin;     /* element offset of interior point (0,0,0) */
+  uint64_t elems;      /* elements allocated per field */
+  uint64_t bytes;      /* bytes allocated per field */
+} sth_layout_info;
+
+/* A caller's dense host array in the reference layout (grid.cpp:20-25):
+ * jmult = zs+2zh, imult = jmult*(ys+2yh), element (i,j,k) at
+ * zh + yh*jmult + xh*imult + k + j*jmult + i*imult.  Halo widths must be at
+ * least the stencil radius. */
+typedef struct {
+  uint32_t xh, yh, zh;
+} sth_host_layout;
+
+/* Tile schedule knobs of the 2.5D sweep (the GPU counterpart of --yb/--zb,
+ * iterators.cpp:42-89).  Zero fields select the built-in defaults. */
+typedef struct {
+  uint32_t rows_per_wave; /* j rows held in registers by one wave: 1, 2, 4, 8 */
+  uint32_t waves;         /* waves per workgroup: 1, 2, 4, 8 */
+  uint32_t chunks;        /* i-range split: 1..64 chunks per (j,k) tile */
+  uint32_t xcd_remap;     /* 1: keep neighbouring tiles on one XCD (default), 0: off */
+} sth_tuning;
+
+typedef struct sth_ctx sth_ctx;
+
+const char *sth_strerror(int status);
+int sth_version(void);
+/* Host-only: device layout for a geometry (no GPU access). */
+int sth_layout(const sth_geom *g, sth_layout_info *out);
+/* Host-only: bytes of one field of the caller's dense layout. */
+int sth_host_elems(const sth_geom *g, const sth_host_layout *hl, uint64_t *elems);
+int sth_device_count(int *n);
+
+/* Creates a context on the current HIP device: one stream and three
+ * hipMalloc'd fields (the reference's aligned_alloc, main.cpp:244-255). */
+int sth_create(const sth_geom *g, sth_ctx **out);
+int sth_destroy(sth_ctx *ctx);
+int sth_get_layout(const sth_ctx *ctx, sth_layout_info *out);
+
+/* ---- field initialisation and transfer ----------------------------------- */
+/* init_arrays (iterators.cpp:292-301): every element of the allocation,
+ * halos included, v = v0, s = s0, u = u0.  REF mode; FP32 storage only. */
+int sth_init_const(sth_ctx *ctx, float v0, float s0, float u0);
+/* Seeded uniform [lo,hi) interior from the counter-based SplitMix64 stream
+ * (seed, field_id) -- identical values to the oracle's generator -- and the
+ * halo filled with halo_value. */
+int sth_init_uniform(sth_ctx *ctx, sth_field f, uint64_t seed, int field_id, double lo,
+                     double hi, float halo_value);
+/* Gaussian pulse amp*exp(-d^2/(2 sigma^2)) about the interior centre plus
+ * seeded uniform noise in [-noise, noise); halo 0. */
+int sth_init_gaussian(sth_ctx *ctx, sth_field f, double amp, double sigma, double noise,
+                      uint64_t seed, int field_id);
+/* Zero field with one interior point set to value (a point source mask). */
+int sth_init_point(sth_ctx *ctx, sth_field f, uint32_t i, uint32_t j, uint32_t k, float value);
+/* FP32 host array in the caller's dense layout -> device field (interior and
+ * the radius-wide halo; wider host halo cells are not read).  16-bit storage
+ * fields are quantized with a power-of-two scale chosen from the data's max. */
+int sth_upload(sth_ctx *ctx, sth_field f, const float *host, const sth_host_layout *hl);
+/* Device field -> FP32 host array (interior and radius-wide halo; other host
+ * cells are left untouched).  Synchronises the stream. */
+int sth_download(sth_ctx *ctx, sth_field f, float *host, const sth_host_layout *hl);
+/* 16-bit storage: raw stored values and the field's scale exponent
+ * (value = stored * 2^exp).  upload_raw sets the field's max from the data. */
+int sth_upload_raw16(sth_ctx *ctx, sth_field f, const uint16_t *host, int exp,
+                     const sth_host_layout *hl);
+int sth_download_raw16(sth_ctx *ctx, sth_field f, uint16_t *host, int *exp, float *maxabs,
+                       const sth_host_layout *hl);
+
+/* ---- the sweep -------------------------------------------------------------- */
+/* ITERATE replacement (iterate_3d, iterators.cpp:130-157): `iterations` REF
+ * sweeps v <- v - A*u + s*(coef0*u + sum c*(u[-r]+u[+r])) in the reference's
+ * FMA order (bit-identical to the reference host build).  FP32 storage.
+ * One kernel launch per sweep; asynchronous. */
+int sth_iterate(sth_ctx *ctx, uint32_t iterations);
+/* LEAPFROG steps (formula B, README.md:10-24):
+ *   v <- c0*u - v + c1*L(u) + c2[n]*s,  L(u) = sum 2*c_axis,r*(u[-r]+u[+r]),
+ * then u and v swap.  c2 is a host array of `steps` per-step source
+ * coefficients (NULL: all zero).  16-bit storage requires zero halos. */
+int sth_leapfrog(sth_ctx *ctx, uint32_t steps, float c0, float c1, const float *c2);
+int sth_sync(sth_ctx *ctx);
+
+/* check (iterators.cpp:304-325): interior points with |v - expected| > 0.1. */
+int sth_check(sth_ctx *ctx, float expected, uint64_t *errors);
+/* Interior max |x| and L2 norm of a field, accumulated in FP64 on the GPU. */
+int sth_norms(sth_ctx *ctx, sth_field f, double *maxabs, double *l2);
+
+/* ---- schedule and timing ---------------------------------------------------- */
+int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
+int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
+/* Times the candidate schedules on this context's geometry (REF or LEAPFROG
+ * sweeps, `reps` launches each, fields are modified), keeps the fastest and
+ * returns it with its mean kernel time in ms. */
+int sth_autotune(sth_ctx *ctx, int leapfrog, uint32_t reps, sth_tuning *best, float *best_ms);
+/* Kernel timing: HIP events recorded on the context's stream around every
+ * sweep launch between timer_start and timer_read.  timer_read synchronises
+ * and returns the summed kernel time (ms) and the number of launches. */
+int sth_timer_start(sth_ctx *ctx);
+int sth_timer_read(sth_ctx *ctx, double *kernel_ms, uint64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

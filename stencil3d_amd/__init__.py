@@ -1,0 +1,301 @@
+"""stencil3d_amd -- Python binding of libstencil_hip, the MI355X (gfx950) sweep
+library of stencil3d.
+
+The library is the product; this module only loads it with ctypes and moves
+numpy arrays in and out.  It mirrors the reference's operator interface
+(iterators.h:6-35): ``init_arrays``, ``iterate`` (ITERATE), ``check``, plus the
+leapfrog and 16-bit storage forms.  Arrays use the reference's dense padded
+layout (grid.cpp:20-25): shape (xs+2xh, ys+2yh, zs+2zh), C order, k fastest.
+
+There is no CPU fallback: if libstencil_hip.so is missing, or no GPU is
+visible, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libstencil_hip.so")
+FRONTEND_PATH = os.path.join(PKG, "hstencil3d_hip")
+
+STH_OK, STH_ERR_ARG, STH_ERR_HIP, STH_ERR_NOMEM, STH_ERR_STATE, STH_ERR_NODEV, \
+    STH_ERR_UNSUPPORTED = range(7)
+DTYPES = {"fp32": 0, "fp16": 1, "bf16": 2}
+FIELDS = {"v": 0, "s": 1, "u": 2}
+
+
+class SthError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {_lib_or_none_strerror(status)} (status {status})")
+
+
+class Geom(C.Structure):
+    _fields_ = [("xs", C.c_uint32), ("ys", C.c_uint32), ("zs", C.c_uint32),
+                ("npts", C.c_uint32), ("dtype", C.c_uint32)]
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [("radius", C.c_uint32), ("elem_bytes", C.c_uint32), ("pitch", C.c_uint64),
+                ("plane", C.c_uint64), ("origin", C.c_uint64), ("elems", C.c_uint64),
+                ("bytes", C.c_uint64)]
+
+
+class HostLayout(C.Structure):
+    _fields_ = [("xh", C.c_uint32), ("yh", C.c_uint32), ("zh", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("rows_per_wave", C.c_uint32), ("waves", C.c_uint32), ("chunks", C.c_uint32),
+                ("xcd_remap", C.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# Every symbol include/stencil_hip.h declares, with its ctypes signature.
+_P = C.POINTER
+_VP = C.c_void_p
+SIGNATURES = {
+    "sth_strerror": (C.c_char_p, [C.c_int]),
+    "sth_version": (C.c_int, []),
+    "sth_layout": (C.c_int, [_P(Geom), _P(LayoutInfo)]),
+    "sth_host_elems": (C.c_int, [_P(Geom), _P(HostLayout), _P(C.c_uint64)]),
+    "sth_device_count": (C.c_int, [_P(C.c_int)]),
+    "sth_create": (C.c_int, [_P(Geom), _P(_VP)]),
+    "sth_destroy": (C.c_int, [_VP]),
+    "sth_get_layout": (C.c_int, [_VP, _P(LayoutInfo)]),
+    "sth_init_const": (C.c_int, [_VP, C.c_float, C.c_float, C.c_float]),
+    "sth_init_uniform": (C.c_int, [_VP, C.c_int, C.c_uint64, C.c_int, C.c_double, C.c_double,
+                                   C.c_float]),
+    "sth_init_gaussian": (C.c_int, [_VP, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_uint64, C.c_int]),
+    "sth_init_point": (C.c_int, [_VP, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float]),
+    "sth_upload": (C.c_int, [_VP, C.c_int, _P(C.c_float), _P(HostLayout)]),
+    "sth_download": (C.c_int, [_VP, C.c_int, _P(C.c_float), _P(HostLayout)]),
+    "sth_upload_raw16": (C.c_int, [_VP, C.c_int, _P(C.c_uint16), C.c_int, _P(HostLayout)]),
+    "sth_download_raw16": (C.c_int, [_VP, C.c_int, _P(C.c_uint16), _P(C.c_int),
+                                     _P(C.c_float), _P(HostLayout)]),
+    "sth_iterate": (C.c_int, [_VP, C.c_uint32]),
+    "sth_leapfrog": (C.c_int, [_VP, C.c_uint32, C.c_float, C.c_float, _P(C.c_float)]),
+    "sth_sync": (C.c_int, [_VP]),
+    "sth_check": (C.c_int, [_VP, C.c_float, _P(C.c_uint64)]),
+    "sth_norms": (C.c_int, [_VP, C.c_int, _P(C.c_double), _P(C.c_double)]),
+    "sth_set_tuning": (C.c_int, [_VP, _P(Tuning)]),
+    "sth_get_tuning": (C.c_int, [_VP, _P(Tuning)]),
+    "sth_autotune": (C.c_int, [_VP, C.c_int, C.c_uint32, _P(Tuning), _P(C.c_float)]),
+    "sth_timer_start": (C.c_int, [_VP]),
+    "sth_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libstencil_hip.so (built in-tree by stencil3d_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m stencil3d_amd.build` "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _lib_or_none_strerror(status):
+    try:
+        return lib().sth_strerror(status).decode()
+    except Exception:  # noqa: BLE001 - only used to format an error message
+        return "error"
+
+
+def _ok(status, what):
+    if status != STH_OK:
+        raise SthError(status, what)
+
+
+def radius(npts):
+    return {7: 1, 13: 2, 25: 4}[npts]
+
+
+def layout(xs, ys, zs, npts, dtype="fp32"):
+    g = Geom(xs, ys, zs, npts, DTYPES[dtype])
+    out = LayoutInfo()
+    _ok(lib().sth_layout(C.byref(g), C.byref(out)), "sth_layout")
+    return out
+
+
+def device_count():
+    n = C.c_int(0)
+    lib().sth_device_count(C.byref(n))
+    return n.value
+
+
+class Stencil:
+    """One run of the sweep on the current GPU: three device fields of an
+    xs x ys x zs interior (i slowest, k unit stride) for an npts-point stencil,
+    stored as dtype ('fp32', 'fp16', 'bf16')."""
+
+    def __init__(self, xs, ys, zs, npts=7, dtype="fp32"):
+        self.xs, self.ys, self.zs, self.npts, self.dtype = xs, ys, zs, npts, dtype
+        self.R = radius(npts)
+        self._g = Geom(xs, ys, zs, npts, DTYPES[dtype])
+        h = C.c_void_p()
+        _ok(lib().sth_create(C.byref(self._g), C.byref(h)), "sth_create")
+        self._h = h
+
+    # -- lifetime
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sth_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise RuntimeError("context closed")
+        return self._h
+
+    def layout(self):
+        out = LayoutInfo()
+        _ok(lib().sth_get_layout(self.handle, C.byref(out)), "sth_get_layout")
+        return out
+
+    # -- fields
+    def host_shape(self, halo):
+        xh, yh, zh = _halo3(halo)
+        return (self.xs + 2 * xh, self.ys + 2 * yh, self.zs + 2 * zh)
+
+    def init_const(self, v=0.0, s=1.0, u=1.0):
+        """init_arrays (iterators.cpp:292-301): v=0, s=1, u=1 everywhere."""
+        _ok(lib().sth_init_const(self.handle, v, s, u), "sth_init_const")
+
+    def init_uniform(self, field, seed, field_id, lo=-1.0, hi=1.0, halo_value=0.0):
+        _ok(lib().sth_init_uniform(self.handle, FIELDS[field], seed, field_id, lo, hi,
+                                   halo_value), "sth_init_uniform")
+
+    def init_gaussian(self, field, amp=1.0, sigma=8.0, noise=1e-3, seed=1, field_id=0):
+        _ok(lib().sth_init_gaussian(self.handle, FIELDS[field], amp, sigma, noise, seed,
+                                    field_id), "sth_init_gaussian")
+
+    def init_point(self, field, i, j, k, value=1.0):
+        _ok(lib().sth_init_point(self.handle, FIELDS[field], i, j, k, value), "sth_init_point")
+
+    def upload(self, field, arr, halo):
+        arr = np.ascontiguousarray(arr, dtype=np.float32)
+        assert arr.shape == self.host_shape(halo), (arr.shape, self.host_shape(halo))
+        hl = HostLayout(*_halo3(halo))
+        _ok(lib().sth_upload(self.handle, FIELDS[field],
+                             arr.ctypes.data_as(_P(C.c_float)), C.byref(hl)), "sth_upload")
+
+    def download(self, field, halo, out=None):
+        if out is None:
+            out = np.zeros(self.host_shape(halo), dtype=np.float32)
+        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"]
+        hl = HostLayout(*_halo3(halo))
+        _ok(lib().sth_download(self.handle, FIELDS[field],
+                               out.ctypes.data_as(_P(C.c_float)), C.byref(hl)), "sth_download")
+        return out
+
+    def upload_raw16(self, field, arr, exp, halo):
+        arr = np.ascontiguousarray(arr, dtype=np.uint16)
+        assert arr.shape == self.host_shape(halo)
+        hl = HostLayout(*_halo3(halo))
+        _ok(lib().sth_upload_raw16(self.handle, FIELDS[field],
+                                   arr.ctypes.data_as(_P(C.c_uint16)), exp, C.byref(hl)),
+            "sth_upload_raw16")
+
+    def download_raw16(self, field, halo):
+        out = np.zeros(self.host_shape(halo), dtype=np.uint16)
+        e, mx = C.c_int(), C.c_float()
+        hl = HostLayout(*_halo3(halo))
+        _ok(lib().sth_download_raw16(self.handle, FIELDS[field],
+                                     out.ctypes.data_as(_P(C.c_uint16)), C.byref(e),
+                                     C.byref(mx), C.byref(hl)), "sth_download_raw16")
+        return out, e.value, mx.value
+
+    # -- the sweep
+    def iterate(self, iterations):
+        """ITERATE replacement: REF sweeps, v updated in place."""
+        _ok(lib().sth_iterate(self.handle, iterations), "sth_iterate")
+
+    def leapfrog(self, steps, c0, c1, c2=None):
+        c2a = None
+        if c2 is not None:
+            c2a = np.ascontiguousarray(np.broadcast_to(np.asarray(c2, np.float32), (steps,)))
+        ptr = None if c2a is None else c2a.ctypes.data_as(_P(C.c_float))
+        _ok(lib().sth_leapfrog(self.handle, steps, c0, c1, ptr), "sth_leapfrog")
+
+    def sync(self):
+        _ok(lib().sth_sync(self.handle), "sth_sync")
+
+    def check(self, expected):
+        """check (iterators.cpp:304-325): count of |v - expected| > 0.1."""
+        n = C.c_uint64()
+        _ok(lib().sth_check(self.handle, expected, C.byref(n)), "sth_check")
+        return n.value
+
+    def norms(self, field):
+        mx, l2 = C.c_double(), C.c_double()
+        _ok(lib().sth_norms(self.handle, FIELDS[field], C.byref(mx), C.byref(l2)), "sth_norms")
+        return mx.value, l2.value
+
+    # -- schedule and timing
+    def set_tuning(self, rows_per_wave=0, waves=0, chunks=0, xcd_remap=1):
+        t = Tuning(rows_per_wave, waves, chunks, xcd_remap)
+        _ok(lib().sth_set_tuning(self.handle, C.byref(t)), "sth_set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        _ok(lib().sth_get_tuning(self.handle, C.byref(t)), "sth_get_tuning")
+        return t.as_dict()
+
+    def autotune(self, leapfrog=True, reps=3):
+        t, ms = Tuning(), C.c_float()
+        _ok(lib().sth_autotune(self.handle, 1 if leapfrog else 0, reps, C.byref(t),
+                               C.byref(ms)), "sth_autotune")
+        return t.as_dict(), ms.value
+
+    def timer_start(self):
+        _ok(lib().sth_timer_start(self.handle), "sth_timer_start")
+
+    def timer_read(self):
+        ms, n = C.c_double(), C.c_uint64()
+        _ok(lib().sth_timer_read(self.handle, C.byref(ms), C.byref(n)), "sth_timer_read")
+        return ms.value, n.value
+
+
+def _halo3(halo):
+    if isinstance(halo, int):
+        return (halo, halo, halo)
+    return tuple(int(h) for h in halo)
+
+
+# Names of the reference's operator interface (iterators.h:6-35).
+def init_arrays(ctx: Stencil):
+    ctx.init_const(0.0, 1.0, 1.0)
+
+
+def iterate_3d(ctx: Stencil, iterations: int):
+    ctx.iterate(iterations)
+
+
+def check(ctx: Stencil, maxiter: int) -> int:
+    return ctx.check(float(maxiter))

@@ -1,0 +1,40 @@
+// fields.h -- descriptors and launchers of the field kernels (fields.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sweep_kernel.h"
+
+namespace sth {
+
+constexpr int kReduceBlocks = 2048;
+
+struct FieldDesc {
+  int64_t pitch, plane;  // elements
+  uint64_t align;        // elements per 256 B (the front pad of every row)
+  uint64_t elems;        // elements allocated
+  int xs, ys, zs, R;
+};
+
+enum { FILL_CONST = 0, FILL_UNIFORM = 1, FILL_GAUSS = 2, FILL_POINT = 3 };
+
+struct FillSpec {
+  int kind;
+  float a;       // CONST value / POINT value
+  float halo;    // UNIFORM halo value
+  double lo, hi; // UNIFORM range; GAUSS: lo = amplitude, hi = sigma
+  double noise;  // GAUSS noise half-width
+  uint64_t key;  // SplitMix64 stream key of (seed, field_id)
+  int pi, pj, pk;
+};
+
+hipError_t launch_fill(const FieldDesc &d, const FillSpec &f, void *dst, int fmt, int e, int pass,
+                       uint32_t *maxbits, hipStream_t st);
+hipError_t launch_check(const FieldDesc &d, const void *v, int fmt, int e, float expected,
+                        unsigned long long *errors, hipStream_t st);
+hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, double *partial,
+                        uint32_t *maxbits, double *out, hipStream_t st);
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st);
+
+}  // namespace sth

@@ -1,0 +1,200 @@
+// fields.hip -- field initialisation, check and norm kernels of libstencil_hip.
+//
+// These run once per run (untimed in the reference too: init_arrays
+// iterators.cpp:292-301, check iterators.cpp:304-325).  Every kernel walks the
+// device layout of sth_common.h: element e of a field is the cell
+//   t = e - align + R;  i = t / plane - R;  j = (t % plane) / pitch - R;
+//   k = (t % plane) % pitch - R,
+// a cell with i < xs+R, j < ys+R, k < zs+R; every other element is padding.
+#include <hip/hip_runtime.h>
+
+#include "fields.h"
+
+namespace sth {
+
+struct Cell {
+  int i, j, k;
+  bool cell;      // inside the halo-padded box
+  bool interior;  // inside the interior
+};
+
+__device__ inline Cell decode(const FieldDesc &d, uint64_t e) {
+  Cell c{0, 0, 0, false, false};
+  const int64_t t = (int64_t)e - (int64_t)d.align + d.R;
+  if (t < 0) return c;
+  const int64_t ip = t / d.plane, rem = t % d.plane;
+  const int64_t jp = rem / d.pitch, kp = rem % d.pitch;
+  c.i = (int)ip - d.R;
+  c.j = (int)jp - d.R;
+  c.k = (int)kp - d.R;
+  c.cell = c.i < d.xs + d.R && c.j < d.ys + d.R && c.k < d.zs + d.R;
+  c.interior = c.cell && c.i >= 0 && c.j >= 0 && c.k >= 0 && c.i < d.xs && c.j < d.ys && c.k < d.zs;
+  return c;
+}
+
+__device__ inline uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Counter-based SplitMix64 value in [0,1) at coordinate (i,j,k) (the same
+// generator as the oracle: value = mix(key + (c+1)*G), top 24 bits).
+__device__ inline double u01(uint64_t key, int i, int j, int k) {
+  const uint64_t c = ((uint64_t)(i + (1 << 20)) << 42) | ((uint64_t)(j + (1 << 20)) << 21) |
+                     (uint64_t)(k + (1 << 20));
+  const uint64_t x = mix64(key + (c + 1) * 0x9E3779B97F4A7C15ull);
+  return (double)(x >> 40) * 0x1.0p-24;
+}
+
+__device__ inline float gen_value(const FillSpec &f, const FieldDesc &d, const Cell &c) {
+  if (!c.cell) return f.kind == FILL_CONST ? f.a : 0.0f;
+  switch (f.kind) {
+    case FILL_CONST:
+      return f.a;
+    case FILL_UNIFORM:
+      if (!c.interior) return f.halo;
+      return (float)(f.lo + (f.hi - f.lo) * u01(f.key, c.i, c.j, c.k));
+    case FILL_GAUSS: {
+      if (!c.interior) return 0.0f;
+      const double di = (double)c.i - 0.5 * (double)(d.xs - 1);
+      const double dj = (double)c.j - 0.5 * (double)(d.ys - 1);
+      const double dk = (double)c.k - 0.5 * (double)(d.zs - 1);
+      const double d2 = di * di + dj * dj + dk * dk;
+      double val = f.lo * exp(-d2 / (2.0 * f.hi * f.hi));
+      if (f.noise != 0.0) val += -f.noise + 2.0 * f.noise * u01(f.key, c.i, c.j, c.k);
+      return (float)val;
+    }
+    case FILL_POINT:
+      return (c.interior && c.i == f.pi && c.j == f.pj && c.k == f.pk) ? f.a : 0.0f;
+  }
+  return 0.0f;
+}
+
+// pass 0: max |value| over cells (atomicMax of FP32 bits); pass 1: store.
+__global__ void fill_kernel(FieldDesc d, FillSpec f, void *dst, int fmt, int e, int pass,
+                            uint32_t *maxbits) {
+  const uint64_t n = d.elems;
+  float mx = 0.0f;
+  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const Cell c = decode(d, idx);
+    const float x = gen_value(f, d, c);
+    if (pass == 0) {
+      if (c.cell) mx = fmaxf(mx, fabsf(x));
+    } else if (fmt == FMT_F32) {
+      ((float *)dst)[idx] = x;
+    } else if (fmt == FMT_F16) {
+      ((uint16_t *)dst)[idx] = Codec<FMT_F16>::enc(x, e);
+    } else {
+      ((uint16_t *)dst)[idx] = Codec<FMT_BF16>::enc(x, e);
+    }
+  }
+  if (pass == 0) {
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(mx));
+  }
+}
+
+__device__ inline float load_dec(const void *p, uint64_t idx, int fmt, int e) {
+  if (fmt == FMT_F32) return ((const float *)p)[idx];
+  if (fmt == FMT_F16) return Codec<FMT_F16>::dec(((const uint16_t *)p)[idx], e);
+  return Codec<FMT_BF16>::dec(((const uint16_t *)p)[idx], e);
+}
+
+// check(): interior cells with |v - expected| > 0.1f (iterators.cpp:319).
+__global__ void check_kernel(FieldDesc d, const void *v, int fmt, int e, float expected,
+                             unsigned long long *errors) {
+  unsigned long long cnt = 0;
+  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < d.elems;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const Cell c = decode(d, idx);
+    if (c.interior && fabsf(load_dec(v, idx, fmt, e) - expected) > 0.1f) cnt++;
+  }
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+  if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(errors, cnt);
+}
+
+// Interior max |x| and per-block sums of x^2 in FP64 (summed in block order
+// by norms_finish, so the L2 norm is deterministic).
+__global__ void norms_kernel(FieldDesc d, const void *x, int fmt, int e, double *partial,
+                             uint32_t *maxbits) {
+  __shared__ double ss[16];
+  __shared__ float sm[16];
+  double s = 0.0;
+  float mx = 0.0f;
+  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < d.elems;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const Cell c = decode(d, idx);
+    if (c.interior) {
+      const float val = load_dec(x, idx, fmt, e);
+      s += (double)val * (double)val;
+      mx = fmaxf(mx, fabsf(val));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s += __shfl_xor(s, o, 64);
+    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    ss[w] = s;
+    sm[w] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    float m = 0.0f;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+      t += ss[q];
+      m = fmaxf(m, sm[q]);
+    }
+    partial[blockIdx.x] = t;
+    atomicMax(maxbits, __float_as_uint(m));
+  }
+}
+
+__global__ void norms_finish(const double *partial, int n, double *out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double t = 0.0;
+    for (int b = 0; b < n; b++) t += partial[b];
+    out[0] = t;
+  }
+}
+
+// Source-free zero fill of a small device buffer (graph-safe replacement for
+// hipMemsetAsync).
+__global__ void zero_u32(uint32_t *p, int n) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = 0u;
+}
+
+hipError_t launch_fill(const FieldDesc &d, const FillSpec &f, void *dst, int fmt, int e, int pass,
+                       uint32_t *maxbits, hipStream_t st) {
+  hipLaunchKernelGGL(fill_kernel, dim3(kReduceBlocks), dim3(256), 0, st, d, f, dst, fmt, e, pass,
+                     maxbits);
+  return hipGetLastError();
+}
+
+hipError_t launch_check(const FieldDesc &d, const void *v, int fmt, int e, float expected,
+                        unsigned long long *errors, hipStream_t st) {
+  hipLaunchKernelGGL(check_kernel, dim3(kReduceBlocks), dim3(256), 0, st, d, v, fmt, e, expected,
+                     errors);
+  return hipGetLastError();
+}
+
+hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, double *partial,
+                        uint32_t *maxbits, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(norms_kernel, dim3(kReduceBlocks), dim3(256), 0, st, d, x, fmt, e, partial,
+                     maxbits);
+  hipLaunchKernelGGL(norms_finish, dim3(1), dim3(64), 0, st, (const double *)partial,
+                     kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(zero_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace sth

@@ -1,0 +1,301 @@
+// hstencil3d_hip -- the benchmark driver of stencil3d on the GPU.
+//
+// Keeps the reference driver's command line, phases and report
+// (main.cpp:94-332, aux.cpp:8-81) and calls libstencil_hip through its C ABI
+// in place of ITERATE / init_arrays / check:
+//   init_arrays(ov,os,ou,..)     main.cpp:289   -> sth_init_const(v=0, s=1, u=1)
+//   ITERATE(..., 1U, ...)        main.cpp:297   -> sth_iterate(ctx, 1)   (warm-up)
+//   check(v,grid,1U)             main.cpp:302   -> sth_check(ctx, 1)
+//   ITERATE(..., maxiter, ...)   main.cpp:307   -> sth_iterate(ctx, maxiter), timed
+//   check(v,grid,maxiter+1)      main.cpp:312   -> sth_check(ctx, maxiter+1)
+// Added options select what the reference fixes at compile time
+// (--stencil = STENCIL_SIZE, config.h:37-39) and the forms the reference does
+// not have (--mode leapfrog, --dtype fp16/bf16, --tune).
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include <vector>
+
+#include "stencil_hip.h"
+
+namespace {
+
+double seconds() {  // aux.cpp:8-13
+  struct timeval tv;
+  gettimeofday(&tv, nullptr);
+  return (double)tv.tv_sec + 1e-6 * tv.tv_usec;
+}
+
+// aux.cpp:15-43: value range and modulo checks, exit(2) on violation.
+int get_int_arg(char *argv[], int iarg, int argc, const char *opt, unsigned &val,
+                unsigned minval, unsigned maxval, unsigned modval) {
+  if (strcmp(argv[iarg], opt) != 0) return 0;
+  if (iarg + 1 >= argc) {
+    fprintf(stderr, "%s requires a numeric argument [%u - %u]\n", opt, minval, maxval);
+    exit(2);
+  }
+  val = (unsigned)atoi(argv[iarg + 1]);
+  if (val < minval) {
+    fprintf(stderr, "%s must be larger or equal to %u\n", opt, minval);
+    exit(2);
+  }
+  if (val > maxval) {
+    fprintf(stderr, "%s must be smaller or equal to %u\n", opt, maxval);
+    exit(2);
+  }
+  if (val % modval != 0) {
+    fprintf(stderr, "%s must be a multiple of %u\n", opt, modval);
+    exit(2);
+  }
+  printf("%s set to %u\n", opt, val);
+  return 2;
+}
+
+// aux.cpp:45-69
+int get_real_arg(char *argv[], int iarg, int argc, const char *opt, double &val, double minval,
+                 double maxval) {
+  if (strcmp(argv[iarg], opt) != 0) return 0;
+  if (iarg + 1 >= argc) {
+    fprintf(stderr, "%s requires a float argument [%f - %f]\n", opt, minval, maxval);
+    exit(2);
+  }
+  val = atof(argv[iarg + 1]);
+  if (val < minval) {
+    fprintf(stderr, "%s must be larger or equal to %f\n", opt, minval);
+    exit(2);
+  }
+  if (val > maxval) {
+    fprintf(stderr, "%s must be smaller or equal to %f\n", opt, maxval);
+    exit(2);
+  }
+  printf("%s set to %f\n", opt, val);
+  return 2;
+}
+
+// aux.cpp:72-81
+int get_flag(char *argv[], int iarg, const char *opt, unsigned &flag) {
+  if (strcmp(argv[iarg], opt) != 0) return 0;
+  flag = 1;
+  printf("%s set to ENABLED\n", opt);
+  return 1;
+}
+
+int get_choice(char *argv[], int iarg, int argc, const char *opt, const char *const *names,
+               int n, int &val) {
+  if (strcmp(argv[iarg], opt) != 0) return 0;
+  if (iarg + 1 < argc)
+    for (int q = 0; q < n; q++)
+      if (strcmp(argv[iarg + 1], names[q]) == 0) {
+        val = q;
+        printf("%s set to %s\n", opt, names[q]);
+        return 2;
+      }
+  fprintf(stderr, "%s requires one of:", opt);
+  for (int q = 0; q < n; q++) fprintf(stderr, " %s", names[q]);
+  fprintf(stderr, "\n");
+  exit(2);
+}
+
+void die(const char *what, int st) {
+  fprintf(stderr, "ERROR: %s: %s\n", what, sth_strerror(st));
+  exit(st == STH_ERR_ARG ? 2 : 3);
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  // defaults of main.cpp:96-113 / config.h:4-17
+  unsigned train = 0, offset = 16, maxiter = 100, cacheiter = 1, dups = 1, prefetch = 64;
+  unsigned xs = 945, ys = 945, zs = 880, xh = 4, yh = 4, zh = 16;
+  unsigned xb = 1, yb = 1, zb = 16;
+  double sbm = 12.0;
+  unsigned npts = 7, seed = 0, tune = 0, rows = 0, waves = 0, chunks = 0;
+  int mode = 0, dtype = 0, init = -1;
+  double c1 = 0.25, c2 = 1e-6, c0 = -1e30;
+  static const char *const kModes[] = {"ref", "leapfrog"};
+  static const char *const kTypes[] = {"fp32", "fp16", "bf16"};
+  static const char *const kInits[] = {"const", "uniform", "gauss"};
+
+  int iarg = 1;
+  while (iarg < argc) {
+    int args = 0;
+    args += get_int_arg(argv, iarg, argc, "--iter", maxiter, 1, 100000, 1);
+    args += get_int_arg(argv, iarg, argc, "--citer", cacheiter, 1, 100000, 1);
+    args += get_int_arg(argv, iarg, argc, "--dups", dups, 1, 64, 1);
+    args += get_int_arg(argv, iarg, argc, "--ofs", offset, 0, 16384, 1);
+    args += get_int_arg(argv, iarg, argc, "--pref", prefetch, 1, 512, 1);
+    args += get_int_arg(argv, iarg, argc, "--xh", xh, 4, 16, 1);
+    args += get_int_arg(argv, iarg, argc, "--yh", yh, 4, 16, 1);
+    args += get_int_arg(argv, iarg, argc, "--zh", zh, 1, 16, 1);
+    args += get_int_arg(argv, iarg, argc, "--xs", xs, 1, 1264, 1);
+    args += get_int_arg(argv, iarg, argc, "--ys", ys, 1, 1264, 1);
+    args += get_int_arg(argv, iarg, argc, "--zs", zs, 1, 1264, 1);
+    args += get_int_arg(argv, iarg, argc, "--xb", xb, 1, 1264, 1);
+    args += get_int_arg(argv, iarg, argc, "--yb", yb, 1, 1264, 1);
+    args += get_int_arg(argv, iarg, argc, "--zb", zb, 1, 1264, 1);
+    args += get_real_arg(argv, iarg, argc, "--sbm", sbm, 1.0, 10.0);
+    args += get_flag(argv, iarg, "--train", train);
+    // GPU build options
+    args += get_int_arg(argv, iarg, argc, "--stencil", npts, 7, 25, 1);
+    args += get_choice(argv, iarg, argc, "--mode", kModes, 2, mode);
+    args += get_choice(argv, iarg, argc, "--dtype", kTypes, 3, dtype);
+    args += get_choice(argv, iarg, argc, "--init", kInits, 3, init);
+    args += get_int_arg(argv, iarg, argc, "--seed", seed, 0, 1u << 30, 1);
+    args += get_real_arg(argv, iarg, argc, "--c1", c1, 0.0, 4.0);
+    args += get_real_arg(argv, iarg, argc, "--c2", c2, -1e6, 1e6);
+    args += get_real_arg(argv, iarg, argc, "--c0", c0, -1e6, 1e6);
+    args += get_flag(argv, iarg, "--tune", tune);
+    args += get_int_arg(argv, iarg, argc, "--rows", rows, 1, 8, 1);
+    args += get_int_arg(argv, iarg, argc, "--waves", waves, 1, 8, 1);
+    args += get_int_arg(argv, iarg, argc, "--chunks", chunks, 1, 64, 1);
+    if (args > 0) {
+      iarg += args;
+    } else {
+      fprintf(stderr, "unsupported argument %s\n", argv[iarg]);
+      return 2;
+    }
+  }
+  if (npts != 7 && npts != 13 && npts != 25) {
+    fprintf(stderr, "--stencil must be 7, 13 or 25\n");
+    return 2;
+  }
+  if ((ys % yb) != 0) {  // main.cpp:171-174
+    fprintf(stderr, "ERROR: --ys must by an exact multiple of --yb\n");
+    exit(2);
+  }
+  if (mode == 0 && dtype != 0) {
+    fprintf(stderr, "ERROR: --mode ref runs FP32 storage only\n");
+    exit(2);
+  }
+  if (init < 0) init = mode == 0 ? 0 : 1;
+  const unsigned R = npts == 7 ? 1 : npts == 13 ? 2 : 4;
+  if (xh < R || yh < R || zh < R) {
+    fprintf(stderr, "ERROR: halos must be at least the stencil radius %u\n", R);
+    exit(2);
+  }
+  // --yb maps onto the tile height (rows per wave x waves) when it is one of
+  // the compiled tiles; --zb is fixed at one wave (64 k) per row on the GPU.
+  if (!rows && !waves && yb > 1) {
+    static const unsigned cand[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {4, 8}, {8, 8}};
+    for (auto &c : cand)
+      if (c[0] * c[1] == yb) {
+        rows = c[0];
+        waves = c[1];
+      }
+  }
+
+  // the report's constants (config.h:73-91)
+  const double flops_pt = npts == 7 ? 12 : npts == 13 ? 29 : 41;
+  const double cache_pt = npts == 7 ? 40 : npts == 13 ? 64 : 112;
+  const double eb = dtype == 0 ? 4.0 : 2.0;
+  const double mem_pt = 4 * eb;
+  const double gvol = (double)xs * ys * zs;
+  const double flops = flops_pt * 1.0e-9 * gvol * maxiter * cacheiter;
+  const double mem_bytes = mem_pt * 1.0e-9 * gvol * maxiter * cacheiter;
+  const double cache_bytes = cache_pt * 1.0e-9 * gvol * maxiter * cacheiter;
+  int phase = 0;
+
+  printf("running STENCIL(%u) benchmark: \n", npts);
+  sth_geom g = {xs, ys, zs, npts, (uint32_t)dtype};
+  sth_layout_info L;
+  int st = sth_layout(&g, &L);
+  if (st) die("layout", st);
+  printf(" -- using a grid size of %ux%ux%u with a halo of %ux%ux%u elements totaling %f GB\n", xs,
+         ys, zs, R, R, R, 1e-9 * (double)L.bytes);
+  printf(" -- device layout: row pitch %llu, plane %llu elements, %s storage, 256-B aligned rows\n",
+         (unsigned long long)L.pitch, (unsigned long long)L.plane, kTypes[dtype]);
+  printf(" -- scheduling: 2.5D sweep along x, one workgroup per (y,z) tile, HIP on gfx950\n");
+
+  sth_ctx *ctx = nullptr;
+  if ((st = sth_create(&g, &ctx))) die("sth_create", st);
+  sth_tuning tn = {rows, waves, chunks, 1};
+  if ((st = sth_set_tuning(ctx, &tn))) die("tuning", st);
+
+  printf("[rank0] Phase #%i: array initialization\n", ++phase);
+  double t0 = seconds();
+  if (init == 0) {
+    st = sth_init_const(ctx, 0.0f, 1.0f, 1.0f);  // init_arrays, iterators.cpp:292-301
+  } else if (init == 1) {
+    st = sth_init_uniform(ctx, STH_FIELD_U, seed, 0, -1.0, 1.0, 0.0f);
+    if (!st) st = sth_init_uniform(ctx, STH_FIELD_S, seed, 1, -1.0, 1.0, 0.0f);
+    if (!st) st = sth_init_uniform(ctx, STH_FIELD_V, seed, 2, -1.0, 1.0, 0.0f);
+  } else {
+    st = sth_init_gaussian(ctx, STH_FIELD_U, 1.0, 8.0, 1e-3, seed, 0);
+    if (!st) st = sth_init_gaussian(ctx, STH_FIELD_V, 1.0, 8.0, 1e-3, seed, 1);
+    if (!st) st = sth_init_point(ctx, STH_FIELD_S, 0, 0, 0, 0.0f);
+  }
+  if (!st) st = sth_sync(ctx);
+  if (st) die("initialization", st);
+  double t1 = seconds();
+  printf("[rank0] Phase #%i: initialization took %f secs\n", ++phase, t1 - t0);
+
+  if (tune) {
+    sth_tuning best;
+    float ms = 0.0f;
+    if ((st = sth_autotune(ctx, mode, 3, &best, &ms))) die("autotune", st);
+    printf(" -- autotune: rows/wave %u, waves %u, chunks %u: %.3f ms/sweep\n", best.rows_per_wave,
+           best.waves, best.chunks, ms);
+    // the tuning sweeps modified the fields: initialise again
+    if (init == 0) st = sth_init_const(ctx, 0.0f, 1.0f, 1.0f);
+    if (st) die("re-initialization", st);
+  }
+
+  const float fc1 = (float)c1;
+  const float fc0 = c0 > -1e29 ? (float)c0 : (float)(2.0 - 6.0 * c1);
+  std::vector<float> c2v(maxiter + 1, (float)c2);
+  auto sweep = [&](unsigned n) {
+    return mode == 0 ? sth_iterate(ctx, n) : sth_leapfrog(ctx, n, fc0, fc1, c2v.data());
+  };
+  auto report_check = [&](unsigned expected) {
+    if (mode != 0) return;
+    uint64_t errors = 0;
+    int s2 = sth_check(ctx, (float)expected, &errors);
+    if (s2) die("check", s2);
+    printf("check: %llu errors found\n", (unsigned long long)errors);
+  };
+
+  printf("[rank0] Phase #%i: initial warmup\n", ++phase);
+  t0 = seconds();
+  if ((st = sweep(1)) || (st = sth_sync(ctx))) die("warmup", st);
+  t1 = seconds();
+  printf("[rank0] Phase #%i: finish warmup in %f secs\n", ++phase, t1 - t0);
+  report_check(1);
+
+  printf("[rank0] Phase #%i: start compute\n", ++phase);
+  if ((st = sth_timer_start(ctx))) die("timer", st);
+  t0 = seconds();
+  if ((st = sweep(maxiter)) || (st = sth_sync(ctx))) die("compute", st);
+  t1 = seconds();
+  double kms = 0.0;
+  uint64_t launches = 0;
+  if ((st = sth_timer_read(ctx, &kms, &launches))) die("timer", st);
+  // main.cpp:309-310
+  printf("[rank0] Phase #%i: finish compute in %f secs (%f %s, mem: %f GB/s, cache: %f GB/s)\n",
+         ++phase, t1 - t0, flops / (t1 - t0), "GFlop/s", mem_bytes / (t1 - t0) / cacheiter,
+         cache_bytes / (t1 - t0));
+  const double per = kms / (double)(launches ? launches : 1);
+  const double gpts = gvol / (per * 1e-3) * 1e-9;
+  printf(" -- GPU: %.4f ms/sweep (HIP events), %.2f Gpts/s, %.1f GB/s algorithmic HBM "
+         "(%.1f%% of 8 TB/s)\n",
+         per, gpts, gpts * mem_pt, 100.0 * gpts * mem_pt / 8000.0);
+  report_check(maxiter + 1);
+  if (mode != 0) {
+    double mx = 0, l2 = 0;
+    if ((st = sth_norms(ctx, STH_FIELD_U, &mx, &l2))) die("norms", st);
+    printf("norms: max |u| = %.9e, ||u||_2 = %.9e\n", mx, l2);
+  }
+  printf("freeing data arrays\n");
+  sth_destroy(ctx);
+  printf("[rank0] Phase #%i: done!\n", ++phase);
+  (void)train;
+  (void)offset;
+  (void)dups;
+  (void)prefetch;
+  (void)xb;
+  (void)zb;
+  (void)sbm;
+  return 0;
+}

@@ -1,0 +1,754 @@
+// sth_api.hip -- the extern "C" boundary of libstencil_hip (include/stencil_hip.h).
+//
+// Host side of the sweep: context (stream + hipMalloc'd fields), host<->device
+// layout translation, initialisers, the per-step launch loop, check/norms,
+// tuning and timing.  No call exits the process; every failure is a status.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "fields.h"
+#include "stencil_hip.h"
+#include "sth_common.h"
+#include "sweep_dispatch.h"
+
+using namespace sth;
+
+struct sth_ctx {
+  sth_geom g;
+  sth_layout_info L;
+  int R = 0, eb = 0, fmt = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void *base[3] = {nullptr, nullptr, nullptr};  // buffers a, b, s
+  int ubuf = 0;                                 // buffer holding logical u
+  // 16-bit storage scale state
+  Q16Slot *qs_dev = nullptr;
+  Q16Slot host_slot{};
+  bool host_dirty = true;
+  int slot = 0;
+  bool halo_zero[3] = {true, true, true};
+  // scratch for reductions
+  uint32_t *maxbits_dev = nullptr;
+  unsigned long long *errors_dev = nullptr;
+  double *partial_dev = nullptr;
+  double *sum_dev = nullptr;
+  // pinned staging for host transfers
+  void *staging = nullptr;
+  size_t staging_bytes = 0;
+  sth_tuning tune{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timing = false;
+  uint64_t launches = 0;
+};
+
+namespace {
+
+inline int hip_status(hipError_t e) {
+  if (e == hipSuccess) return STH_OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return STH_ERR_NOMEM;
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return STH_ERR_NODEV;
+  return STH_ERR_HIP;
+}
+
+#define STH_TRY(expr)                            \
+  do {                                           \
+    hipError_t e_ = (expr);                      \
+    if (e_ != hipSuccess) return hip_status(e_); \
+  } while (0)
+
+inline int buf_of(const sth_ctx *c, sth_field f) {
+  return f == STH_FIELD_U ? c->ubuf : f == STH_FIELD_V ? 1 - c->ubuf : 2;
+}
+
+inline FieldDesc desc_of(const sth_ctx *c) {
+  FieldDesc d;
+  d.pitch = (int64_t)c->L.pitch;
+  d.plane = (int64_t)c->L.plane;
+  d.align = kRowAlignBytes / c->eb;
+  d.elems = c->L.elems;
+  d.xs = (int)c->g.xs;
+  d.ys = (int)c->g.ys;
+  d.zs = (int)c->g.zs;
+  d.R = c->R;
+  return d;
+}
+
+// Element offset of the first element of plane i (cell (i, -R, -R)).
+inline uint64_t plane_start(const sth_ctx *c, int64_t i) {
+  return c->L.origin - c->R - c->R * c->L.pitch + (uint64_t)(i * (int64_t)c->L.plane);
+}
+
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline uint64_t field_key(uint64_t seed, int field) {
+  return mix64(seed + (uint64_t)(field + 1) * 0x9E3779B97F4A7C15ull);
+}
+
+inline uint32_t f2u(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+inline float u2f(uint32_t u) {
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+// Host encoders/decoders of the 16-bit storage formats (round to nearest even).
+inline uint16_t enc16(int fmt, float x, int e) {
+  const float y = ldexpf(x, -e);
+  if (fmt == FMT_F16) {
+    _Float16 h = (_Float16)y;
+    uint16_t b;
+    memcpy(&b, &h, 2);
+    return b;
+  }
+  uint32_t b = f2u(y);
+  if ((b & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((b >> 16) | 0x40u);
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+inline float dec16(int fmt, uint16_t q, int e) {
+  if (fmt == FMT_F16) {
+    _Float16 h;
+    memcpy(&h, &q, 2);
+    return ldexpf((float)h, e);
+  }
+  return ldexpf(u2f((uint32_t)q << 16), e);
+}
+
+int ensure_staging(sth_ctx *c) {
+  if (c->staging) return STH_OK;
+  const size_t want = std::max<size_t>((size_t)64 << 20, c->L.plane * c->eb);
+  STH_TRY(hipHostMalloc(&c->staging, want, hipHostMallocDefault));
+  c->staging_bytes = want;
+  return STH_OK;
+}
+
+int check_host_layout(const sth_ctx *c, const sth_host_layout *hl) {
+  if (!hl) return STH_ERR_ARG;
+  if ((int)hl->xh < c->R || (int)hl->yh < c->R || (int)hl->zh < c->R) return STH_ERR_ARG;
+  return STH_OK;
+}
+
+struct HostIdx {
+  int64_t jm, im, ofs;
+};
+inline HostIdx host_idx(const sth_ctx *c, const sth_host_layout *hl) {
+  HostIdx h;
+  h.jm = (int64_t)c->g.zs + 2 * hl->zh;
+  h.im = h.jm * ((int64_t)c->g.ys + 2 * hl->yh);
+  h.ofs = (int64_t)hl->zh + (int64_t)hl->yh * h.jm + (int64_t)hl->xh * h.im;
+  return h;
+}
+
+// Moves planes [-R, xs+R) between a host dense array and a device buffer via
+// the pinned staging buffer.  `row` is called per (i, j) cell row with the
+// host row pointer offset (k = -R) and the staging row (k = -R).
+template <class RowFn>
+int transfer_planes(sth_ctx *c, int buf, bool to_device, RowFn row) {
+  int st = ensure_staging(c);
+  if (st) return st;
+  const int64_t R = c->R, xs = c->g.xs, ys = c->g.ys;
+  const size_t plane_bytes = c->L.plane * c->eb;
+  const int64_t per = std::max<int64_t>(1, (int64_t)(c->staging_bytes / plane_bytes));
+  char *dev = (char *)c->base[buf];
+  for (int64_t ia = -R; ia < xs + R; ia += per) {
+    const int64_t ib = std::min<int64_t>(xs + R, ia + per);
+    const size_t bytes = (size_t)(ib - ia) * plane_bytes;
+    char *dptr = dev + plane_start(c, ia) * c->eb;
+    if (!to_device) {
+      STH_TRY(hipMemcpyAsync(c->staging, dptr, bytes, hipMemcpyDeviceToHost, c->stream));
+      STH_TRY(hipStreamSynchronize(c->stream));
+    }
+    for (int64_t i = ia; i < ib; i++)
+      for (int64_t j = -R; j < ys + R; j++) {
+        char *srow = (char *)c->staging +
+                     ((size_t)(i - ia) * c->L.plane + (size_t)(j + R) * c->L.pitch) * c->eb;
+        row(i, j, srow);
+      }
+    if (to_device) {
+      STH_TRY(hipMemcpyAsync(dptr, c->staging, bytes, hipMemcpyHostToDevice, c->stream));
+      STH_TRY(hipStreamSynchronize(c->stream));
+    }
+  }
+  return STH_OK;
+}
+
+// Current 16-bit scale state of the context (from the device after sweeps).
+int fetch_slot(sth_ctx *c) {
+  if (c->fmt == FMT_F32 || c->host_dirty) return STH_OK;
+  STH_TRY(hipMemcpyAsync(&c->host_slot, c->qs_dev + c->slot, sizeof(Q16Slot),
+                         hipMemcpyDeviceToHost, c->stream));
+  STH_TRY(hipStreamSynchronize(c->stream));
+  c->host_dirty = true;  // host copy is now authoritative until the next sweep
+  return STH_OK;
+}
+
+int push_slot(sth_ctx *c) {
+  if (c->fmt == FMT_F32 || !c->host_dirty) return STH_OK;
+  Q16Slot img[2];
+  img[0] = c->host_slot;
+  memset(&img[1], 0, sizeof(Q16Slot));
+  // slot s holds the state, slot s+1 must start with zero maxima
+  STH_TRY(hipMemcpyAsync(c->qs_dev + c->slot, &img[0], sizeof(Q16Slot), hipMemcpyHostToDevice,
+                         c->stream));
+  STH_TRY(hipMemcpyAsync(c->qs_dev + (c->slot + 1) % 3, &img[1], sizeof(Q16Slot),
+                         hipMemcpyHostToDevice, c->stream));
+  STH_TRY(hipStreamSynchronize(c->stream));
+  c->host_dirty = false;
+  return STH_OK;
+}
+
+int device_fill(sth_ctx *c, int buf, const FillSpec &f, bool halo_zero) {
+  const FieldDesc d = desc_of(c);
+  int e = 0;
+  uint32_t mbits = 0;
+  if (c->fmt != FMT_F32) {
+    int st = fetch_slot(c);
+    if (st) return st;
+    STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
+    STH_TRY(launch_fill(d, f, c->base[buf], c->fmt, 0, 0, c->maxbits_dev, c->stream));
+    STH_TRY(hipMemcpyAsync(&mbits, c->maxbits_dev, 4, hipMemcpyDeviceToHost, c->stream));
+    STH_TRY(hipStreamSynchronize(c->stream));
+    e = scale_exp((double)u2f(mbits));
+  }
+  STH_TRY(launch_fill(d, f, c->base[buf], c->fmt, e, 1, c->maxbits_dev, c->stream));
+  if (c->fmt != FMT_F32) {
+    // the buffer's max is that of the stored (rounded) values
+    STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
+    STH_TRY(launch_norms(d, c->base[buf], c->fmt, e, c->partial_dev, c->maxbits_dev, c->sum_dev,
+                         c->stream));
+    STH_TRY(hipMemcpyAsync(&mbits, c->maxbits_dev, 4, hipMemcpyDeviceToHost, c->stream));
+    STH_TRY(hipStreamSynchronize(c->stream));
+    c->host_slot.exp[buf] = e;
+    c->host_slot.maxbits[buf] = mbits;
+  }
+  c->halo_zero[buf] = halo_zero;
+  return STH_OK;
+}
+
+void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
+  Coeffs cf;
+  stencil_coeffs(c->g.npts, cf);
+  int ax[12], rr[12];
+  const int T = term_order(c->g.npts, ax, rr);
+  for (int t = 0; t < 12; t++) a.w[t] = 0.0f;
+  for (int t = 0; t < T; t++) {
+    const float w = cf.w[ax[t]][rr[t] - 1];
+    a.w[t] = leap ? 2.0f * w : w;
+  }
+  a.coef0 = cf.coef0;
+  a.mA = -cf.A;
+  a.gain = lap_gain(c->g.npts);
+}
+
+struct Plan {
+  int mj, nw, chunks, chunk_len, ntk, ntj;
+  unsigned grid;
+};
+
+Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
+  Plan p;
+  p.mj = t.rows_per_wave ? (int)t.rows_per_wave : 4;
+  p.nw = t.waves ? (int)t.waves : 4;
+  const int TJ = p.mj * p.nw;
+  p.ntk = ((int)c->g.zs + 63) / 64;
+  p.ntj = ((int)c->g.ys + TJ - 1) / TJ;
+  const int xs = (int)c->g.xs;
+  int ch = (int)t.chunks;
+  if (ch == 0) {
+    // enough workgroups to cover the 256 CUs several times, but chunks no
+    // shorter than 4R planes (each chunk re-reads 2R planes of u)
+    const int tiles = p.ntk * p.ntj;
+    ch = (2048 + tiles - 1) / tiles;
+    ch = std::min(ch, std::max(1, xs / std::max(8, 4 * c->R)));
+    ch = std::max(ch, 1);
+  }
+  ch = std::min(ch, xs);
+  p.chunk_len = (xs + ch - 1) / ch;
+  p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
+  p.grid = (unsigned)(p.ntk * p.ntj * p.chunks);
+  return p;
+}
+
+void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a) {
+  const size_t o = c->L.origin * c->eb;
+  const int vb = 1 - c->ubuf;
+  a.u = (const char *)c->base[c->ubuf] + o;
+  a.v = (char *)c->base[vb] + o;
+  a.s = (const char *)c->base[2] + o;
+  a.pitch = (int64_t)c->L.pitch;
+  a.plane = (int64_t)c->L.plane;
+  a.xs = (int)c->g.xs;
+  a.ys = (int)c->g.ys;
+  a.zs = (int)c->g.zs;
+  a.ntk = p.ntk;
+  a.ntj = p.ntj;
+  a.chunks = p.chunks;
+  a.chunk_len = p.chunk_len;
+  a.remap = c->tune.xcd_remap ? 1 : 0;
+  a.qs = c->qs_dev;
+  a.slot = c->slot;
+  a.ubuf = c->ubuf;
+  a.vbuf = vb;
+  a.c0 = a.c1 = a.c2n = 0.0f;
+}
+
+int launch_one(sth_ctx *c, const Plan &p, int mode, const SweepArgs &a) {
+  const hipError_t e = launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.grid, c->stream, a);
+  if (e != hipSuccess) return hip_status(e);
+  if (c->timing) c->launches++;
+  return STH_OK;
+}
+
+int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
+  SweepArgs a;
+  memset(&a, 0, sizeof(a));
+  base_args(c, p, a);
+  fill_weights(c, false, a);
+  for (uint32_t n = 0; n < iterations; n++) {
+    int st = launch_one(c, p, MODE_REF, a);
+    if (st) return st;
+  }
+  return STH_OK;
+}
+
+int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, const float *c2) {
+  int st = push_slot(c);
+  if (st) return st;
+  SweepArgs a;
+  memset(&a, 0, sizeof(a));
+  fill_weights(c, true, a);
+  for (uint32_t n = 0; n < steps; n++) {
+    base_args(c, p, a);
+    a.c0 = c0;
+    a.c1 = c1;
+    a.c2n = c2 ? c2[n] : 0.0f;
+    st = launch_one(c, p, MODE_LEAP, a);
+    if (st) return st;
+    c->ubuf = 1 - c->ubuf;
+    c->slot = (c->slot + 1) % 3;
+  }
+  return STH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sth_strerror(int status) {
+  switch (status) {
+    case STH_OK: return "ok";
+    case STH_ERR_ARG: return "invalid argument";
+    case STH_ERR_HIP: return "HIP runtime error";
+    case STH_ERR_NOMEM: return "device out of memory";
+    case STH_ERR_STATE: return "invalid state for this call";
+    case STH_ERR_NODEV: return "no usable GPU";
+    case STH_ERR_UNSUPPORTED: return "unsupported combination";
+    default: return "unknown status";
+  }
+}
+
+int sth_version(void) { return STH_VERSION; }
+
+int sth_layout(const sth_geom *g, sth_layout_info *out) {
+  if (!g || !out) return STH_ERR_ARG;
+  return compute_layout(*g, *out);
+}
+
+int sth_host_elems(const sth_geom *g, const sth_host_layout *hl, uint64_t *elems) {
+  if (!g || !hl || !elems) return STH_ERR_ARG;
+  *elems = ((uint64_t)g->xs + 2 * hl->xh) * ((uint64_t)g->ys + 2 * hl->yh) *
+           ((uint64_t)g->zs + 2 * hl->zh);
+  return STH_OK;
+}
+
+int sth_device_count(int *n) {
+  if (!n) return STH_ERR_ARG;
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  return c > 0 ? STH_OK : STH_ERR_NODEV;
+}
+
+int sth_destroy(sth_ctx *c) {
+  if (!c) return STH_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (int b = 0; b < 3; b++)
+    if (c->base[b]) (void)hipFree(c->base[b]);
+  if (c->qs_dev) (void)hipFree(c->qs_dev);
+  if (c->maxbits_dev) (void)hipFree(c->maxbits_dev);
+  if (c->errors_dev) (void)hipFree(c->errors_dev);
+  if (c->partial_dev) (void)hipFree(c->partial_dev);
+  if (c->sum_dev) (void)hipFree(c->sum_dev);
+  if (c->staging) (void)hipHostFree(c->staging);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return STH_OK;
+}
+
+int sth_create(const sth_geom *g, sth_ctx **out) {
+  if (!g || !out) return STH_ERR_ARG;
+  *out = nullptr;
+  sth_layout_info L;
+  int st = compute_layout(*g, L);
+  if (st) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return STH_ERR_NODEV;
+  sth_ctx *c = new (std::nothrow) sth_ctx();
+  if (!c) return STH_ERR_NOMEM;
+  c->g = *g;
+  c->L = L;
+  c->R = (int)L.radius;
+  c->eb = (int)L.elem_bytes;
+  c->fmt = g->dtype == STH_FP32 ? FMT_F32 : g->dtype == STH_FP16 ? FMT_F16 : FMT_BF16;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  for (int b = 0; b < 3 && e == hipSuccess; b++) e = hipMalloc(&c->base[b], L.bytes);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->qs_dev, 3 * sizeof(Q16Slot));
+  if (e == hipSuccess) e = hipMalloc((void **)&c->maxbits_dev, 256);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->errors_dev, 256);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->partial_dev, kReduceBlocks * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&c->sum_dev, 256);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipMemsetAsync(c->qs_dev, 0, 3 * sizeof(Q16Slot), c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    const int rc = hip_status(e);
+    sth_destroy(c);
+    return rc;
+  }
+  c->tune.xcd_remap = 1;
+  *out = c;
+  return STH_OK;
+}
+
+int sth_get_layout(const sth_ctx *c, sth_layout_info *out) {
+  if (!c || !out) return STH_ERR_ARG;
+  *out = c->L;
+  return STH_OK;
+}
+
+int sth_init_const(sth_ctx *c, float v0, float s0, float u0) {
+  if (!c) return STH_ERR_ARG;
+  if (c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  const float vals[3] = {v0, s0, u0};
+  const sth_field fs[3] = {STH_FIELD_V, STH_FIELD_S, STH_FIELD_U};
+  for (int q = 0; q < 3; q++) {
+    FillSpec f;
+    memset(&f, 0, sizeof(f));
+    f.kind = FILL_CONST;
+    f.a = vals[q];
+    int st = device_fill(c, buf_of(c, fs[q]), f, vals[q] == 0.0f);
+    if (st) return st;
+  }
+  return STH_OK;
+}
+
+int sth_init_uniform(sth_ctx *c, sth_field fld, uint64_t seed, int field_id, double lo, double hi,
+                     float halo_value) {
+  if (!c || fld < 0 || fld > 2) return STH_ERR_ARG;
+  FillSpec f;
+  memset(&f, 0, sizeof(f));
+  f.kind = FILL_UNIFORM;
+  f.lo = lo;
+  f.hi = hi;
+  f.halo = halo_value;
+  f.key = field_key(seed, field_id);
+  return device_fill(c, buf_of(c, fld), f, halo_value == 0.0f);
+}
+
+int sth_init_gaussian(sth_ctx *c, sth_field fld, double amp, double sigma, double noise,
+                      uint64_t seed, int field_id) {
+  if (!c || fld < 0 || fld > 2 || !(sigma > 0.0)) return STH_ERR_ARG;
+  FillSpec f;
+  memset(&f, 0, sizeof(f));
+  f.kind = FILL_GAUSS;
+  f.lo = amp;
+  f.hi = sigma;
+  f.noise = noise;
+  f.key = field_key(seed, field_id);
+  return device_fill(c, buf_of(c, fld), f, true);
+}
+
+int sth_init_point(sth_ctx *c, sth_field fld, uint32_t i, uint32_t j, uint32_t k, float value) {
+  if (!c || fld < 0 || fld > 2) return STH_ERR_ARG;
+  if (i >= c->g.xs || j >= c->g.ys || k >= c->g.zs) return STH_ERR_ARG;
+  FillSpec f;
+  memset(&f, 0, sizeof(f));
+  f.kind = FILL_POINT;
+  f.a = value;
+  f.pi = (int)i;
+  f.pj = (int)j;
+  f.pk = (int)k;
+  return device_fill(c, buf_of(c, fld), f, true);
+}
+
+int sth_upload(sth_ctx *c, sth_field fld, const float *host, const sth_host_layout *hl) {
+  if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
+  int st = check_host_layout(c, hl);
+  if (st) return st;
+  const HostIdx h = host_idx(c, hl);
+  const int64_t R = c->R, zs = c->g.zs, xs = c->g.xs, ys = c->g.ys;
+  const int buf = buf_of(c, fld);
+  // halo-zero flag and (16-bit) the max over the cells the sweep reads
+  bool hz = true;
+  float mx = 0.0f;
+  for (int64_t i = -R; i < xs + R; i++)
+    for (int64_t j = -R; j < ys + R; j++) {
+      const float *src = host + h.ofs + i * h.im + j * h.jm - R;
+      const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
+      for (int64_t k = 0; k < zs + 2 * R; k++) {
+        const float x = src[k];
+        mx = std::max(mx, fabsf(x));
+        if (x != 0.0f && (!inrow || k < R || k >= zs + R)) hz = false;
+      }
+    }
+  int e = 0;
+  if (c->fmt != FMT_F32) {
+    st = fetch_slot(c);
+    if (st) return st;
+    e = scale_exp((double)mx);
+  }
+  float qmax = 0.0f;
+  st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow) {
+    const float *src = host + h.ofs + i * h.im + j * h.jm - R;
+    if (c->fmt == FMT_F32) {
+      memcpy(srow, src, (size_t)(zs + 2 * R) * 4);
+    } else {
+      uint16_t *d = (uint16_t *)srow;
+      for (int64_t k = 0; k < zs + 2 * R; k++) {
+        d[k] = enc16(c->fmt, src[k], e);
+        qmax = std::max(qmax, fabsf(dec16(c->fmt, d[k], e)));
+      }
+    }
+  });
+  if (st) return st;
+  if (c->fmt != FMT_F32) {
+    c->host_slot.exp[buf] = e;
+    c->host_slot.maxbits[buf] = f2u(qmax);
+  }
+  c->halo_zero[buf] = hz;
+  return STH_OK;
+}
+
+int sth_download(sth_ctx *c, sth_field fld, float *host, const sth_host_layout *hl) {
+  if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
+  int st = check_host_layout(c, hl);
+  if (st) return st;
+  st = fetch_slot(c);
+  if (st) return st;
+  const HostIdx h = host_idx(c, hl);
+  const int64_t R = c->R, zs = c->g.zs;
+  const int buf = buf_of(c, fld);
+  const int e = c->fmt == FMT_F32 ? 0 : c->host_slot.exp[buf];
+  return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow) {
+    float *dst = host + h.ofs + i * h.im + j * h.jm - R;
+    if (c->fmt == FMT_F32) {
+      memcpy(dst, srow, (size_t)(zs + 2 * R) * 4);
+    } else {
+      const uint16_t *s = (const uint16_t *)srow;
+      for (int64_t k = 0; k < zs + 2 * R; k++) dst[k] = dec16(c->fmt, s[k], e);
+    }
+  });
+}
+
+int sth_upload_raw16(sth_ctx *c, sth_field fld, const uint16_t *host, int exp,
+                     const sth_host_layout *hl) {
+  if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
+  if (c->fmt == FMT_F32) return STH_ERR_UNSUPPORTED;
+  int st = check_host_layout(c, hl);
+  if (st) return st;
+  st = fetch_slot(c);
+  if (st) return st;
+  const HostIdx h = host_idx(c, hl);
+  const int64_t R = c->R, zs = c->g.zs, xs = c->g.xs, ys = c->g.ys;
+  const int buf = buf_of(c, fld);
+  float qmax = 0.0f;
+  bool hz = true;
+  st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow) {
+    const uint16_t *src = host + h.ofs + i * h.im + j * h.jm - R;
+    memcpy(srow, src, (size_t)(zs + 2 * R) * 2);
+    const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
+    for (int64_t k = 0; k < zs + 2 * R; k++) {
+      const float x = dec16(c->fmt, src[k], exp);
+      qmax = std::max(qmax, fabsf(x));
+      if (x != 0.0f && (!inrow || k < R || k >= zs + R)) hz = false;
+    }
+  });
+  if (st) return st;
+  c->host_slot.exp[buf] = exp;
+  c->host_slot.maxbits[buf] = f2u(qmax);
+  c->halo_zero[buf] = hz;
+  return STH_OK;
+}
+
+int sth_download_raw16(sth_ctx *c, sth_field fld, uint16_t *host, int *exp, float *maxabs,
+                       const sth_host_layout *hl) {
+  if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
+  if (c->fmt == FMT_F32) return STH_ERR_UNSUPPORTED;
+  int st = check_host_layout(c, hl);
+  if (st) return st;
+  st = fetch_slot(c);
+  if (st) return st;
+  const HostIdx h = host_idx(c, hl);
+  const int64_t R = c->R, zs = c->g.zs;
+  const int buf = buf_of(c, fld);
+  if (exp) *exp = c->host_slot.exp[buf];
+  if (maxabs) *maxabs = u2f(c->host_slot.maxbits[buf]);
+  return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow) {
+    memcpy(host + h.ofs + i * h.im + j * h.jm - R, srow, (size_t)(zs + 2 * R) * 2);
+  });
+}
+
+int sth_iterate(sth_ctx *c, uint32_t iterations) {
+  if (!c) return STH_ERR_ARG;
+  if (c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  return run_ref(c, make_plan(c, c->tune), iterations);
+}
+
+int sth_leapfrog(sth_ctx *c, uint32_t steps, float c0, float c1, const float *c2) {
+  if (!c) return STH_ERR_ARG;
+  if (c->fmt != FMT_F32 && !(c->halo_zero[0] && c->halo_zero[1])) return STH_ERR_UNSUPPORTED;
+  return run_leap(c, make_plan(c, c->tune), steps, c0, c1, c2);
+}
+
+int sth_sync(sth_ctx *c) {
+  if (!c) return STH_ERR_ARG;
+  STH_TRY(hipStreamSynchronize(c->stream));
+  return STH_OK;
+}
+
+int sth_check(sth_ctx *c, float expected, uint64_t *errors) {
+  if (!c || !errors) return STH_ERR_ARG;
+  int st = fetch_slot(c);
+  if (st) return st;
+  const int buf = 1 - c->ubuf;
+  const int e = c->fmt == FMT_F32 ? 0 : c->host_slot.exp[buf];
+  unsigned long long cnt = 0;
+  STH_TRY(hipMemsetAsync(c->errors_dev, 0, 8, c->stream));
+  STH_TRY(launch_check(desc_of(c), c->base[buf], c->fmt, e, expected, c->errors_dev, c->stream));
+  STH_TRY(hipMemcpyAsync(&cnt, c->errors_dev, 8, hipMemcpyDeviceToHost, c->stream));
+  STH_TRY(hipStreamSynchronize(c->stream));
+  *errors = cnt;
+  return STH_OK;
+}
+
+int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
+  if (!c || fld < 0 || fld > 2) return STH_ERR_ARG;
+  int st = fetch_slot(c);
+  if (st) return st;
+  const int buf = buf_of(c, fld);
+  const int e = c->fmt == FMT_F32 ? 0 : c->host_slot.exp[buf];
+  uint32_t mb = 0;
+  double ss = 0.0;
+  STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
+  STH_TRY(launch_norms(desc_of(c), c->base[buf], c->fmt, e, c->partial_dev, c->maxbits_dev,
+                       c->sum_dev, c->stream));
+  STH_TRY(hipMemcpyAsync(&mb, c->maxbits_dev, 4, hipMemcpyDeviceToHost, c->stream));
+  STH_TRY(hipMemcpyAsync(&ss, c->sum_dev, 8, hipMemcpyDeviceToHost, c->stream));
+  STH_TRY(hipStreamSynchronize(c->stream));
+  if (maxabs) *maxabs = (double)u2f(mb);
+  if (l2) *l2 = sqrt(ss);
+  return STH_OK;
+}
+
+int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
+  if (!c || !t) return STH_ERR_ARG;
+  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : 4;
+  const int nw = t->waves ? (int)t->waves : 4;
+  if (!tile_supported(mj, nw) || t->chunks > 64) return STH_ERR_ARG;
+  c->tune = *t;
+  return STH_OK;
+}
+
+int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
+  if (!c || !t) return STH_ERR_ARG;
+  *t = c->tune;
+  const Plan p = make_plan(c, c->tune);
+  t->rows_per_wave = (uint32_t)p.mj;
+  t->waves = (uint32_t)p.nw;
+  t->chunks = (uint32_t)p.chunks;
+  return STH_OK;
+}
+
+int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, float *best_ms) {
+  if (!c || reps == 0) return STH_ERR_ARG;
+  if (!leapfrog && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  static const int tiles[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {1, 8}, {2, 8},
+                                 {4, 8}, {8, 8}, {2, 2}, {4, 2}, {8, 2}};
+  static const int chunk_opts[] = {0, 1, 2, 4, 8};
+  const sth_tuning saved = c->tune;
+  const bool saved_timing = c->timing;
+  c->timing = false;
+  sth_tuning bt = saved;
+  float bms = 1e30f;
+  for (const auto &tl : tiles) {
+    for (int ch : chunk_opts) {
+      sth_tuning t = saved;
+      t.rows_per_wave = (uint32_t)tl[0];
+      t.waves = (uint32_t)tl[1];
+      t.chunks = (uint32_t)ch;
+      const Plan p = make_plan(c, t);
+      if (ch != 0 && p.chunks != ch) continue;
+      c->tune = t;
+      int st = leapfrog ? run_leap(c, p, 1, 1.0f, 0.0f, nullptr) : run_ref(c, p, 1);
+      if (st) { c->tune = saved; c->timing = saved_timing; return st; }
+      STH_TRY(hipEventRecord(c->ev0, c->stream));
+      st = leapfrog ? run_leap(c, p, reps, 1.0f, 0.0f, nullptr) : run_ref(c, p, reps);
+      if (st) { c->tune = saved; c->timing = saved_timing; return st; }
+      STH_TRY(hipEventRecord(c->ev1, c->stream));
+      STH_TRY(hipEventSynchronize(c->ev1));
+      float ms = 0.0f;
+      STH_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      ms /= (float)reps;
+      if (ms < bms) {
+        bms = ms;
+        bt = t;
+        bt.chunks = (uint32_t)p.chunks;
+      }
+    }
+  }
+  c->tune = bt;
+  c->timing = saved_timing;
+  if (best) *best = bt;
+  if (best_ms) *best_ms = bms;
+  return STH_OK;
+}
+
+int sth_timer_start(sth_ctx *c) {
+  if (!c) return STH_ERR_ARG;
+  STH_TRY(hipEventRecord(c->ev0, c->stream));
+  c->timing = true;
+  c->launches = 0;
+  return STH_OK;
+}
+
+int sth_timer_read(sth_ctx *c, double *kernel_ms, uint64_t *launches) {
+  if (!c || !c->timing) return STH_ERR_STATE;
+  STH_TRY(hipEventRecord(c->ev1, c->stream));
+  STH_TRY(hipEventSynchronize(c->ev1));
+  float ms = 0.0f;
+  STH_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->timing = false;
+  if (kernel_ms) *kernel_ms = (double)ms;
+  if (launches) *launches = c->launches;
+  return STH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,11 @@
+// sweep_7.hip -- the 7-point sweep kernels (one compile unit per stencil size).
+#include "sweep_dispatch.h"
+
+namespace sth {
+
+hipError_t launch_sweep_7(int mode, int fmt, int mj, int nw, unsigned grid, hipStream_t st,
+                          const SweepArgs &a) {
+  return launch_sweep_n<7>(mode, fmt, mj, nw, grid, st, a);
+}
+
+}  // namespace sth

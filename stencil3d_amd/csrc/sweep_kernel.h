@@ -1,0 +1,322 @@
+// sweep_kernel.h -- the 2.5D cross-stencil sweep for gfx950.
+//
+// One workgroup owns a (j,k) tile of TJ x 64 points and streams it along i
+// (the reference's slowest axis x, grid.cpp:20-22), replacing the OpenMP
+// loop nest of iterate_3d (iterators.cpp:137-156) and the CPU cache blocking
+// of iterate_{2d,3d}_jk_blocked / Hilbert / checkerboard (iterators.cpp:
+// 159-289), which exist only to reuse u from cache.
+//
+//  * k (unit stride): one wave covers 64 consecutive k of a row, so every
+//    global access of a wave is one 256-byte (FP32) row segment.
+//  * j: each wave holds MJ consecutive rows in registers; NW waves stack
+//    to TJ = NW*MJ rows.  j-neighbours inside a wave's rows come from
+//    registers, the others from LDS.
+//  * i: a register queue of 2R+1 planes per point; each step loads only the
+//    leading plane i+R of u (and s, v at plane i), one step ahead.
+//  * LDS: the centre plane of the tile plus an R-wide j/k halo, double
+//    buffered, one barrier per plane.
+//
+// Per point the arithmetic follows the reference order exactly (REF) or the
+// documented LEAPFROG order; see sweep_point_* below.  Compiled with
+// -ffp-contract=off: every product that is not an explicit fmaf is rounded.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sth_common.h"
+
+namespace sth {
+
+enum { MODE_REF = 0, MODE_LEAP = 1 };
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
+
+// Per-step scale state of the 16-bit storage form (a ring of 3 slots: step n
+// reads slot n%3, publishes slot (n+1)%3 and clears the max of slot (n+2)%3).
+struct Q16Slot {
+  uint32_t maxbits[3];  // max |decoded value| of buffers a, b, s as FP32 bits
+  int32_t exp[3];       // power-of-two exponents of buffers a, b, s
+};
+
+struct SweepArgs {
+  const void *u;  // field origins: element (0,0,0) of the interior
+  void *v;
+  const void *s;
+  int64_t pitch, plane;  // elements
+  int xs, ys, zs;
+  int ntk, ntj, chunks, chunk_len;
+  int remap;
+  float coef0, mA;      // REF: coef0 and -A
+  float c0, c1, c2n;    // LEAPFROG coefficients (c2n: this step's source coefficient)
+  float w[12];          // per-term weight, evaluation order (LEAPFROG: 2*w)
+  Q16Slot *qs;          // 16-bit storage: the 3-slot ring
+  int slot;             // slot read by this step
+  int ubuf, vbuf;       // ring indices (0 = a, 1 = b) of u and v
+  double gain;          // sum over terms of 2*|w| (LEAPFROG weights)
+};
+
+template <int FMT>
+struct Codec;
+template <>
+struct Codec<FMT_F32> {
+  using T = float;
+  __device__ static inline float dec(T x, int) { return x; }
+  __device__ static inline T enc(float x, int) { return x; }
+};
+template <>
+struct Codec<FMT_F16> {
+  using T = uint16_t;
+  __device__ static inline float dec(T q, int e) {
+    return ldexpf((float)__builtin_bit_cast(_Float16, q), e);
+  }
+  __device__ static inline T enc(float x, int e) {
+    return __builtin_bit_cast(uint16_t, (_Float16)ldexpf(x, -e));  // v_cvt_f16_f32, RNE
+  }
+};
+template <>
+struct Codec<FMT_BF16> {
+  using T = uint16_t;
+  __device__ static inline float dec(T q, int e) {
+    return ldexpf(__uint_as_float((uint32_t)q << 16), e);
+  }
+  __device__ static inline T enc(float x, int e) {
+    uint32_t b = __float_as_uint(ldexpf(x, -e));
+    if ((b & 0x7fffffffu) > 0x7f800000u) return (T)((b >> 16) | 0x40u);
+    b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even
+    return (T)(b >> 16);
+  }
+};
+
+// Smallest e with B*2^-e < 2^15 (frexp exponent - 15); B <= 0 -> 0.
+__host__ __device__ inline int scale_exp(double B) {
+  if (!(B > 0.0)) return 0;
+  int E;
+  frexp(B, &E);
+  return E - 15;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs; this bijection gives
+// each XCD a contiguous run of tiles so the j-halo rows two neighbouring tiles
+// share are served from that XCD's L2.
+__device__ inline int xcd_remap(int bid, int nb) {
+  const int per = (nb + kXcds - 1) / kXcds;
+  const int tall = nb - (per - 1) * kXcds;
+  const int x = bid % kXcds, l = bid / kXcds;
+  return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
+}
+
+template <int NPTS, int MODE, int FMT, int MJ, int NW>
+__device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, int ib, int ie,
+                                           float (*lds)[(NW * MJ + 2 * Terms<NPTS>::R) *
+                                                        (64 + 2 * Terms<NPTS>::R)],
+                                           float *red) {
+  using Tm = Terms<NPTS>;
+  constexpr int R = Tm::R, T = Tm::T;
+  using CD = Codec<FMT>;
+  using ST = typename CD::T;
+  constexpr int TJ = NW * MJ, LW = 64 + 2 * R;
+  constexpr int HJ = (2 * R + NW - 1) / NW;  // j-halo rows loaded per wave
+  static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int tid = threadIdx.x;
+  const int k0 = tk * 64, j0 = tj * TJ;
+  const int64_t P = a.plane, Pj = a.pitch;
+
+  // 16-bit storage: exponents of u, v, s and of the output.
+  int eu = 0, ev = 0, es = 0, eo = 0;
+  if constexpr (FMT != FMT_F32) {
+    const Q16Slot &cur = a.qs[a.slot];
+    eu = cur.exp[a.ubuf];
+    ev = cur.exp[a.vbuf];
+    es = cur.exp[2];
+    const double Mu = (double)__uint_as_float(cur.maxbits[a.ubuf]);
+    const double Mv = (double)__uint_as_float(cur.maxbits[a.vbuf]);
+    const double Ms = (double)__uint_as_float(cur.maxbits[2]);
+    const double gmul = fabs((double)a.c0) + fabs((double)a.c1) * a.gain;
+    const double B = gmul * Mu + Mv + fabs((double)a.c2n) * Ms;
+    eo = scale_exp(B);
+  }
+
+  // Tile base: element (i = 0, j0 - R, k0 - R); all in-tile offsets >= 0.
+  const ST *Ub = (const ST *)a.u + ((int64_t)(j0 - R) * Pj + (k0 - R));
+  ST *Vb = (ST *)a.v + ((int64_t)(j0 - R) * Pj + (k0 - R));
+  const ST *Sb = (const ST *)a.s + ((int64_t)(j0 - R) * Pj + (k0 - R));
+
+  // Clamp reads to the halo-padded extent: lanes/rows past the interior still
+  // load the halo values their neighbours need, and never leave the field.
+  const int jmax = a.ys + R - 1 - j0;  // last readable tile row
+  const int kmax = a.zs + R - 1 - k0;  // last readable tile column
+  const int kc = min(lane, kmax);
+  uint32_t off[MJ];
+  bool valid[MJ];
+#pragma unroll
+  for (int m = 0; m < MJ; m++) {
+    const int row = w * MJ + m;
+    off[m] = (uint32_t)((int64_t)(min(row, jmax) + R) * Pj + (kc + R));
+    valid[m] = (j0 + row < a.ys) && (k0 + lane < a.zs);
+  }
+  uint32_t hoff[HJ];
+  int hrow[HJ];
+#pragma unroll
+  for (int h = 0; h < HJ; h++) {
+    const int hh = w + NW * h;  // halo row id 0..2R-1
+    const int trow = hh < R ? hh - R : TJ + hh - R;
+    hrow[h] = hh < 2 * R ? (hh < R ? hh : TJ + hh) : -1;  // LDS row, -1: none
+    hoff[h] = (uint32_t)((int64_t)(min(trow, jmax) + R) * Pj + (kc + R));
+  }
+  const bool has_kh = tid < TJ * 2 * R;
+  uint32_t khoff = 0;
+  int khpos = 0;
+  if (has_kh) {
+    const int row = tid / (2 * R), c = tid % (2 * R);
+    const int tcol = c < R ? c - R : 64 + c - R;
+    khoff = (uint32_t)((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R));
+    khpos = (row + R) * LW + (c < R ? c : 64 + c);
+  }
+
+  // Register queue: q[m][t] = u at plane i - R + t.
+  float q[MJ][2 * R + 1];
+#pragma unroll
+  for (int t = 0; t < 2 * R; t++) {
+    const ST *Up = Ub + (int64_t)(ib - R + t) * P;
+#pragma unroll
+    for (int m = 0; m < MJ; m++) q[m][t] = CD::dec(Up[off[m]], eu);
+  }
+  ST nu[MJ], ns[MJ], nv[MJ], nhj[HJ], nhk;
+  auto prefetch = [&](int i) {
+    const ST *Up = Ub + (int64_t)(i + R) * P;
+    const ST *Uc = Ub + (int64_t)i * P;
+    const ST *Sp = Sb + (int64_t)i * P;
+    const ST *Vp = Vb + (int64_t)i * P;
+#pragma unroll
+    for (int m = 0; m < MJ; m++) {
+      nu[m] = Up[off[m]];
+      ns[m] = valid[m] ? Sp[off[m]] : ST(0);
+      nv[m] = valid[m] ? Vp[off[m]] : ST(0);
+    }
+#pragma unroll
+    for (int h = 0; h < HJ; h++) nhj[h] = hrow[h] >= 0 ? Uc[hoff[h]] : ST(0);
+    nhk = has_kh ? Uc[khoff] : ST(0);
+  };
+  prefetch(ib);
+
+  float mx = 0.0f;
+  for (int i = ib; i < ie; i++) {
+    float *L = lds[(i - ib) & 1];
+    ST cs[MJ], cv[MJ], chj[HJ], chk;
+#pragma unroll
+    for (int m = 0; m < MJ; m++) {
+      q[m][2 * R] = CD::dec(nu[m], eu);
+      cs[m] = ns[m];
+      cv[m] = nv[m];
+    }
+#pragma unroll
+    for (int h = 0; h < HJ; h++) chj[h] = nhj[h];
+    chk = nhk;
+    if (i + 1 < ie) prefetch(i + 1);
+
+#pragma unroll
+    for (int m = 0; m < MJ; m++) L[(w * MJ + m + R) * LW + lane + R] = q[m][R];
+#pragma unroll
+    for (int h = 0; h < HJ; h++)
+      if (hrow[h] >= 0) L[hrow[h] * LW + lane + R] = CD::dec(chj[h], eu);
+    if (has_kh) L[khpos] = CD::dec(chk, eu);
+    __syncthreads();
+
+    ST *Vp = Vb + (int64_t)i * P;
+#pragma unroll
+    for (int m = 0; m < MJ; m++) {
+      const int lr = w * MJ + m + R, col = lane + R;
+      auto pair = [&](int ax, int r) -> float {
+        if (ax == 0) return q[m][R - r] + q[m][R + r];
+        if (ax == 2) return L[lr * LW + col - r] + L[lr * LW + col + r];
+        const float lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R] : L[(lr - r) * LW + col];
+        const float hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R] : L[(lr + r) * LW + col];
+        return lo + hi;
+      };
+      const float u0 = q[m][R];
+      const float s0 = CD::dec(cs[m], es);
+      const float v0 = CD::dec(cv[m], ev);
+      float out;
+      if constexpr (MODE == MODE_REF) {
+        // stencil_N_ref_1: p = coef0*u0 + sum c*pair (the first product
+        // rounded, the rest fused), then v0 - A*u0 + s0*p.
+        float p = fmaf(a.coef0, u0, a.w[0] * pair(Tm::axis[0], Tm::r[0]));
+#pragma unroll
+        for (int t = 1; t < T; t++) p = fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
+        out = fmaf(s0, p, fmaf(a.mA, u0, v0));
+      } else {
+        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+#pragma unroll
+        for (int t = 1; t < T; t++) Lp = fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        out = fmaf(a.c2n, s0, fmaf(a.c1, Lp, fmaf(a.c0, u0, -v0)));
+      }
+      if (valid[m]) {
+        const ST qo = CD::enc(out, eo);
+        Vp[off[m]] = qo;
+        if constexpr (FMT != FMT_F32) mx = fmaxf(mx, fabsf(CD::dec(qo, eo)));
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MJ; m++)
+#pragma unroll
+      for (int t = 0; t < 2 * R; t++) q[m][t] = q[m][t + 1];
+  }
+
+  if constexpr (FMT != FMT_F32) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) red[w] = mx;
+    __syncthreads();
+    if (tid == 0) {
+      float b = red[0];
+#pragma unroll
+      for (int t = 1; t < NW; t++) b = fmaxf(b, red[t]);
+      atomicMax(&a.qs[(a.slot + 1) % 3].maxbits[a.vbuf], __float_as_uint(b));
+    }
+  }
+}
+
+// Publishes the next slot of the 16-bit scale ring (block 0, thread 0).
+template <int FMT>
+__device__ inline void q16_publish(const SweepArgs &a) {
+  if constexpr (FMT != FMT_F32) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const Q16Slot &cur = a.qs[a.slot];
+      Q16Slot &nxt = a.qs[(a.slot + 1) % 3];
+      Q16Slot &clr = a.qs[(a.slot + 2) % 3];
+      const double Mu = (double)__uint_as_float(cur.maxbits[a.ubuf]);
+      const double Mv = (double)__uint_as_float(cur.maxbits[a.vbuf]);
+      const double Ms = (double)__uint_as_float(cur.maxbits[2]);
+      const double gmul = fabs((double)a.c0) + fabs((double)a.c1) * a.gain;
+      nxt.exp[a.vbuf] = scale_exp(gmul * Mu + Mv + fabs((double)a.c2n) * Ms);
+      nxt.exp[a.ubuf] = cur.exp[a.ubuf];
+      nxt.exp[2] = cur.exp[2];
+      nxt.maxbits[a.ubuf] = cur.maxbits[a.ubuf];
+      nxt.maxbits[2] = cur.maxbits[2];
+      clr.maxbits[0] = clr.maxbits[1] = clr.maxbits[2] = 0u;
+    }
+  }
+}
+
+template <int NPTS, int MODE, int FMT, int MJ, int NW>
+__global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
+  constexpr int R = Terms<NPTS>::R;
+  __shared__ float lds[2][(NW * MJ + 2 * R) * (64 + 2 * R)];
+  __shared__ float red[NW];
+  q16_publish<FMT>(a);
+  int bid = blockIdx.x;
+  if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
+  const int tk = bid % a.ntk;
+  const int t = bid / a.ntk;
+  const int tj = t % a.ntj;
+  const int ch = t / a.ntj;
+  const int ib = ch * a.chunk_len;
+  const int ie = min(a.xs, ib + a.chunk_len);
+  if (ib >= ie) return;
+  sweep_tile<NPTS, MODE, FMT, MJ, NW>(a, tk, tj, ib, ie, lds, red);
+}
+
+}  // namespace sth

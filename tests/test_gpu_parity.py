@@ -1,0 +1,191 @@
+"""GPU parity: libstencil_hip (through its C ABI) against the CPU oracle on the
+same seeded fields.  FP32 REF and LEAPFROG must be bit-identical; the 16-bit
+storage form must be bit-identical to the oracle's emulation of it."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(16, 16, 16), (12, 14, 20), (33, 19, 70), (5, 7, 129)]
+
+
+def seeded(orc, g, seed, halo=0.0):
+    u = orc.fill_uniform(g, seed, 0, halo=halo)
+    s = orc.fill_uniform(g, seed, 1, halo=halo)
+    v = orc.fill_uniform(g, seed, 2, halo=halo)
+    return v, s, u
+
+
+def same_bits(a, b):
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def ulp_diff(a, b):
+    ai = a.view(np.int32).astype(np.int64)
+    bi = b.view(np.int32).astype(np.int64)
+    ai = np.where(ai < 0, -(ai & 0x7fffffff), ai)
+    bi = np.where(bi < 0, -(bi & 0x7fffffff), bi)
+    return int(np.abs(ai - bi).max()) if a.size else 0
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_ref_sweep_bit_exact(gpu, orc, npts, shape):
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    v, s, u = seeded(orc, g, seed=0)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        for f, a in (("v", v), ("s", s), ("u", u)):
+            ctx.upload(f, a, R)
+        ctx.iterate(5)
+        out = ctx.download("v", R)
+    orc.ref_sweep_f32(npts, g, v, s, u, 5)
+    I = g.interior
+    assert same_bits(out[I], v[I]), f"max ulp {ulp_diff(out[I], v[I])}"
+
+
+@pytest.mark.parametrize("npts", [7, 25])
+@pytest.mark.parametrize("tile", [(1, 4, 1), (2, 4, 3), (4, 4, 2), (8, 4, 1), (1, 8, 5),
+                                  (2, 8, 1), (4, 8, 4), (8, 8, 2), (2, 2, 1), (4, 2, 7),
+                                  (8, 2, 3)])
+def test_ref_every_tile_schedule(gpu, orc, npts, tile):
+    """Every compiled tile/chunk schedule gives the same bits (the schedule is
+    result-neutral, like the reference's iterators)."""
+    shape = (21, 37, 75)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    v, s, u = seeded(orc, g, seed=3)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.set_tuning(*tile, xcd_remap=tile[2] % 2)
+        for f, a in (("v", v), ("s", s), ("u", u)):
+            ctx.upload(f, a, R)
+        ctx.iterate(3)
+        out = ctx.download("v", R)
+    orc.ref_sweep_f32(npts, g, v, s, u, 3)
+    I = g.interior
+    assert same_bits(out[I], v[I])
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_device_seeded_init_matches_oracle(gpu, orc, npts):
+    shape = (9, 10, 67)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.init_uniform("u", 7, 0, -1.0, 1.0, 0.25)
+        got = ctx.download("u", R)
+    want = orc.fill_uniform(g, 7, 0, -1.0, 1.0, 0.25)
+    assert same_bits(got, want)
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_reference_kat(gpu, npts):
+    """The reference's own check: u=s=1, v=0 everywhere (halos included),
+    one warm-up sweep then 20 more -> v == #sweeps within 0.1
+    (main.cpp:289-312, iterators.cpp:292-325)."""
+    with gpu.Stencil(24, 24, 24, npts=npts) as ctx:
+        gpu.init_arrays(ctx)
+        gpu.iterate_3d(ctx, 1)
+        assert gpu.check(ctx, 1) == 0
+        gpu.iterate_3d(ctx, 20)
+        assert gpu.check(ctx, 21) == 0
+        # and a wrong expectation is caught
+        assert gpu.check(ctx, 22) == 24 ** 3
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_kat_exact_recurrence(gpu, orc, npts):
+    """With u = s = 1 the stencil value P is a constant, so every interior v
+    follows the scalar FP32 recurrence v <- fma(1, P, fma(-A, 1, v)) exactly."""
+    from fractions import Fraction as F
+
+    def rnd(x):
+        return float(np.float32(float(x)))  # exact rational -> nearest float32
+
+    coef0, A, w, R = orc.coeffs(npts)
+    terms = orc.terms(npts)
+    a0, r0 = terms[0]
+    p = rnd(F(float(coef0)) + F(rnd(F(float(w[a0, r0 - 1])) * 2)))
+    for ax, r in terms[1:]:
+        p = rnd(F(float(w[ax, r - 1])) * 2 + F(p))
+    vN = 0.0
+    for _ in range(13):
+        vN = rnd(F(p) + F(rnd(F(vN) - F(float(A)))))
+    with gpu.Stencil(10, 11, 12, npts=npts) as ctx:
+        ctx.init_const(0.0, 1.0, 1.0)
+        ctx.iterate(13)
+        out = ctx.download("v", R)
+    g = orc.Grid(10, 11, 12, R, R, R)
+    assert np.all(out[g.interior] == np.float32(vN))
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("shape", [(16, 16, 16), (19, 33, 70)])
+def test_leapfrog_bit_exact(gpu, orc, npts, shape):
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    v, s, u = seeded(orc, g, seed=0)  # zero halos
+    c1 = np.float32(0.25 if npts != 25 else 0.16)
+    c0 = np.float32(2.0 - orc.lap_gain(npts) * float(c1))
+    c2 = (np.arange(1, 8, dtype=np.float32) * np.float32(1e-3)).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.leapfrog(7, float(c0), float(c1), c2)
+        gu = ctx.download("u", R)
+        gv = ctx.download("v", R)
+    a, b = u.copy(), v.copy()
+    newest = orc.leapfrog_f32(npts, g, a, b, s, float(c0), float(c1), c2, 7)
+    want_u, want_v = (a, b) if newest == 0 else (b, a)
+    I = g.interior
+    assert same_bits(gu[I], want_u[I])
+    assert same_bits(gv[I], want_v[I])
+
+
+@pytest.mark.parametrize("fmt,dtype", [(1, "fp16"), (2, "bf16")])
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_q16_leapfrog_bit_exact(gpu, orc, fmt, dtype, npts):
+    shape = (14, 17, 70)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    v, s, u = seeded(orc, g, seed=5)
+    s *= np.float32(1e-3)
+    qs, exps, maxs = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(fmt, g, x, e)
+        qs.append(q)
+        exps.append(e)
+        maxs.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.linspace(0.5, -0.5, 9).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts, dtype=dtype) as ctx:
+        ctx.upload_raw16("u", qs[0], exps[0], R)
+        ctx.upload_raw16("v", qs[1], exps[1], R)
+        ctx.upload_raw16("s", qs[2], exps[2], R)
+        ctx.leapfrog(9, c0, c1, c2)
+        gu, eu, mu = ctx.download_raw16("u", R)
+        gv, ev, mv = ctx.download_raw16("v", R)
+    a, b = qs[0].copy(), qs[1].copy()
+    newest = orc.leapfrog_q16(npts, fmt, g, a, b, qs[2], exps, maxs, np.float32(c0),
+                              np.float32(c1), c2, 9)
+    (wu, weu, wmu), (wv, wev, wmv) = (
+        ((a, exps[0], maxs[0]), (b, exps[1], maxs[1])) if newest == 0 else
+        ((b, exps[1], maxs[1]), (a, exps[0], maxs[0])))
+    I = g.interior
+    assert (eu, ev) == (weu, wev)
+    assert np.array_equal(gu[I], wu[I]) and np.array_equal(gv[I], wv[I])
+    assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
+
+
+def test_norms_match_oracle(gpu, orc):
+    shape = (20, 21, 22)
+    g = orc.Grid(*shape, 1, 1, 1)
+    x = orc.fill_uniform(g, 11, 0)
+    with gpu.Stencil(*shape, npts=7) as ctx:
+        ctx.upload("u", x, 1)
+        mx, l2 = ctx.norms("u")
+    omx, ol2 = orc.norms(g, x)
+    assert mx == omx
+    assert abs(l2 - ol2) <= 1e-12 * ol2
