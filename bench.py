@@ -1,0 +1,230 @@
+#!/usr/bin/env python3
+"""bench.py -- throughput of the stencil3d sweep on one MI355X.
+
+Workload (the headline metric): 25-point leapfrog sweep over a 1040^3 interior
+(halo 4 = 0), u = v = Gaussian pulse (sigma 8 cells) at the centre plus
+U[-1e-3, 1e-3) noise from seed 1, s = 0, c1 = 0.16, c0 = 2 - 12 c1.  A step is
+one sweep over the whole grid (one kernel launch); fields are resident in HBM
+before the timed region starts.  FP32 storage gives `value`; the same run with
+FP16 and BF16 storage (FP32 arithmetic, per-field power-of-two scale) is
+reported beside it, with its error against an FP64 run of the Ricker-source
+medium (256^3, c dt/dx = 0.4, 1000 steps).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints ONE JSON line.  The CPU baseline is the reference's own hstencil3d
+(25-point, OpenMP static) timed on this host's cores on a bounded number of
+sweeps of the same grid, when oracle/_ref holds it (built from the reference's
+sources in the build container), else the repo's CPU restatement.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Gpts/s + HBM GB/s (% of peak), 25-pt 1040^3: FP16-store vs FP32 @ stated err"
+PEAK_GBS = 8000.0
+NPTS = 25
+FLOPS_PT = {7: 12, 13: 29, 25: 41}  # config.h:75,81,87
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def leap_coeffs(npts, c1):
+    import stencil3d_amd as S  # noqa: F401
+    gain = {7: 6.0, 13: 6.0, 25: 12.0}[npts]
+    return 2.0 - gain * c1, c1
+
+
+def run_throughput(S, n, dtype, steps, warmup, tuning=None):
+    """Returns wall seconds of `steps` timed steps, the HIP-event kernel time
+    and launch count, and the tuning used."""
+    c0, c1 = leap_coeffs(NPTS, 0.16)
+    with S.Stencil(n, n, n, npts=NPTS, dtype=dtype) as ctx:
+        ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 0)
+        ctx.init_gaussian("v", 1.0, 8.0, 1e-3, 1, 1)
+        ctx.init_point("s", 0, 0, 0, 0.0)
+        if tuning:
+            ctx.set_tuning(**tuning)
+        ctx.leapfrog(warmup, c0, c1)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.timer_start()
+        ctx.leapfrog(steps, c0, c1)
+        kms, launches = ctx.timer_read()  # synchronises the stream
+        t1 = time.perf_counter()
+        mx, l2 = ctx.norms("u")
+        used = ctx.get_tuning()
+    return t1 - t0, kms, launches, used, (mx, l2)
+
+
+def ricker_c2(steps, dt=1e-3, f=15.0):
+    t0 = 1.0 / f
+    out = []
+    for n in range(steps):
+        a = (math.pi * f * (n * dt - t0)) ** 2
+        out.append(dt * dt * (1.0 - 2.0 * a) * math.exp(-a))
+    return out
+
+
+def run_error(S, dtype, n=256, steps=1000):
+    """Config-4 medium: 25-point, c dt/dx = 0.4, Ricker source (15 Hz, dt 1 ms)
+    at the centre; error of the `dtype` run against the FP64 GPU run."""
+    import numpy as np
+    c0, c1 = leap_coeffs(NPTS, 0.16)
+    c2 = ricker_c2(steps)
+    res = {}
+    for dt in (dtype, "fp64"):
+        with S.Stencil(n, n, n, npts=NPTS, dtype=dt) as ctx:
+            ctx.init_point("u", 0, 0, 0, 0.0)
+            ctx.init_point("v", 0, 0, 0, 0.0)
+            ctx.init_point("s", n // 2, n // 2, n // 2, 1.0)
+            ctx.leapfrog(steps, c0, c1, c2)
+            res[dt] = ctx.download("u", 4, dtype=np.float64)
+    x, r = res[dtype], res["fp64"]
+    err = np.abs(x - r)
+    return float(err.max() / np.abs(r).max()), float(np.sqrt((err ** 2).sum() / (r ** 2).sum()))
+
+
+def cpu_baseline(n, sweeps):
+    """hstencil3d (the reference, 25-point, iterate_3d, OpenMP static) on this
+    host, or the repo's CPU restatement when the reference build is absent."""
+    cores = os.cpu_count() or 1
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread",
+               OMP_PLACES="cores")
+    binary = os.path.join(ROOT, "oracle", "_ref", f"hstencil3d_{NPTS}")
+    if os.path.exists(binary):
+        cmd = [binary, "--xs", str(n), "--ys", str(n), "--zs", str(n), "--xh", "4", "--yh", "4",
+               "--zh", "4", "--iter", str(sweeps)]
+        out = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600).stdout
+        m = re.search(r"finish compute in ([0-9.]+) secs", out)
+        if not m:
+            raise RuntimeError("hstencil3d printed no timing")
+        secs = float(m.group(1))
+        kind = "reference"
+        what = (f"hstencil3d_25 (reference iterate_3d + stencil_25_ref_1, REF update, "
+                f"OpenMP static) {n}^3, {sweeps} timed sweeps after 1 warm-up")
+    else:
+        import numpy as np  # noqa: F401
+        from oracle import pyoracle as O
+        g = O.Grid(n, n, n, 4, 4, 4)
+        a = O.fill_gaussian(g, 1.0, 8.0, 1e-3, 1, 0)
+        b = O.fill_gaussian(g, 1.0, 8.0, 1e-3, 1, 1)
+        s = g.zeros()
+        c0, c1 = leap_coeffs(NPTS, 0.16)
+        O.leapfrog_f32(NPTS, g, a, b, s, c0, c1, None, 1, cores)
+        t0 = time.perf_counter()
+        O.leapfrog_f32(NPTS, g, a, b, s, c0, c1, None, sweeps, cores)
+        secs = time.perf_counter() - t0
+        kind = "port"
+        what = f"oracle leapfrog_f32 (OpenMP static) {n}^3, {sweeps} timed sweeps"
+    gpts = n ** 3 * sweeps / secs / 1e9
+    cpu = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            cpu = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                       "")
+    except OSError:
+        pass
+    return {"value": round(gpts, 4), "unit": "Gpts/s", "cores": cores, "kind": kind,
+            "sample": what, "secs": round(secs, 3), "cpu": cpu}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=1040)
+    ap.add_argument("--cpu-sweeps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--no-16bit", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the sweep is not sharded across GPUs in this build)", file=sys.stderr)
+        sys.exit(2)
+
+    import stencil3d_amd as S
+    if S.device_count() < 1:
+        sys.exit("no GPU visible")
+    n, K, W = args.size, args.steps, args.warmup
+    pts = float(n) ** 3
+
+    log(f"[bench] fp32 {NPTS}-pt {n}^3: {W} warm-up + {K} timed steps")
+    wall, kms, launches, tune, nrm = run_throughput(S, n, "fp32", K, W)
+    gpts = pts * K / wall / 1e9
+    per_launch_s = kms / launches * 1e-3
+    achieved = 16.0 * pts / per_launch_s / 1e9
+    log(f"[bench] fp32: {gpts:.2f} Gpts/s, kernel {kms / launches:.4f} ms, {achieved:.0f} GB/s")
+
+    extra = {}
+    if not args.no_16bit:
+        for dt in ("fp16", "bf16"):
+            log(f"[bench] {dt}: {W} warm-up + {K} timed steps")
+            w2, k2, l2, t2, _ = run_throughput(S, n, dt, K, W)
+            g2 = pts * K / w2 / 1e9
+            a2 = 8.0 * pts / (k2 / l2 * 1e-3) / 1e9
+            extra[dt] = {"gpts": round(g2, 3), "ms_per_step": round(w2 / K * 1e3, 4),
+                         "kernel_ms": round(k2 / l2, 4), "hbm_gbs_alg": round(a2, 1),
+                         "frac_of_peak": round(a2 / PEAK_GBS, 4),
+                         "speedup_vs_fp32": round(g2 / gpts, 3), "tuning": t2}
+            log(f"[bench] {dt}: {g2:.2f} Gpts/s ({g2 / gpts:.2f}x fp32)")
+        if not args.no_error:
+            for dt in ("fp16", "bf16"):
+                log(f"[bench] error of {dt} vs FP64 (256^3, 1000 steps, Ricker source)")
+                mr, lr = run_error(S, dt)
+                extra[dt]["err_vs_fp64"] = {"maxrel": mr, "l2rel": lr,
+                                            "setup": "25-pt 256^3, c dt/dx=0.4, 1000 steps, "
+                                                     "Ricker 15 Hz at the centre"}
+                log(f"[bench] {dt}: max-rel {mr:.3e}, L2-rel {lr:.3e}")
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: {args.cpu_sweeps} sweeps")
+        cpu = cpu_baseline(n, args.cpu_sweeps)
+        log(f"[bench] cpu: {cpu['value']} Gpts/s ({cpu['kind']}, {cpu['cores']} cores)")
+
+    line = {
+        "metric": METRIC,
+        "value": round(gpts, 3),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(wall / K * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: Gaussian pulse (sigma 8) + U[-1e-3,1e-3) noise, seed 1; s = 0",
+        "config": {"workload": f"{NPTS}-point leapfrog sweep, {n}^3 interior, FP32 storage",
+                   "npts": NPTS, "grid": [n, n, n], "halo": 4, "c1": 0.16,
+                   "tuning": tune},
+        "hbm_gbs_alg": round(16.0 * gpts, 1),
+        "gflops_nominal": round(FLOPS_PT[NPTS] * gpts, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
+                     "traffic": None, "kernel_ms": round(kms / launches, 5),
+                     "bytes_per_launch_alg": 16.0 * pts},
+        "cpu_baseline": cpu,
+        "norms_u": {"max": nrm[0], "l2": nrm[1]},
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -43,7 +43,10 @@ typedef enum {
   STH_ERR_UNSUPPORTED = 6
 } sth_status;
 
-typedef enum { STH_FP32 = 0, STH_FP16 = 1, STH_BF16 = 2 } sth_dtype;
+/* FP64 storage computes in FP64 with the FP32 stencil constants promoted
+ * (the same discrete operator as FP32, for on-GPU error references; the
+ * reference's -DUSE_DOUBLE build, config.h:96-100, is the analogue). */
+typedef enum { STH_FP32 = 0, STH_FP16 = 1, STH_BF16 = 2, STH_FP64 = 3 } sth_dtype;
 
 /* Logical fields.  In REF mode v is updated in place and u, s are read-only.
  * In LEAPFROG mode u is the newest field and v the previous one; each step
@@ -67,7 +70,7 @@ typedef struct {
  * 256-byte boundary. */
 typedef struct {
   uint32_t radius;     /* R = 1, 2, 4 */
-  uint32_t elem_bytes; /* 4 or 2 */
+  uint32_t elem_bytes; /* 8, 4 or 2 */
   uint64_t pitch;      /* elements between consecutive j rows */
   uint64_t plane;      /* elements between consecutive i planes */
   uint64_t origin;     /* element offset of interior point (0,0,0) */
@@ -110,7 +113,7 @@ int sth_get_layout(const sth_ctx *ctx, sth_layout_info *out);
 
 /* ---- field initialisation and transfer ----------------------------------- */
 /* init_arrays (iterators.cpp:292-301): every element of the allocation,
- * halos included, v = v0, s = s0, u = u0.  REF mode; FP32 storage only. */
+ * halos included, v = v0, s = s0, u = u0.  FP32 or FP64 storage. */
 int sth_init_const(sth_ctx *ctx, float v0, float s0, float u0);
 /* Seeded uniform [lo,hi) interior from the counter-based SplitMix64 stream
  * (seed, field_id) -- identical values to the oracle's generator -- and the
@@ -130,6 +133,10 @@ int sth_upload(sth_ctx *ctx, sth_field f, const float *host, const sth_host_layo
 /* Device field -> FP32 host array (interior and radius-wide halo; other host
  * cells are left untouched).  Synchronises the stream. */
 int sth_download(sth_ctx *ctx, sth_field f, float *host, const sth_host_layout *hl);
+/* FP64 host arrays: upload into FP32 or FP64 storage (FP32 rounds each
+ * value), download from any storage (16-bit values decoded exactly). */
+int sth_upload_f64(sth_ctx *ctx, sth_field f, const double *host, const sth_host_layout *hl);
+int sth_download_f64(sth_ctx *ctx, sth_field f, double *host, const sth_host_layout *hl);
 /* 16-bit storage: raw stored values and the field's scale exponent
  * (value = stored * 2^exp).  upload_raw sets the field's max from the data. */
 int sth_upload_raw16(sth_ctx *ctx, sth_field f, const uint16_t *host, int exp,
@@ -140,7 +147,7 @@ int sth_download_raw16(sth_ctx *ctx, sth_field f, uint16_t *host, int *exp, floa
 /* ---- the sweep -------------------------------------------------------------- */
 /* ITERATE replacement (iterate_3d, iterators.cpp:130-157): `iterations` REF
  * sweeps v <- v - A*u + s*(coef0*u + sum c*(u[-r]+u[+r])) in the reference's
- * FMA order (bit-identical to the reference host build).  FP32 storage.
+ * FMA order (bit-identical to the reference host build).  FP32 or FP64 storage.
  * One kernel launch per sweep; asynchronous. */
 int sth_iterate(sth_ctx *ctx, uint32_t iterations);
 /* LEAPFROG steps (formula B, README.md:10-24):
@@ -167,6 +174,45 @@ int sth_autotune(sth_ctx *ctx, int leapfrog, uint32_t reps, sth_tuning *best, fl
  * and returns the summed kernel time (ms) and the number of launches. */
 int sth_timer_start(sth_ctx *ctx);
 int sth_timer_read(sth_ctx *ctx, double *kernel_ms, uint64_t *launches);
+
+/* ---- batching: many independent grids ("shots") advanced together ------ */
+/* The reference runs one grid per process; a batch holds nshots grids of
+ * uneven size (dims = nshots x {xs, ys, zs}), all with the same stencil and
+ * FP32 storage, and advances them together with the LEAPFROG update.
+ * sth_batch_shot returns the context of one shot, through which every
+ * single-grid call (init, upload, download, norms) works on that shot; the
+ * batch orders its stream against the shots' streams. */
+typedef struct sth_batch sth_batch;
+
+#define STH_BATCH_PERSISTENT 0 /* one persistent launch per step over all shots */
+#define STH_BATCH_SEQUENTIAL 1 /* one plain sweep launch per shot per step */
+#define STH_BATCH_GRAPH 0x100  /* flag: capture the call's time loop in one hipGraph */
+
+int sth_batch_create(const uint32_t *dims, uint32_t nshots, uint32_t npts, uint32_t dtype,
+                     sth_batch **out);
+int sth_batch_destroy(sth_batch *b);
+sth_ctx *sth_batch_shot(sth_batch *b, uint32_t shot);
+uint32_t sth_batch_size(const sth_batch *b);
+/* Tile knobs shared by all shots; planes = i-planes per work item (0: the
+ * whole x-range); grid_blocks = persistent grid (0: the resident capacity). */
+int sth_batch_set_tuning(sth_batch *b, const sth_tuning *t, uint32_t planes,
+                         uint32_t grid_blocks);
+int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,
+                         uint32_t *grid_blocks);
+/* `steps` LEAPFROG steps of every shot (c2: per-step source coefficients,
+ * NULL = 0); method = STH_BATCH_PERSISTENT or STH_BATCH_SEQUENTIAL, optionally
+ * | STH_BATCH_GRAPH.  Asynchronous. */
+int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2,
+                       uint32_t method);
+int sth_batch_sync(sth_batch *b);
+/* Times tile x planes-per-item candidates of the persistent step (fields are
+ * modified), keeps the fastest and returns it. */
+int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *planes,
+                       float *best_ms);
+/* Accumulates the GPU time of every sth_batch_leapfrog call (events around
+ * the launched work) between start and read. */
+int sth_batch_timer_start(sth_batch *b);
+int sth_batch_timer_read(sth_batch *b, double *ms, uint64_t *launches);
 
 #ifdef __cplusplus
 }

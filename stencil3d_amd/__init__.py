@@ -23,7 +23,7 @@ FRONTEND_PATH = os.path.join(PKG, "hstencil3d_hip")
 
 STH_OK, STH_ERR_ARG, STH_ERR_HIP, STH_ERR_NOMEM, STH_ERR_STATE, STH_ERR_NODEV, \
     STH_ERR_UNSUPPORTED = range(7)
-DTYPES = {"fp32": 0, "fp16": 1, "bf16": 2}
+DTYPES = {"fp32": 0, "fp16": 1, "bf16": 2, "fp64": 3}
 FIELDS = {"v": 0, "s": 1, "u": 2}
 
 
@@ -76,6 +76,8 @@ SIGNATURES = {
     "sth_init_point": (C.c_int, [_VP, C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float]),
     "sth_upload": (C.c_int, [_VP, C.c_int, _P(C.c_float), _P(HostLayout)]),
     "sth_download": (C.c_int, [_VP, C.c_int, _P(C.c_float), _P(HostLayout)]),
+    "sth_upload_f64": (C.c_int, [_VP, C.c_int, _P(C.c_double), _P(HostLayout)]),
+    "sth_download_f64": (C.c_int, [_VP, C.c_int, _P(C.c_double), _P(HostLayout)]),
     "sth_upload_raw16": (C.c_int, [_VP, C.c_int, _P(C.c_uint16), C.c_int, _P(HostLayout)]),
     "sth_download_raw16": (C.c_int, [_VP, C.c_int, _P(C.c_uint16), _P(C.c_int),
                                      _P(C.c_float), _P(HostLayout)]),
@@ -89,7 +91,23 @@ SIGNATURES = {
     "sth_autotune": (C.c_int, [_VP, C.c_int, C.c_uint32, _P(Tuning), _P(C.c_float)]),
     "sth_timer_start": (C.c_int, [_VP]),
     "sth_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
+    "sth_batch_create": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32,
+                                   _P(_VP)]),
+    "sth_batch_destroy": (C.c_int, [_VP]),
+    "sth_batch_shot": (_VP, [_VP, C.c_uint32]),
+    "sth_batch_size": (C.c_uint32, [_VP]),
+    "sth_batch_set_tuning": (C.c_int, [_VP, _P(Tuning), C.c_uint32, C.c_uint32]),
+    "sth_batch_get_tuning": (C.c_int, [_VP, _P(Tuning), _P(C.c_uint32), _P(C.c_uint32)]),
+    "sth_batch_leapfrog": (C.c_int, [_VP, C.c_uint32, C.c_float, C.c_float, _P(C.c_float),
+                                     C.c_uint32]),
+    "sth_batch_sync": (C.c_int, [_VP]),
+    "sth_batch_autotune": (C.c_int, [_VP, C.c_uint32, _P(Tuning), _P(C.c_uint32),
+                                     _P(C.c_float)]),
+    "sth_batch_timer_start": (C.c_int, [_VP]),
+    "sth_batch_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
 }
+
+BATCH_PERSISTENT, BATCH_SEQUENTIAL, BATCH_GRAPH = 0, 1, 0x100
 
 _lib = None
 
@@ -142,20 +160,25 @@ def device_count():
 class Stencil:
     """One run of the sweep on the current GPU: three device fields of an
     xs x ys x zs interior (i slowest, k unit stride) for an npts-point stencil,
-    stored as dtype ('fp32', 'fp16', 'bf16')."""
+    stored as dtype ('fp32', 'fp16', 'bf16', 'fp64')."""
 
-    def __init__(self, xs, ys, zs, npts=7, dtype="fp32"):
+    def __init__(self, xs, ys, zs, npts=7, dtype="fp32", _handle=None):
         self.xs, self.ys, self.zs, self.npts, self.dtype = xs, ys, zs, npts, dtype
         self.R = radius(npts)
         self._g = Geom(xs, ys, zs, npts, DTYPES[dtype])
-        h = C.c_void_p()
-        _ok(lib().sth_create(C.byref(self._g), C.byref(h)), "sth_create")
-        self._h = h
+        self._owned = _handle is None
+        if _handle is None:
+            h = C.c_void_p()
+            _ok(lib().sth_create(C.byref(self._g), C.byref(h)), "sth_create")
+            self._h = h
+        else:
+            self._h = C.c_void_p(_handle)
 
     # -- lifetime
     def close(self):
         if getattr(self, "_h", None):
-            lib().sth_destroy(self._h)
+            if self._owned:
+                lib().sth_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -199,17 +222,31 @@ class Stencil:
         _ok(lib().sth_init_point(self.handle, FIELDS[field], i, j, k, value), "sth_init_point")
 
     def upload(self, field, arr, halo):
+        """float32 arrays go through sth_upload, float64 through sth_upload_f64."""
+        hl = HostLayout(*_halo3(halo))
+        if np.asarray(arr).dtype == np.float64:
+            arr = np.ascontiguousarray(arr)
+            assert arr.shape == self.host_shape(halo)
+            _ok(lib().sth_upload_f64(self.handle, FIELDS[field],
+                                     arr.ctypes.data_as(_P(C.c_double)), C.byref(hl)),
+                "sth_upload_f64")
+            return
         arr = np.ascontiguousarray(arr, dtype=np.float32)
         assert arr.shape == self.host_shape(halo), (arr.shape, self.host_shape(halo))
-        hl = HostLayout(*_halo3(halo))
         _ok(lib().sth_upload(self.handle, FIELDS[field],
                              arr.ctypes.data_as(_P(C.c_float)), C.byref(hl)), "sth_upload")
 
-    def download(self, field, halo, out=None):
+    def download(self, field, halo, out=None, dtype=np.float32):
         if out is None:
-            out = np.zeros(self.host_shape(halo), dtype=np.float32)
-        assert out.dtype == np.float32 and out.flags["C_CONTIGUOUS"]
+            out = np.zeros(self.host_shape(halo), dtype=dtype)
+        assert out.flags["C_CONTIGUOUS"] and out.shape == self.host_shape(halo)
         hl = HostLayout(*_halo3(halo))
+        if out.dtype == np.float64:
+            _ok(lib().sth_download_f64(self.handle, FIELDS[field],
+                                       out.ctypes.data_as(_P(C.c_double)), C.byref(hl)),
+                "sth_download_f64")
+            return out
+        assert out.dtype == np.float32
         _ok(lib().sth_download(self.handle, FIELDS[field],
                                out.ctypes.data_as(_P(C.c_float)), C.byref(hl)), "sth_download")
         return out
@@ -279,6 +316,86 @@ class Stencil:
     def timer_read(self):
         ms, n = C.c_double(), C.c_uint64()
         _ok(lib().sth_timer_read(self.handle, C.byref(ms), C.byref(n)), "sth_timer_read")
+        return ms.value, n.value
+
+
+class Batch:
+    """Many independent grids advanced together (sth_batch_*).  dims is a list
+    of (xs, ys, zs); shot(s) returns a Stencil view of one shot's fields."""
+
+    def __init__(self, dims, npts=13, dtype="fp32"):
+        flat = (C.c_uint32 * (3 * len(dims)))(*[int(d) for xyz in dims for d in xyz])
+        h = C.c_void_p()
+        _ok(lib().sth_batch_create(flat, len(dims), npts, DTYPES[dtype], C.byref(h)),
+            "sth_batch_create")
+        self._h = h
+        self.dims = [tuple(d) for d in dims]
+        self.npts, self.dtype = npts, dtype
+        self._shots = [Stencil(*d, npts=npts, dtype=dtype,
+                               _handle=lib().sth_batch_shot(h, q))
+                       for q, d in enumerate(self.dims)]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            for s in self._shots:
+                s.close()
+            lib().sth_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self._shots)
+
+    def shot(self, q) -> Stencil:
+        return self._shots[q]
+
+    def set_tuning(self, rows_per_wave=0, waves=0, planes=0, grid_blocks=0, xcd_remap=1):
+        t = Tuning(rows_per_wave, waves, 0, xcd_remap)
+        _ok(lib().sth_batch_set_tuning(self._h, C.byref(t), planes, grid_blocks),
+            "sth_batch_set_tuning")
+
+    def get_tuning(self):
+        t, p, g = Tuning(), C.c_uint32(), C.c_uint32()
+        _ok(lib().sth_batch_get_tuning(self._h, C.byref(t), C.byref(p), C.byref(g)),
+            "sth_batch_get_tuning")
+        d = t.as_dict()
+        d.update(planes=p.value, grid_blocks=g.value)
+        return d
+
+    def leapfrog(self, steps, c0, c1, c2=None, method=BATCH_PERSISTENT, graph=False):
+        c2a = None
+        if c2 is not None:
+            c2a = np.ascontiguousarray(np.broadcast_to(np.asarray(c2, np.float32), (steps,)))
+        ptr = None if c2a is None else c2a.ctypes.data_as(_P(C.c_float))
+        m = method | (BATCH_GRAPH if graph else 0)
+        _ok(lib().sth_batch_leapfrog(self._h, steps, c0, c1, ptr, m), "sth_batch_leapfrog")
+
+    def sync(self):
+        _ok(lib().sth_batch_sync(self._h), "sth_batch_sync")
+
+    def autotune(self, reps=2):
+        t, p, ms = Tuning(), C.c_uint32(), C.c_float()
+        _ok(lib().sth_batch_autotune(self._h, reps, C.byref(t), C.byref(p), C.byref(ms)),
+            "sth_batch_autotune")
+        d = t.as_dict()
+        d.update(planes=p.value)
+        return d, ms.value
+
+    def timer_start(self):
+        _ok(lib().sth_batch_timer_start(self._h), "sth_batch_timer_start")
+
+    def timer_read(self):
+        ms, n = C.c_double(), C.c_uint64()
+        _ok(lib().sth_batch_timer_read(self._h, C.byref(ms), C.byref(n)),
+            "sth_batch_timer_read")
         return ms.value, n.value
 
 

@@ -34,7 +34,8 @@ HIP_FLAGS = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-HIP_SOURCES = ["sth_api.hip", "fields.hip", "sweep_7.hip", "sweep_13.hip", "sweep_25.hip"]
+HIP_SOURCES = ["sth_api.hip", "fields.hip", "sweep_7.hip", "sweep_13.hip", "sweep_25.hip",
+               "batch.hip"]
 
 
 def _run(cmd, quiet):

@@ -1,0 +1,493 @@
+// batch.hip -- many independent grids ("shots") advanced together.
+//
+// The reference runs one grid per process (main.cpp); a survey of many
+// shots of uneven size is the production use of the leapfrog sweep.  Two
+// ways to run a time step over all shots:
+//  * persistent: ONE launch per step.  A grid sized to the resident capacity
+//    of the GPU pulls work items -- (shot, k-tile, j-tile, i-chunk) -- from a
+//    queue (an atomic ticket counter), largest items first, and runs each with
+//    the same tile engine as the single-grid sweep (sweep_tile).
+//  * sequential: one plain sweep launch per shot per step.
+// Either way the time loop can be captured into one hipGraph per call.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "sth_internal.h"
+#include "sweep_dispatch.h"
+
+using namespace sth;
+using namespace sth::detail;
+
+namespace sth {
+
+struct BatchItem {
+  int shot, tk, tj, ch;
+};
+
+struct BatchArgs {
+  const SweepArgs *shots;  // per-shot args: u = buffer a, v = buffer b
+  const BatchItem *items;  // sorted work list
+  int total_items;
+  unsigned *ticket;        // queue counter
+  unsigned ticket_base;    // tickets taken by earlier steps since the last reset
+  int parity;              // 0: u = a, v = b; 1: u = b, v = a
+  float c0, c1, c2n;
+};
+
+// Every block takes tickets until it draws one past the end, so a step
+// consumes exactly total_items + gridDim.x tickets: step n of a launch
+// sequence starts at ticket_base = n * (total_items + grid).
+template <int NPTS, int MODE, int FMT, int MJ, int NW>
+__global__ __launch_bounds__(64 * NW) void batch_kernel(const BatchArgs b) {
+  constexpr int R = Terms<NPTS>::R;
+  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 + 2 * R)];
+  __shared__ float red[NW];
+  __shared__ int sh_item;
+  for (;;) {
+    if (threadIdx.x == 0) sh_item = (int)(atomicAdd(b.ticket, 1u) - b.ticket_base);
+    __syncthreads();
+    const int item = sh_item;
+    __syncthreads();
+    if (item >= b.total_items) break;
+    const BatchItem it = b.items[item];
+    SweepArgs a = b.shots[it.shot];
+    if (b.parity) {
+      const void *t = a.u;
+      a.u = a.v;
+      a.v = const_cast<void *>(t);
+    }
+    a.c0 = b.c0;
+    a.c1 = b.c1;
+    a.c2n = b.c2n;
+    const int ib = it.ch * a.chunk_len;
+    const int ie = min(a.xs, ib + a.chunk_len);
+    sweep_tile<NPTS, MODE, FMT, MJ, NW>(a, it.tk, it.tj, ib, ie, lds, red);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <int NPTS>
+const void *batch_fn(int mj, int nw) {
+#define STH_BATCH_FN(MJ, NW) \
+  if (mj == MJ && nw == NW) return (const void *)&batch_kernel<NPTS, MODE_LEAP, FMT_F32, MJ, NW>;
+  STH_TILE_LIST(STH_BATCH_FN)
+#undef STH_BATCH_FN
+  return nullptr;
+}
+
+template <int NPTS>
+hipError_t launch_batch_n(int mj, int nw, unsigned grid, hipStream_t st, const BatchArgs &b) {
+#define STH_BATCH_CASE(MJ, NW)                                                                  \
+  if (mj == MJ && nw == NW) {                                                                   \
+    hipLaunchKernelGGL((batch_kernel<NPTS, MODE_LEAP, FMT_F32, MJ, NW>), dim3(grid),            \
+                       dim3(64 * NW), 0, st, b);                                                \
+    return hipGetLastError();                                                                   \
+  }
+  STH_TILE_LIST(STH_BATCH_CASE)
+#undef STH_BATCH_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace sth
+
+struct sth_batch {
+  std::vector<sth_ctx *> shots;
+  uint32_t npts = 0, dtype = 0;
+  int device = 0, cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, evs = nullptr;
+  sth_tuning tune{};
+  uint32_t planes = 0;       // i-planes per work item (0: whole x-range)
+  uint32_t grid_blocks = 0;  // persistent grid (0: resident capacity)
+  // device tables for the persistent kernel (rebuilt when the tuning changes)
+  SweepArgs *shot_args = nullptr;
+  BatchItem *items = nullptr;
+  unsigned *ticket = nullptr;
+  int total_items = 0;
+  unsigned ticket_base = 0;
+  bool tables_ok = false;
+  hipGraphExec_t exec = nullptr;
+  bool timing = false;
+  double timed_ms = 0.0;
+  uint64_t launches = 0;
+};
+
+namespace {
+
+hipError_t launch_batch(uint32_t npts, int mj, int nw, unsigned grid, hipStream_t st,
+                        const BatchArgs &b) {
+  if (npts == 7) return launch_batch_n<7>(mj, nw, grid, st, b);
+  if (npts == 13) return launch_batch_n<13>(mj, nw, grid, st, b);
+  if (npts == 25) return launch_batch_n<25>(mj, nw, grid, st, b);
+  return hipErrorInvalidValue;
+}
+
+const void *batch_kernel_fn(uint32_t npts, int mj, int nw) {
+  if (npts == 7) return batch_fn<7>(mj, nw);
+  if (npts == 13) return batch_fn<13>(mj, nw);
+  if (npts == 25) return batch_fn<25>(mj, nw);
+  return nullptr;
+}
+
+#define STH_TRY(expr)                            \
+  do {                                           \
+    hipError_t e_ = (expr);                      \
+    if (e_ != hipSuccess) return hip_status(e_); \
+  } while (0)
+
+// Per-shot plan for the batch tuning: the tile of the batch, i-chunks of
+// `planes` planes.
+Plan shot_plan(const sth_batch *b, const sth_ctx *c) {
+  sth_tuning t = b->tune;
+  const int xs = (int)c->g.xs;
+  const int planes = b->planes ? (int)b->planes : xs;
+  t.chunks = (uint32_t)std::min(64, std::max(1, (xs + planes - 1) / planes));
+  return make_plan(c, t);
+}
+
+unsigned persistent_grid(const sth_batch *b, int mj, int nw) {
+  if (b->grid_blocks) return b->grid_blocks;
+  int per_cu = 0;
+  const void *fn = batch_kernel_fn(b->npts, mj, nw);
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * nw, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (unsigned)(per_cu * b->cus);
+}
+
+int build_tables(sth_batch *b) {
+  if (b->tables_ok) return STH_OK;
+  std::vector<SweepArgs> args(b->shots.size());
+  std::vector<BatchItem> items;
+  std::vector<int> len;
+  for (size_t s = 0; s < b->shots.size(); s++) {
+    sth_ctx *c = b->shots[s];
+    const Plan p = shot_plan(b, c);
+    SweepArgs &a = args[s];
+    memset(&a, 0, sizeof(a));
+    const int ub = c->ubuf;
+    c->ubuf = 0;  // table holds u = buffer a, v = buffer b; the kernel swaps by parity
+    base_args(c, p, a);
+    c->ubuf = ub;
+    fill_weights(c, true, a);
+    a.remap = 0;
+    for (int ch = 0; ch < p.chunks; ch++)
+      for (int tj = 0; tj < p.ntj; tj++)
+        for (int tk = 0; tk < p.ntk; tk++) {
+          items.push_back({(int)s, tk, tj, ch});
+          len.push_back(std::min((int)c->g.xs, (ch + 1) * p.chunk_len) - ch * p.chunk_len);
+        }
+  }
+  // largest items first (stable: items of one shot stay together)
+  std::vector<int> order(items.size());
+  for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return len[x] > len[y]; });
+  std::vector<BatchItem> sorted(items.size());
+  for (size_t q = 0; q < order.size(); q++) sorted[q] = items[order[q]];
+  if (b->shot_args) (void)hipFree(b->shot_args);
+  if (b->items) (void)hipFree(b->items);
+  b->shot_args = nullptr;
+  b->items = nullptr;
+  STH_TRY(hipMalloc((void **)&b->shot_args, args.size() * sizeof(SweepArgs)));
+  STH_TRY(hipMalloc((void **)&b->items, std::max<size_t>(1, sorted.size()) * sizeof(BatchItem)));
+  STH_TRY(hipMemcpyAsync(b->shot_args, args.data(), args.size() * sizeof(SweepArgs),
+                         hipMemcpyHostToDevice, b->stream));
+  STH_TRY(hipMemcpyAsync(b->items, sorted.data(), sorted.size() * sizeof(BatchItem),
+                         hipMemcpyHostToDevice, b->stream));
+  STH_TRY(hipStreamSynchronize(b->stream));
+  b->total_items = (int)sorted.size();
+  b->tables_ok = true;
+  return STH_OK;
+}
+
+// Orders the batch stream after all work queued on the shots' streams.
+int join_shots(sth_batch *b) {
+  for (sth_ctx *c : b->shots) {
+    STH_TRY(hipEventRecord(b->evs, c->stream));
+    STH_TRY(hipStreamWaitEvent(b->stream, b->evs, 0));
+  }
+  return STH_OK;
+}
+
+// Orders every shot's stream after the work queued on the batch stream.
+int fork_shots(sth_batch *b) {
+  STH_TRY(hipEventRecord(b->evs, b->stream));
+  for (sth_ctx *c : b->shots) STH_TRY(hipStreamWaitEvent(c->stream, b->evs, 0));
+  return STH_OK;
+}
+
+// Queues `steps` steps on the batch stream (eagerly or while capturing).
+int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2, int method,
+                  int parity0, uint64_t *launches) {
+  const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : 4;
+  const int nw = b->tune.waves ? (int)b->tune.waves : 4;
+  if (method == STH_BATCH_PERSISTENT) {
+    const unsigned G = persistent_grid(b, mj, nw);
+    const unsigned per_step = (unsigned)b->total_items + G;
+    if ((uint64_t)b->ticket_base + (uint64_t)per_step * steps >= 0xF0000000ull) {
+      STH_TRY(launch_zero_u32(b->ticket, 1, b->stream));
+      b->ticket_base = 0;
+    }
+    BatchArgs a;
+    a.shots = b->shot_args;
+    a.items = b->items;
+    a.total_items = b->total_items;
+    a.ticket = b->ticket;
+    a.c0 = c0;
+    a.c1 = c1;
+    for (uint32_t n = 0; n < steps; n++) {
+      a.ticket_base = b->ticket_base;
+      a.parity = (parity0 + (int)n) & 1;
+      a.c2n = c2 ? c2[n] : 0.0f;
+      STH_TRY(launch_batch(b->npts, mj, nw, G, b->stream, a));
+      b->ticket_base += per_step;
+      (*launches)++;
+    }
+    return STH_OK;
+  }
+  std::vector<Plan> plans(b->shots.size());
+  std::vector<SweepArgs> args(b->shots.size());
+  for (size_t s = 0; s < b->shots.size(); s++) {
+    plans[s] = shot_plan(b, b->shots[s]);
+    memset(&args[s], 0, sizeof(SweepArgs));
+    fill_weights(b->shots[s], true, args[s]);
+  }
+  for (uint32_t n = 0; n < steps; n++) {
+    for (size_t s = 0; s < b->shots.size(); s++) {
+      sth_ctx *c = b->shots[s];
+      const int ub = c->ubuf;
+      c->ubuf = (parity0 + (int)n) & 1;
+      base_args(c, plans[s], args[s]);
+      c->ubuf = ub;
+      args[s].c0 = c0;
+      args[s].c1 = c1;
+      args[s].c2n = c2 ? c2[n] : 0.0f;
+      STH_TRY(launch_sweep((int)b->npts, MODE_LEAP, FMT_F32, plans[s].mj, plans[s].nw,
+                           plans[s].grid, b->stream, args[s]));
+      (*launches)++;
+    }
+  }
+  return STH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sth_batch_destroy(sth_batch *b) {
+  if (!b) return STH_ERR_ARG;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  for (sth_ctx *c : b->shots)
+    if (c) sth_destroy(c);
+  if (b->shot_args) (void)hipFree(b->shot_args);
+  if (b->items) (void)hipFree(b->items);
+  if (b->ticket) (void)hipFree(b->ticket);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->evs) (void)hipEventDestroy(b->evs);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return STH_OK;
+}
+
+int sth_batch_create(const uint32_t *dims, uint32_t nshots, uint32_t npts, uint32_t dtype,
+                     sth_batch **out) {
+  if (!dims || !out || nshots == 0 || nshots > 65536) return STH_ERR_ARG;
+  *out = nullptr;
+  if (dtype != STH_FP32) return STH_ERR_UNSUPPORTED;
+  if (radius_of(npts) < 0) return STH_ERR_ARG;
+  sth_batch *b = new (std::nothrow) sth_batch();
+  if (!b) return STH_ERR_NOMEM;
+  b->npts = npts;
+  b->dtype = dtype;
+  b->tune.xcd_remap = 1;
+  int st = STH_OK;
+  hipError_t e = hipGetDevice(&b->device);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->evs, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc((void **)&b->ticket, 256);
+  if (e == hipSuccess) e = launch_zero_u32(b->ticket, 1, b->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) st = hip_status(e);
+  for (uint32_t s = 0; s < nshots && st == STH_OK; s++) {
+    sth_geom g = {dims[3 * s], dims[3 * s + 1], dims[3 * s + 2], npts, dtype};
+    sth_ctx *c = nullptr;
+    st = sth_create(&g, &c);
+    if (st == STH_OK) b->shots.push_back(c);
+  }
+  if (st != STH_OK) {
+    sth_batch_destroy(b);
+    return st;
+  }
+  *out = b;
+  return STH_OK;
+}
+
+sth_ctx *sth_batch_shot(sth_batch *b, uint32_t shot) {
+  if (!b || shot >= b->shots.size()) return nullptr;
+  return b->shots[shot];
+}
+
+uint32_t sth_batch_size(const sth_batch *b) { return b ? (uint32_t)b->shots.size() : 0; }
+
+int sth_batch_set_tuning(sth_batch *b, const sth_tuning *t, uint32_t planes, uint32_t grid_blocks) {
+  if (!b || !t) return STH_ERR_ARG;
+  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : 4;
+  const int nw = t->waves ? (int)t->waves : 4;
+  if (!tile_supported(mj, nw)) return STH_ERR_ARG;
+  b->tune = *t;
+  b->planes = planes;
+  b->grid_blocks = grid_blocks;
+  b->tables_ok = false;
+  for (sth_ctx *c : b->shots) c->tune = *t;
+  return STH_OK;
+}
+
+int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,
+                         uint32_t *grid_blocks) {
+  if (!b) return STH_ERR_ARG;
+  if (t) {
+    *t = b->tune;
+    t->rows_per_wave = b->tune.rows_per_wave ? b->tune.rows_per_wave : 4;
+    t->waves = b->tune.waves ? b->tune.waves : 4;
+  }
+  if (planes) *planes = b->planes;
+  if (grid_blocks) {
+    const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : 4;
+    const int nw = b->tune.waves ? (int)b->tune.waves : 4;
+    *grid_blocks = persistent_grid(b, mj, nw);
+  }
+  return STH_OK;
+}
+
+int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2,
+                       uint32_t method) {
+  if (!b) return STH_ERR_ARG;
+  const int base = (int)(method & 0xff);
+  const bool graph = (method & STH_BATCH_GRAPH) != 0;
+  if (base != STH_BATCH_PERSISTENT && base != STH_BATCH_SEQUENTIAL) return STH_ERR_ARG;
+  if (steps == 0) return STH_OK;
+  const int parity0 = b->shots[0]->ubuf;
+  int st = build_tables(b);
+  if (st) return st;
+  if ((st = join_shots(b))) return st;
+  if (b->exec) {  // the previous call's graph has been launched before: free it
+    STH_TRY(hipStreamSynchronize(b->stream));
+    STH_TRY(hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  uint64_t launches = 0;
+  if (graph) {
+    hipGraph_t g = nullptr;
+    // a fresh ticket count for every replay of the graph
+    const unsigned saved_base = b->ticket_base;
+    STH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = launch_zero_u32(b->ticket, 1, b->stream);
+    b->ticket_base = 0;
+    if (e == hipSuccess) st = enqueue_steps(b, steps, c0, c1, c2, base, parity0, &launches);
+    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || st != STH_OK || ec != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      b->ticket_base = saved_base;
+      return st != STH_OK ? st : hip_status(e != hipSuccess ? e : ec);
+    }
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_status(e);
+    if (b->timing) STH_TRY(hipEventRecord(b->ev0, b->stream));
+    STH_TRY(hipGraphLaunch(b->exec, b->stream));
+    if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
+  } else {
+    if (b->timing) STH_TRY(hipEventRecord(b->ev0, b->stream));
+    st = enqueue_steps(b, steps, c0, c1, c2, base, parity0, &launches);
+    if (st) return st;
+    if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
+  }
+  if (b->timing) {
+    STH_TRY(hipEventSynchronize(b->ev1));
+    float ms = 0.0f;
+    STH_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->timed_ms += ms;
+    b->launches += launches;
+  }
+  for (sth_ctx *c : b->shots) c->ubuf = (parity0 + (int)steps) & 1;
+  return fork_shots(b);
+}
+
+int sth_batch_sync(sth_batch *b) {
+  if (!b) return STH_ERR_ARG;
+  STH_TRY(hipStreamSynchronize(b->stream));
+  for (sth_ctx *c : b->shots) STH_TRY(hipStreamSynchronize(c->stream));
+  return STH_OK;
+}
+
+int sth_batch_timer_start(sth_batch *b) {
+  if (!b) return STH_ERR_ARG;
+  b->timing = true;
+  b->timed_ms = 0.0;
+  b->launches = 0;
+  return STH_OK;
+}
+
+int sth_batch_timer_read(sth_batch *b, double *ms, uint64_t *launches) {
+  if (!b || !b->timing) return STH_ERR_STATE;
+  b->timing = false;
+  if (ms) *ms = b->timed_ms;
+  if (launches) *launches = b->launches;
+  return STH_OK;
+}
+
+int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *best_planes,
+                       float *best_ms) {
+  if (!b || reps == 0) return STH_ERR_ARG;
+  static const int tiles[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {2, 8}, {4, 8}, {4, 2}, {8, 2}};
+  static const uint32_t plane_opts[] = {0, 128, 64, 32};
+  const sth_tuning saved = b->tune;
+  const uint32_t saved_planes = b->planes, saved_grid = b->grid_blocks;
+  const bool saved_timing = b->timing;
+  sth_tuning bt = saved;
+  uint32_t bp = saved_planes;
+  float bms = 1e30f;
+  for (const auto &tl : tiles)
+    for (uint32_t pl : plane_opts) {
+      sth_tuning t = saved;
+      t.rows_per_wave = (uint32_t)tl[0];
+      t.waves = (uint32_t)tl[1];
+      int st = sth_batch_set_tuning(b, &t, pl, saved_grid);
+      if (st) return st;
+      b->timing = false;
+      st = sth_batch_leapfrog(b, 1, 1.0f, 0.0f, nullptr, STH_BATCH_PERSISTENT);
+      if (st) return st;
+      b->timing = true;
+      b->timed_ms = 0.0;
+      b->launches = 0;
+      st = sth_batch_leapfrog(b, reps, 1.0f, 0.0f, nullptr, STH_BATCH_PERSISTENT);
+      if (st) return st;
+      const float ms = (float)(b->timed_ms / reps);
+      if (ms < bms) {
+        bms = ms;
+        bt = t;
+        bp = pl;
+      }
+    }
+  b->timing = saved_timing;
+  b->timed_ms = 0.0;
+  b->launches = 0;
+  int st = sth_batch_set_tuning(b, &bt, bp, saved_grid);
+  if (st) return st;
+  if (best) *best = bt;
+  if (best_planes) *best_planes = bp;
+  if (best_ms) *best_ms = bms;
+  return STH_OK;
+}
+
+}  // extern "C"

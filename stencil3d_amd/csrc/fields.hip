@@ -47,28 +47,29 @@ __device__ inline double u01(uint64_t key, int i, int j, int k) {
   return (double)(x >> 40) * 0x1.0p-24;
 }
 
-__device__ inline float gen_value(const FillSpec &f, const FieldDesc &d, const Cell &c) {
-  if (!c.cell) return f.kind == FILL_CONST ? f.a : 0.0f;
+// Field value at a cell in double; FP32 and 16-bit storage round it to FP32.
+__device__ inline double gen_value(const FillSpec &f, const FieldDesc &d, const Cell &c) {
+  if (!c.cell) return f.kind == FILL_CONST ? f.a : 0.0;
   switch (f.kind) {
     case FILL_CONST:
       return f.a;
     case FILL_UNIFORM:
       if (!c.interior) return f.halo;
-      return (float)(f.lo + (f.hi - f.lo) * u01(f.key, c.i, c.j, c.k));
+      return f.lo + (f.hi - f.lo) * u01(f.key, c.i, c.j, c.k);
     case FILL_GAUSS: {
-      if (!c.interior) return 0.0f;
+      if (!c.interior) return 0.0;
       const double di = (double)c.i - 0.5 * (double)(d.xs - 1);
       const double dj = (double)c.j - 0.5 * (double)(d.ys - 1);
       const double dk = (double)c.k - 0.5 * (double)(d.zs - 1);
       const double d2 = di * di + dj * dj + dk * dk;
       double val = f.lo * exp(-d2 / (2.0 * f.hi * f.hi));
       if (f.noise != 0.0) val += -f.noise + 2.0 * f.noise * u01(f.key, c.i, c.j, c.k);
-      return (float)val;
+      return val;
     }
     case FILL_POINT:
-      return (c.interior && c.i == f.pi && c.j == f.pj && c.k == f.pk) ? f.a : 0.0f;
+      return (c.interior && c.i == f.pi && c.j == f.pj && c.k == f.pk) ? f.a : 0.0;
   }
-  return 0.0f;
+  return 0.0;
 }
 
 // pass 0: max |value| over cells (atomicMax of FP32 bits); pass 1: store.
@@ -79,9 +80,12 @@ __global__ void fill_kernel(FieldDesc d, FillSpec f, void *dst, int fmt, int e, 
   for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
        idx += (uint64_t)gridDim.x * blockDim.x) {
     const Cell c = decode(d, idx);
-    const float x = gen_value(f, d, c);
+    const double xd = gen_value(f, d, c);
+    const float x = (float)xd;
     if (pass == 0) {
       if (c.cell) mx = fmaxf(mx, fabsf(x));
+    } else if (fmt == FMT_F64) {
+      ((double *)dst)[idx] = xd;
     } else if (fmt == FMT_F32) {
       ((float *)dst)[idx] = x;
     } else if (fmt == FMT_F16) {
@@ -96,7 +100,8 @@ __global__ void fill_kernel(FieldDesc d, FillSpec f, void *dst, int fmt, int e, 
   }
 }
 
-__device__ inline float load_dec(const void *p, uint64_t idx, int fmt, int e) {
+__device__ inline double load_dec(const void *p, uint64_t idx, int fmt, int e) {
+  if (fmt == FMT_F64) return ((const double *)p)[idx];
   if (fmt == FMT_F32) return ((const float *)p)[idx];
   if (fmt == FMT_F16) return Codec<FMT_F16>::dec(((const uint16_t *)p)[idx], e);
   return Codec<FMT_BF16>::dec(((const uint16_t *)p)[idx], e);
@@ -109,7 +114,13 @@ __global__ void check_kernel(FieldDesc d, const void *v, int fmt, int e, float e
   for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < d.elems;
        idx += (uint64_t)gridDim.x * blockDim.x) {
     const Cell c = decode(d, idx);
-    if (c.interior && fabsf(load_dec(v, idx, fmt, e) - expected) > 0.1f) cnt++;
+    if (c.interior) {
+      // FP32: the reference's float test; FP64: the same test in double
+      const double x = load_dec(v, idx, fmt, e);
+      const bool bad = fmt == FMT_F64 ? fabs(x - (double)expected) > (double)0.1f
+                                      : fabsf((float)x - expected) > 0.1f;
+      if (bad) cnt++;
+    }
   }
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(errors, cnt);
@@ -127,9 +138,9 @@ __global__ void norms_kernel(FieldDesc d, const void *x, int fmt, int e, double 
        idx += (uint64_t)gridDim.x * blockDim.x) {
     const Cell c = decode(d, idx);
     if (c.interior) {
-      const float val = load_dec(x, idx, fmt, e);
-      s += (double)val * (double)val;
-      mx = fmaxf(mx, fabsf(val));
+      const double val = load_dec(x, idx, fmt, e);
+      s += val * val;
+      mx = fmaxf(mx, (float)fabs(val));
     }
   }
   for (int o = 32; o > 0; o >>= 1) {

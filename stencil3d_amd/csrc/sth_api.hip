@@ -17,40 +17,15 @@
 #include "stencil_hip.h"
 #include "sth_common.h"
 #include "sweep_dispatch.h"
+#include "sth_internal.h"
 
 using namespace sth;
 
-struct sth_ctx {
-  sth_geom g;
-  sth_layout_info L;
-  int R = 0, eb = 0, fmt = 0;
-  int device = 0;
-  hipStream_t stream = nullptr;
-  void *base[3] = {nullptr, nullptr, nullptr};  // buffers a, b, s
-  int ubuf = 0;                                 // buffer holding logical u
-  // 16-bit storage scale state
-  Q16Slot *qs_dev = nullptr;
-  Q16Slot host_slot{};
-  bool host_dirty = true;
-  int slot = 0;
-  bool halo_zero[3] = {true, true, true};
-  // scratch for reductions
-  uint32_t *maxbits_dev = nullptr;
-  unsigned long long *errors_dev = nullptr;
-  double *partial_dev = nullptr;
-  double *sum_dev = nullptr;
-  // pinned staging for host transfers
-  void *staging = nullptr;
-  size_t staging_bytes = 0;
-  sth_tuning tune{};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timing = false;
-  uint64_t launches = 0;
-};
 
-namespace {
+namespace sth {
+namespace detail {
 
-inline int hip_status(hipError_t e) {
+int hip_status(hipError_t e) {
   if (e == hipSuccess) return STH_OK;
   if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return STH_ERR_NOMEM;
   if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return STH_ERR_NODEV;
@@ -62,6 +37,8 @@ inline int hip_status(hipError_t e) {
     hipError_t e_ = (expr);                      \
     if (e_ != hipSuccess) return hip_status(e_); \
   } while (0)
+
+inline bool is16(const sth_ctx *c) { return c->fmt == FMT_F16 || c->fmt == FMT_BF16; }
 
 inline int buf_of(const sth_ctx *c, sth_field f) {
   return f == STH_FIELD_U ? c->ubuf : f == STH_FIELD_V ? 1 - c->ubuf : 2;
@@ -188,7 +165,7 @@ int transfer_planes(sth_ctx *c, int buf, bool to_device, RowFn row) {
 
 // Current 16-bit scale state of the context (from the device after sweeps).
 int fetch_slot(sth_ctx *c) {
-  if (c->fmt == FMT_F32 || c->host_dirty) return STH_OK;
+  if (!is16(c) || c->host_dirty) return STH_OK;
   STH_TRY(hipMemcpyAsync(&c->host_slot, c->qs_dev + c->slot, sizeof(Q16Slot),
                          hipMemcpyDeviceToHost, c->stream));
   STH_TRY(hipStreamSynchronize(c->stream));
@@ -197,7 +174,7 @@ int fetch_slot(sth_ctx *c) {
 }
 
 int push_slot(sth_ctx *c) {
-  if (c->fmt == FMT_F32 || !c->host_dirty) return STH_OK;
+  if (!is16(c) || !c->host_dirty) return STH_OK;
   Q16Slot img[2];
   img[0] = c->host_slot;
   memset(&img[1], 0, sizeof(Q16Slot));
@@ -215,7 +192,7 @@ int device_fill(sth_ctx *c, int buf, const FillSpec &f, bool halo_zero) {
   const FieldDesc d = desc_of(c);
   int e = 0;
   uint32_t mbits = 0;
-  if (c->fmt != FMT_F32) {
+  if (is16(c)) {
     int st = fetch_slot(c);
     if (st) return st;
     STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
@@ -225,7 +202,7 @@ int device_fill(sth_ctx *c, int buf, const FillSpec &f, bool halo_zero) {
     e = scale_exp((double)u2f(mbits));
   }
   STH_TRY(launch_fill(d, f, c->base[buf], c->fmt, e, 1, c->maxbits_dev, c->stream));
-  if (c->fmt != FMT_F32) {
+  if (is16(c)) {
     // the buffer's max is that of the stored (rounded) values
     STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
     STH_TRY(launch_norms(d, c->base[buf], c->fmt, e, c->partial_dev, c->maxbits_dev, c->sum_dev,
@@ -254,10 +231,6 @@ void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
   a.gain = lap_gain(c->g.npts);
 }
 
-struct Plan {
-  int mj, nw, chunks, chunk_len, ntk, ntj;
-  unsigned grid;
-};
 
 Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   Plan p;
@@ -344,7 +317,10 @@ int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, cons
   return STH_OK;
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace sth
+
+using namespace sth::detail;
 
 extern "C" {
 
@@ -415,7 +391,10 @@ int sth_create(const sth_geom *g, sth_ctx **out) {
   c->L = L;
   c->R = (int)L.radius;
   c->eb = (int)L.elem_bytes;
-  c->fmt = g->dtype == STH_FP32 ? FMT_F32 : g->dtype == STH_FP16 ? FMT_F16 : FMT_BF16;
+  c->fmt = g->dtype == STH_FP32   ? FMT_F32
+           : g->dtype == STH_FP64 ? FMT_F64
+           : g->dtype == STH_FP16 ? FMT_F16
+                                  : FMT_BF16;
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   for (int b = 0; b < 3 && e == hipSuccess; b++) e = hipMalloc(&c->base[b], L.bytes);
@@ -446,7 +425,7 @@ int sth_get_layout(const sth_ctx *c, sth_layout_info *out) {
 
 int sth_init_const(sth_ctx *c, float v0, float s0, float u0) {
   if (!c) return STH_ERR_ARG;
-  if (c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (is16(c)) return STH_ERR_UNSUPPORTED;
   const float vals[3] = {v0, s0, u0};
   const sth_field fs[3] = {STH_FIELD_V, STH_FIELD_S, STH_FIELD_U};
   for (int q = 0; q < 3; q++) {
@@ -499,7 +478,13 @@ int sth_init_point(sth_ctx *c, sth_field fld, uint32_t i, uint32_t j, uint32_t k
   return device_fill(c, buf_of(c, fld), f, true);
 }
 
-int sth_upload(sth_ctx *c, sth_field fld, const float *host, const sth_host_layout *hl) {
+}  // extern "C"
+
+namespace {
+
+// Host (float or double) dense array -> device field of any storage type.
+template <class HT>
+int upload_any(sth_ctx *c, sth_field fld, const HT *host, const sth_host_layout *hl) {
   if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
   int st = check_host_layout(c, hl);
   if (st) return st;
@@ -511,35 +496,40 @@ int sth_upload(sth_ctx *c, sth_field fld, const float *host, const sth_host_layo
   float mx = 0.0f;
   for (int64_t i = -R; i < xs + R; i++)
     for (int64_t j = -R; j < ys + R; j++) {
-      const float *src = host + h.ofs + i * h.im + j * h.jm - R;
+      const HT *src = host + h.ofs + i * h.im + j * h.jm - R;
       const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
       for (int64_t k = 0; k < zs + 2 * R; k++) {
-        const float x = src[k];
+        const float x = (float)src[k];
         mx = std::max(mx, fabsf(x));
-        if (x != 0.0f && (!inrow || k < R || k >= zs + R)) hz = false;
+        if (src[k] != 0 && (!inrow || k < R || k >= zs + R)) hz = false;
       }
     }
   int e = 0;
-  if (c->fmt != FMT_F32) {
+  if (is16(c)) {
     st = fetch_slot(c);
     if (st) return st;
     e = scale_exp((double)mx);
   }
   float qmax = 0.0f;
   st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow) {
-    const float *src = host + h.ofs + i * h.im + j * h.jm - R;
+    const HT *src = host + h.ofs + i * h.im + j * h.jm - R;
+    const int64_t n = zs + 2 * R;
     if (c->fmt == FMT_F32) {
-      memcpy(srow, src, (size_t)(zs + 2 * R) * 4);
+      float *d = (float *)srow;
+      for (int64_t k = 0; k < n; k++) d[k] = (float)src[k];
+    } else if (c->fmt == FMT_F64) {
+      double *d = (double *)srow;
+      for (int64_t k = 0; k < n; k++) d[k] = (double)src[k];
     } else {
       uint16_t *d = (uint16_t *)srow;
-      for (int64_t k = 0; k < zs + 2 * R; k++) {
-        d[k] = enc16(c->fmt, src[k], e);
+      for (int64_t k = 0; k < n; k++) {
+        d[k] = enc16(c->fmt, (float)src[k], e);
         qmax = std::max(qmax, fabsf(dec16(c->fmt, d[k], e)));
       }
     }
   });
   if (st) return st;
-  if (c->fmt != FMT_F32) {
+  if (is16(c)) {
     c->host_slot.exp[buf] = e;
     c->host_slot.maxbits[buf] = f2u(qmax);
   }
@@ -547,7 +537,9 @@ int sth_upload(sth_ctx *c, sth_field fld, const float *host, const sth_host_layo
   return STH_OK;
 }
 
-int sth_download(sth_ctx *c, sth_field fld, float *host, const sth_host_layout *hl) {
+// Device field of any storage type -> host (float or double) dense array.
+template <class HT>
+int download_any(sth_ctx *c, sth_field fld, HT *host, const sth_host_layout *hl) {
   if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
   int st = check_host_layout(c, hl);
   if (st) return st;
@@ -556,22 +548,48 @@ int sth_download(sth_ctx *c, sth_field fld, float *host, const sth_host_layout *
   const HostIdx h = host_idx(c, hl);
   const int64_t R = c->R, zs = c->g.zs;
   const int buf = buf_of(c, fld);
-  const int e = c->fmt == FMT_F32 ? 0 : c->host_slot.exp[buf];
+  const int e = is16(c) ? c->host_slot.exp[buf] : 0;
   return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow) {
-    float *dst = host + h.ofs + i * h.im + j * h.jm - R;
+    HT *dst = host + h.ofs + i * h.im + j * h.jm - R;
+    const int64_t n = zs + 2 * R;
     if (c->fmt == FMT_F32) {
-      memcpy(dst, srow, (size_t)(zs + 2 * R) * 4);
+      const float *s = (const float *)srow;
+      for (int64_t k = 0; k < n; k++) dst[k] = (HT)s[k];
+    } else if (c->fmt == FMT_F64) {
+      const double *s = (const double *)srow;
+      for (int64_t k = 0; k < n; k++) dst[k] = (HT)s[k];
     } else {
       const uint16_t *s = (const uint16_t *)srow;
-      for (int64_t k = 0; k < zs + 2 * R; k++) dst[k] = dec16(c->fmt, s[k], e);
+      for (int64_t k = 0; k < n; k++) dst[k] = (HT)dec16(c->fmt, s[k], e);
     }
   });
+}
+
+}  // namespace
+
+extern "C" {
+
+int sth_upload(sth_ctx *c, sth_field fld, const float *host, const sth_host_layout *hl) {
+  return upload_any<float>(c, fld, host, hl);
+}
+
+int sth_download(sth_ctx *c, sth_field fld, float *host, const sth_host_layout *hl) {
+  return download_any<float>(c, fld, host, hl);
+}
+
+int sth_upload_f64(sth_ctx *c, sth_field fld, const double *host, const sth_host_layout *hl) {
+  if (c && is16(c)) return STH_ERR_UNSUPPORTED;
+  return upload_any<double>(c, fld, host, hl);
+}
+
+int sth_download_f64(sth_ctx *c, sth_field fld, double *host, const sth_host_layout *hl) {
+  return download_any<double>(c, fld, host, hl);
 }
 
 int sth_upload_raw16(sth_ctx *c, sth_field fld, const uint16_t *host, int exp,
                      const sth_host_layout *hl) {
   if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
-  if (c->fmt == FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (!is16(c)) return STH_ERR_UNSUPPORTED;
   int st = check_host_layout(c, hl);
   if (st) return st;
   st = fetch_slot(c);
@@ -601,7 +619,7 @@ int sth_upload_raw16(sth_ctx *c, sth_field fld, const uint16_t *host, int exp,
 int sth_download_raw16(sth_ctx *c, sth_field fld, uint16_t *host, int *exp, float *maxabs,
                        const sth_host_layout *hl) {
   if (!c || !host || fld < 0 || fld > 2) return STH_ERR_ARG;
-  if (c->fmt == FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (!is16(c)) return STH_ERR_UNSUPPORTED;
   int st = check_host_layout(c, hl);
   if (st) return st;
   st = fetch_slot(c);
@@ -618,13 +636,13 @@ int sth_download_raw16(sth_ctx *c, sth_field fld, uint16_t *host, int *exp, floa
 
 int sth_iterate(sth_ctx *c, uint32_t iterations) {
   if (!c) return STH_ERR_ARG;
-  if (c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (is16(c)) return STH_ERR_UNSUPPORTED;
   return run_ref(c, make_plan(c, c->tune), iterations);
 }
 
 int sth_leapfrog(sth_ctx *c, uint32_t steps, float c0, float c1, const float *c2) {
   if (!c) return STH_ERR_ARG;
-  if (c->fmt != FMT_F32 && !(c->halo_zero[0] && c->halo_zero[1])) return STH_ERR_UNSUPPORTED;
+  if (is16(c) && !(c->halo_zero[0] && c->halo_zero[1])) return STH_ERR_UNSUPPORTED;
   return run_leap(c, make_plan(c, c->tune), steps, c0, c1, c2);
 }
 
@@ -639,7 +657,7 @@ int sth_check(sth_ctx *c, float expected, uint64_t *errors) {
   int st = fetch_slot(c);
   if (st) return st;
   const int buf = 1 - c->ubuf;
-  const int e = c->fmt == FMT_F32 ? 0 : c->host_slot.exp[buf];
+  const int e = is16(c) ? c->host_slot.exp[buf] : 0;
   unsigned long long cnt = 0;
   STH_TRY(hipMemsetAsync(c->errors_dev, 0, 8, c->stream));
   STH_TRY(launch_check(desc_of(c), c->base[buf], c->fmt, e, expected, c->errors_dev, c->stream));
@@ -654,7 +672,7 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
   int st = fetch_slot(c);
   if (st) return st;
   const int buf = buf_of(c, fld);
-  const int e = c->fmt == FMT_F32 ? 0 : c->host_slot.exp[buf];
+  const int e = is16(c) ? c->host_slot.exp[buf] : 0;
   uint32_t mb = 0;
   double ss = 0.0;
   STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
@@ -689,7 +707,7 @@ int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
 
 int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, float *best_ms) {
   if (!c || reps == 0) return STH_ERR_ARG;
-  if (!leapfrog && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (!leapfrog && is16(c)) return STH_ERR_UNSUPPORTED;
   static const int tiles[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {1, 8}, {2, 8},
                                  {4, 8}, {8, 8}, {2, 2}, {4, 2}, {8, 2}};
   static const int chunk_opts[] = {0, 1, 2, 4, 8};

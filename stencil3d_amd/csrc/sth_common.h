@@ -15,7 +15,8 @@ inline int radius_of(uint32_t npts) {
   return npts == 7 ? 1 : npts == 13 ? 2 : npts == 25 ? 4 : -1;
 }
 inline int elem_bytes_of(uint32_t dtype) {
-  return dtype == STH_FP32 ? 4 : (dtype == STH_FP16 || dtype == STH_BF16) ? 2 : -1;
+  return dtype == STH_FP32 ? 4 : dtype == STH_FP64 ? 8
+       : (dtype == STH_FP16 || dtype == STH_BF16) ? 2 : -1;
 }
 
 // Device layout: halo = R on every side; rows padded to a multiple of 256 B

@@ -32,10 +32,13 @@ hipError_t launch_tile(int mj, int nw, unsigned grid, hipStream_t st, const Swee
 template <int NPTS>
 hipError_t launch_sweep_n(int mode, int fmt, int mj, int nw, unsigned grid, hipStream_t st,
                           const SweepArgs &a) {
-  if (mode == MODE_REF && fmt == FMT_F32)
-    return launch_tile<NPTS, MODE_REF, FMT_F32>(mj, nw, grid, st, a);
+  if (mode == MODE_REF) {
+    if (fmt == FMT_F32) return launch_tile<NPTS, MODE_REF, FMT_F32>(mj, nw, grid, st, a);
+    if (fmt == FMT_F64) return launch_tile<NPTS, MODE_REF, FMT_F64>(mj, nw, grid, st, a);
+  }
   if (mode == MODE_LEAP) {
     if (fmt == FMT_F32) return launch_tile<NPTS, MODE_LEAP, FMT_F32>(mj, nw, grid, st, a);
+    if (fmt == FMT_F64) return launch_tile<NPTS, MODE_LEAP, FMT_F64>(mj, nw, grid, st, a);
     if (fmt == FMT_F16) return launch_tile<NPTS, MODE_LEAP, FMT_F16>(mj, nw, grid, st, a);
     if (fmt == FMT_BF16) return launch_tile<NPTS, MODE_LEAP, FMT_BF16>(mj, nw, grid, st, a);
   }

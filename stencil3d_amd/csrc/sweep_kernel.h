@@ -29,7 +29,7 @@
 namespace sth {
 
 enum { MODE_REF = 0, MODE_LEAP = 1 };
-enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2 };
+enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2, FMT_F64 = 3 };
 
 // Per-step scale state of the 16-bit storage form (a ring of 3 slots: step n
 // reads slot n%3, publishes slot (n+1)%3 and clears the max of slot (n+2)%3).
@@ -60,12 +60,14 @@ struct Codec;
 template <>
 struct Codec<FMT_F32> {
   using T = float;
+  using A = float;
   __device__ static inline float dec(T x, int) { return x; }
   __device__ static inline T enc(float x, int) { return x; }
 };
 template <>
 struct Codec<FMT_F16> {
   using T = uint16_t;
+  using A = float;
   __device__ static inline float dec(T q, int e) {
     return ldexpf((float)__builtin_bit_cast(_Float16, q), e);
   }
@@ -76,6 +78,7 @@ struct Codec<FMT_F16> {
 template <>
 struct Codec<FMT_BF16> {
   using T = uint16_t;
+  using A = float;
   __device__ static inline float dec(T q, int e) {
     return ldexpf(__uint_as_float((uint32_t)q << 16), e);
   }
@@ -86,6 +89,17 @@ struct Codec<FMT_BF16> {
     return (T)(b >> 16);
   }
 };
+
+template <>
+struct Codec<FMT_F64> {
+  using T = double;
+  using A = double;
+  __device__ static inline double dec(T x, int) { return x; }
+  __device__ static inline T enc(double x, int) { return x; }
+};
+
+__device__ inline float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Smallest e with B*2^-e < 2^15 (frexp exponent - 15); B <= 0 -> 0.
 __host__ __device__ inline int scale_exp(double B) {
@@ -106,14 +120,15 @@ __device__ inline int xcd_remap(int bid, int nb) {
 }
 
 template <int NPTS, int MODE, int FMT, int MJ, int NW>
-__device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, int ib, int ie,
-                                           float (*lds)[(NW * MJ + 2 * Terms<NPTS>::R) *
-                                                        (64 + 2 * Terms<NPTS>::R)],
-                                           float *red) {
+__device__ __forceinline__ void sweep_tile(
+    const SweepArgs &a, int tk, int tj, int ib, int ie,
+    typename Codec<FMT>::A (*lds)[(NW * MJ + 2 * Terms<NPTS>::R) * (64 + 2 * Terms<NPTS>::R)],
+    float *red) {
   using Tm = Terms<NPTS>;
   constexpr int R = Tm::R, T = Tm::T;
   using CD = Codec<FMT>;
   using ST = typename CD::T;
+  using AT = typename CD::A;  // arithmetic type: FP32, or FP64 for FP64 storage
   constexpr int TJ = NW * MJ, LW = 64 + 2 * R;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // j-halo rows loaded per wave
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
@@ -126,7 +141,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, i
 
   // 16-bit storage: exponents of u, v, s and of the output.
   int eu = 0, ev = 0, es = 0, eo = 0;
-  if constexpr (FMT != FMT_F32) {
+  if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
     const Q16Slot &cur = a.qs[a.slot];
     eu = cur.exp[a.ubuf];
     ev = cur.exp[a.vbuf];
@@ -177,7 +192,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, i
   }
 
   // Register queue: q[m][t] = u at plane i - R + t.
-  float q[MJ][2 * R + 1];
+  AT q[MJ][2 * R + 1];
 #pragma unroll
   for (int t = 0; t < 2 * R; t++) {
     const ST *Up = Ub + (int64_t)(ib - R + t) * P;
@@ -204,7 +219,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, i
 
   float mx = 0.0f;
   for (int i = ib; i < ie; i++) {
-    float *L = lds[(i - ib) & 1];
+    AT *L = lds[(i - ib) & 1];
     ST cs[MJ], cv[MJ], chj[HJ], chk;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
@@ -229,34 +244,34 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, i
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       const int lr = w * MJ + m + R, col = lane + R;
-      auto pair = [&](int ax, int r) -> float {
+      auto pair = [&](int ax, int r) -> AT {
         if (ax == 0) return q[m][R - r] + q[m][R + r];
         if (ax == 2) return L[lr * LW + col - r] + L[lr * LW + col + r];
-        const float lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R] : L[(lr - r) * LW + col];
-        const float hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R] : L[(lr + r) * LW + col];
+        const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R] : L[(lr - r) * LW + col];
+        const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R] : L[(lr + r) * LW + col];
         return lo + hi;
       };
-      const float u0 = q[m][R];
-      const float s0 = CD::dec(cs[m], es);
-      const float v0 = CD::dec(cv[m], ev);
-      float out;
+      const AT u0 = q[m][R];
+      const AT s0 = CD::dec(cs[m], es);
+      const AT v0 = CD::dec(cv[m], ev);
+      AT out;
       if constexpr (MODE == MODE_REF) {
         // stencil_N_ref_1: p = coef0*u0 + sum c*pair (the first product
         // rounded, the rest fused), then v0 - A*u0 + s0*p.
-        float p = fmaf(a.coef0, u0, a.w[0] * pair(Tm::axis[0], Tm::r[0]));
+        AT p = fmaT((AT)a.coef0, u0, (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]));
 #pragma unroll
-        for (int t = 1; t < T; t++) p = fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
-        out = fmaf(s0, p, fmaf(a.mA, u0, v0));
+        for (int t = 1; t < T; t++) p = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
+        out = fmaT(s0, p, fmaT((AT)a.mA, u0, v0));
       } else {
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+        AT Lp = (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
-        for (int t = 1; t < T; t++) Lp = fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        out = fmaf(a.c2n, s0, fmaf(a.c1, Lp, fmaf(a.c0, u0, -v0)));
+        for (int t = 1; t < T; t++) Lp = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        out = fmaT((AT)a.c2n, s0, fmaT((AT)a.c1, Lp, fmaT((AT)a.c0, u0, -v0)));
       }
       if (valid[m]) {
         const ST qo = CD::enc(out, eo);
         Vp[off[m]] = qo;
-        if constexpr (FMT != FMT_F32) mx = fmaxf(mx, fabsf(CD::dec(qo, eo)));
+        if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) mx = fmaxf(mx, fabsf(CD::dec(qo, eo)));
       }
     }
 #pragma unroll
@@ -265,7 +280,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, i
       for (int t = 0; t < 2 * R; t++) q[m][t] = q[m][t + 1];
   }
 
-  if constexpr (FMT != FMT_F32) {
+  if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) red[w] = mx;
@@ -282,7 +297,7 @@ __device__ __forceinline__ void sweep_tile(const SweepArgs &a, int tk, int tj, i
 // Publishes the next slot of the 16-bit scale ring (block 0, thread 0).
 template <int FMT>
 __device__ inline void q16_publish(const SweepArgs &a) {
-  if constexpr (FMT != FMT_F32) {
+  if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       const Q16Slot &cur = a.qs[a.slot];
       Q16Slot &nxt = a.qs[(a.slot + 1) % 3];
@@ -304,7 +319,7 @@ __device__ inline void q16_publish(const SweepArgs &a) {
 template <int NPTS, int MODE, int FMT, int MJ, int NW>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
   constexpr int R = Terms<NPTS>::R;
-  __shared__ float lds[2][(NW * MJ + 2 * R) * (64 + 2 * R)];
+  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 + 2 * R)];
   __shared__ float red[NW];
   q16_publish<FMT>(a);
   int bid = blockIdx.x;

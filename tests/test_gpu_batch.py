@@ -1,0 +1,113 @@
+"""GPU tests of the batching form: many uneven shots advanced by one
+persistent launch per step (or one plain launch per shot per step), with or
+without a hipGraph around the time loop.  Every shot must be bit-identical to
+the CPU oracle's LEAPFROG on the same fields, and every method to every other."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+DIMS = [(16, 20, 70), (33, 17, 65), (9, 40, 130), (24, 24, 24), (50, 8, 64)]
+
+
+def shot_fields(orc, dims, npts, seed):
+    R = orc.radius(npts)
+    out = []
+    for q, d in enumerate(dims):
+        g = orc.Grid(*d, R, R, R)
+        a = orc.fill_uniform(g, seed + q, 0)
+        b = orc.fill_uniform(g, seed + q, 1)
+        s = orc.fill_uniform(g, seed + q, 2)
+        out.append((g, a, b, s))
+    return out
+
+
+def load(batch, fields, R):
+    for q, (g, a, b, s) in enumerate(fields):
+        sh = batch.shot(q)
+        sh.upload("u", a, R)
+        sh.upload("v", b, R)
+        sh.upload("s", s, R)
+
+
+def coeffs(orc, npts, c1=0.2):
+    return float(np.float32(2.0 - orc.lap_gain(npts) * c1)), float(np.float32(c1))
+
+
+@pytest.mark.parametrize("method,graph", [(0, False), (1, False), (0, True), (1, True)])
+@pytest.mark.parametrize("npts", [13, 25])
+def test_batch_matches_oracle(gpu, orc, method, graph, npts):
+    R = orc.radius(npts)
+    fields = shot_fields(orc, DIMS, npts, 10)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-2, 1e-2, 6).astype(np.float32)
+    with gpu.Batch(DIMS, npts=npts) as bt:
+        bt.set_tuning(2, 4, planes=7)
+        load(bt, fields, R)
+        bt.leapfrog(6, c0, c1, c2, method=method, graph=graph)
+        got = [(bt.shot(q).download("u", R), bt.shot(q).download("v", R))
+               for q in range(len(DIMS))]
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, 6)
+        wu, wv = (a, b) if newest == 0 else (b, a)
+        I = g.interior
+        assert np.array_equal(got[q][0][I].view(np.uint32), wu[I].view(np.uint32)), q
+        assert np.array_equal(got[q][1][I].view(np.uint32), wv[I].view(np.uint32)), q
+
+
+def test_persistent_equals_sequential_config_shapes(gpu, orc):
+    """Eight shots with sides from the survey config's set, 12 steps: the
+    persistent, sequential and graphed runs agree bit for bit."""
+    rng = np.random.default_rng(2)
+    dims = [tuple(int(x) for x in rng.choice([96, 128, 192], 3)) for _ in range(8)]
+    npts, R = 13, 2
+    c0, c1 = coeffs(orc, npts)
+    results = []
+    for method, graph in ((0, False), (1, False), (0, True)):
+        with gpu.Batch(dims, npts=npts) as bt:
+            for q in range(len(dims)):
+                bt.shot(q).init_gaussian("u", 1.0, 6.0, 1e-3, 1, q)
+                bt.shot(q).init_gaussian("v", 1.0, 6.0, 1e-3, 1, 100 + q)
+                bt.shot(q).init_point("s", dims[q][0] // 2, dims[q][1] // 3, dims[q][2] // 4, 1.0)
+            bt.leapfrog(12, c0, c1, np.full(12, 1e-3, np.float32), method=method, graph=graph)
+            results.append([bt.shot(q).download("u", R) for q in range(len(dims))])
+    for r in results[1:]:
+        for q in range(len(dims)):
+            assert np.array_equal(r[q].view(np.uint32), results[0][q].view(np.uint32))
+
+
+def test_graph_replays_match_eager(gpu, orc):
+    """Repeated graphed calls, with eager work and synchronisation between
+    them, give the same fields as the same calls made eagerly."""
+    npts, R = 13, 2
+    c0, c1 = coeffs(orc, npts)
+    outs = []
+    for graph in (False, True):
+        fields = shot_fields(orc, DIMS[:3], npts, 40)
+        with gpu.Batch(DIMS[:3], npts=npts) as bt:
+            load(bt, fields, R)
+            for rep in range(3):
+                bt.leapfrog(4, c0, c1, np.full(4, 1e-3 * (rep + 1), np.float32), graph=graph)
+                bt.sync()
+                # eager work between replays: norms on every shot
+                for q in range(3):
+                    bt.shot(q).norms("u")
+            outs.append([bt.shot(q).download("u", R) for q in range(3)])
+    for q in range(3):
+        assert np.array_equal(outs[0][q].view(np.uint32), outs[1][q].view(np.uint32))
+
+
+@pytest.mark.parametrize("tile,planes", [((1, 4), 0), ((4, 4), 16), ((8, 2), 5), ((2, 8), 33)])
+def test_batch_tunings_agree(gpu, orc, tile, planes):
+    npts, R = 13, 2
+    fields = shot_fields(orc, DIMS, npts, 70)
+    c0, c1 = coeffs(orc, npts)
+    with gpu.Batch(DIMS, npts=npts) as bt:
+        bt.set_tuning(*tile, planes=planes)
+        load(bt, fields, R)
+        bt.leapfrog(3, c0, c1)
+        got = [bt.shot(q).download("u", R) for q in range(len(DIMS))]
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, None, 3)
+        want = a if newest == 0 else b
+        assert np.array_equal(got[q][g.interior].view(np.uint32), want[g.interior].view(np.uint32))

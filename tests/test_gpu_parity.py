@@ -189,3 +189,34 @@ def test_norms_match_oracle(gpu, orc):
     omx, ol2 = orc.norms(g, x)
     assert mx == omx
     assert abs(l2 - ol2) <= 1e-12 * ol2
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_fp64_ref_and_leapfrog_bit_exact(gpu, orc, npts):
+    """FP64 storage: REF with the FP32 constants promoted, and LEAPFROG, both
+    bit-identical to the oracle's FP64 sweeps."""
+    shape = (13, 22, 45)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, s, v = (orc.fill_uniform(g, 8, f).astype(np.float64) for f in range(3))
+    with gpu.Stencil(*shape, npts=npts, dtype="fp64") as ctx:
+        for f, x in (("u", u), ("s", s), ("v", v)):
+            ctx.upload(f, x, R)
+        ctx.iterate(4)
+        got = ctx.download("v", R, dtype=np.float64)
+    want = v.copy()
+    orc.ref_sweep_f64(npts, g, want, s, u, 4, coeff_mode=0)
+    assert np.array_equal(got[g.interior], want[g.interior])
+
+    c1 = float(np.float32(0.16))
+    c0 = float(np.float32(2.0 - orc.lap_gain(npts) * 0.16))
+    c2 = np.array([1e-3, -2e-3, 3e-3], np.float32)
+    with gpu.Stencil(*shape, npts=npts, dtype="fp64") as ctx:
+        for f, x in (("u", u), ("s", s), ("v", v)):
+            ctx.upload(f, x, R)
+        ctx.leapfrog(3, c0, c1, c2)
+        got = ctx.download("u", R, dtype=np.float64)
+    a, b = u.copy(), v.copy()
+    newest = orc.leapfrog_f64(npts, g, a, b, s, c0, c1, c2.astype(np.float64), 3)
+    want = a if newest == 0 else b
+    assert np.array_equal(got[g.interior], want[g.interior])
