@@ -47,9 +47,10 @@ def leap_coeffs(npts, c1):
     return 2.0 - gain * c1, c1
 
 
-def run_throughput(S, n, dtype, steps, warmup, tuning=None):
+def run_throughput(S, n, dtype, steps, warmup, tuning=None, copy_bytes=0):
     """Returns wall seconds of `steps` timed steps, the HIP-event kernel time
-    and launch count, and the tuning used."""
+    and launch count, the tuning used, the u norms and (copy_bytes > 0) the
+    mean time of a plain copy kernel of copy_bytes."""
     c0, c1 = leap_coeffs(NPTS, 0.16)
     with S.Stencil(n, n, n, npts=NPTS, dtype=dtype) as ctx:
         ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 0)
@@ -66,7 +67,8 @@ def run_throughput(S, n, dtype, steps, warmup, tuning=None):
         t1 = time.perf_counter()
         mx, l2 = ctx.norms("u")
         used = ctx.get_tuning()
-    return t1 - t0, kms, launches, used, (mx, l2)
+        copy_ms = ctx.stream_copy(copy_bytes, 5) if copy_bytes else None
+    return t1 - t0, kms, launches, used, (mx, l2), copy_ms
 
 
 def ricker_c2(steps, dt=1e-3, f=15.0):
@@ -164,7 +166,10 @@ def main():
     pts = float(n) ** 3
 
     log(f"[bench] fp32 {NPTS}-pt {n}^3: {W} warm-up + {K} timed steps")
-    wall, kms, launches, tune, nrm = run_throughput(S, n, "fp32", K, W)
+    copy_bytes = (n * n * n * 4) // 16 * 16
+    wall, kms, launches, tune, nrm, copy_ms = run_throughput(S, n, "fp32", K, W,
+                                                             copy_bytes=copy_bytes)
+    copy_gbs = 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9
     gpts = pts * K / wall / 1e9
     per_launch_s = kms / launches * 1e-3
     achieved = 16.0 * pts / per_launch_s / 1e9
@@ -174,7 +179,7 @@ def main():
     if not args.no_16bit:
         for dt in ("fp16", "bf16"):
             log(f"[bench] {dt}: {W} warm-up + {K} timed steps")
-            w2, k2, l2, t2, _ = run_throughput(S, n, dt, K, W)
+            w2, k2, l2, t2, _, _ = run_throughput(S, n, dt, K, W)
             g2 = pts * K / w2 / 1e9
             a2 = 8.0 * pts / (k2 / l2 * 1e-3) / 1e9
             extra[dt] = {"gpts": round(g2, 3), "ms_per_step": round(w2 / K * 1e3, 4),
@@ -218,7 +223,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / PEAK_GBS, 4),
                      "traffic": None, "kernel_ms": round(kms / launches, 5),
-                     "bytes_per_launch_alg": 16.0 * pts},
+                     "bytes_per_launch_alg": 16.0 * pts,
+                     "copy_kernel_gbs": round(copy_gbs, 1),
+                     "copy_kernel_bytes": 2 * copy_bytes},
         "cpu_baseline": cpu,
         "norms_u": {"max": nrm[0], "l2": nrm[1]},
     }

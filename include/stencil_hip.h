@@ -174,6 +174,12 @@ int sth_autotune(sth_ctx *ctx, int leapfrog, uint32_t reps, sth_tuning *best, fl
  * and returns the summed kernel time (ms) and the number of launches. */
 int sth_timer_start(sth_ctx *ctx);
 int sth_timer_read(sth_ctx *ctx, double *kernel_ms, uint64_t *launches);
+/* Diagnostic with no counterpart in the reference: a plain streaming copy of
+ * `bytes` (at most one field) from the u field into the v field, `reps`
+ * times; returns the mean ms per copy.  Overwrites v.  The copy moves 2*bytes
+ * of HBM traffic per launch: the achievable-bandwidth context for the sweep's
+ * roofline, and the calibration kernel for the traffic counters. */
+int sth_stream_copy(sth_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms);
 
 /* ---- batching: many independent grids ("shots") advanced together ------ */
 /* The reference runs one grid per process; a batch holds nshots grids of

@@ -91,6 +91,7 @@ SIGNATURES = {
     "sth_autotune": (C.c_int, [_VP, C.c_int, C.c_uint32, _P(Tuning), _P(C.c_float)]),
     "sth_timer_start": (C.c_int, [_VP]),
     "sth_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
+    "sth_stream_copy": (C.c_int, [_VP, C.c_uint64, C.c_uint32, _P(C.c_double)]),
     "sth_batch_create": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P(_VP)]),
     "sth_batch_destroy": (C.c_int, [_VP]),
@@ -312,6 +313,12 @@ class Stencil:
 
     def timer_start(self):
         _ok(lib().sth_timer_start(self.handle), "sth_timer_start")
+
+    def stream_copy(self, nbytes, reps=5):
+        """Plain copy kernel u -> v of nbytes; returns mean ms per copy."""
+        ms = C.c_double()
+        _ok(lib().sth_stream_copy(self.handle, nbytes, reps, C.byref(ms)), "sth_stream_copy")
+        return ms.value
 
     def timer_read(self):
         ms, n = C.c_double(), C.c_uint64()

@@ -36,5 +36,6 @@ hipError_t launch_check(const FieldDesc &d, const void *v, int fmt, int e, float
 hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, double *partial,
                         uint32_t *maxbits, double *out, hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st);
+hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 
 }  // namespace sth

@@ -180,6 +180,23 @@ __global__ void zero_u32(uint32_t *p, int n) {
   if (t < n) p[t] = 0u;
 }
 
+// Plain streaming copy (16-byte vectors, grid-stride): the reference point
+// for the HBM bandwidth a simple kernel reaches, and the calibration kernel
+// for the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void stream_copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                   uint64_t n4) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    dst[t] = src[t];
+}
+
+hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
+  const uint64_t n4 = bytes / 16;
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(8192), dim3(256), 0, st, (const float4 *)src,
+                     (float4 *)dst, n4);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill(const FieldDesc &d, const FillSpec &f, void *dst, int fmt, int e, int pass,
                        uint32_t *maxbits, hipStream_t st) {
   hipLaunchKernelGGL(fill_kernel, dim3(kReduceBlocks), dim3(256), 0, st, d, f, dst, fmt, e, pass,

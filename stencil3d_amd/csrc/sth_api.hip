@@ -749,6 +749,20 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   return STH_OK;
 }
 
+int sth_stream_copy(sth_ctx *c, uint64_t bytes, uint32_t reps, double *ms) {
+  if (!c || reps == 0 || bytes == 0 || bytes > c->L.bytes || bytes % 16 != 0) return STH_ERR_ARG;
+  void *src = c->base[c->ubuf], *dst = c->base[1 - c->ubuf];
+  STH_TRY(launch_stream_copy(src, dst, bytes, c->stream));  // warm-up
+  STH_TRY(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t r = 0; r < reps; r++) STH_TRY(launch_stream_copy(src, dst, bytes, c->stream));
+  STH_TRY(hipEventRecord(c->ev1, c->stream));
+  STH_TRY(hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  STH_TRY(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  if (ms) *ms = (double)t / reps;
+  return STH_OK;
+}
+
 int sth_timer_start(sth_ctx *c) {
   if (!c) return STH_ERR_ARG;
   STH_TRY(hipEventRecord(c->ev0, c->stream));
