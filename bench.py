@@ -99,6 +99,55 @@ def run_error(S, dtype, n=256, steps=1000):
     return float(err.max() / np.abs(r).max()), float(np.sqrt((err ** 2).sum() / (r ** 2).sum()))
 
 
+def run_batch(S, steps=100, nshots=64, seed=2):
+    """Batching form: nshots independent 13-point FP32 shots, sides drawn from
+    seed 2 uniformly among {96,128,192,256,320,384} per axis, a Ricker source at
+    a random interior point of each, c dt/dx = 0.4; `steps` steps in one call,
+    timed for the persistent launch (in a hipGraph), and for one plain launch
+    per shot per step (eager and in a hipGraph).  Tile and planes per item come
+    from the autotuner."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dims = [tuple(int(x) for x in rng.choice([96, 128, 192, 256, 320, 384], 3))
+            for _ in range(nshots)]
+    srcs = [tuple(int(rng.integers(0, d)) for d in dd) for dd in dims]
+    pts = float(sum(a * b * c for a, b, c in dims))
+    c1 = 0.16
+    c0 = 2.0 - 6.0 * c1
+    c2 = ricker_c2(steps)
+    out = {"shots": nshots, "npts": 13, "steps": steps, "points_per_step": int(pts)}
+    with S.Batch(dims, npts=13) as bt:
+        tuned, tms = bt.autotune(reps=2)
+        out["tuning"] = bt.get_tuning()
+        for name, method, graph in (("persistent_graph", S.BATCH_PERSISTENT, True),
+                                    ("persistent_eager", S.BATCH_PERSISTENT, False),
+                                    ("sequential_eager", S.BATCH_SEQUENTIAL, False),
+                                    ("sequential_graph", S.BATCH_SEQUENTIAL, True)):
+            for q, (d, sp) in enumerate(zip(dims, srcs)):
+                sh = bt.shot(q)
+                sh.init_point("u", 0, 0, 0, 0.0)
+                sh.init_point("v", 0, 0, 0, 0.0)
+                sh.init_point("s", *sp, 1.0)
+            bt.leapfrog(2, c0, c1, None, method=method, graph=graph)  # warm-up
+            bt.sync()
+            bt.timer_start()
+            t0 = time.perf_counter()
+            bt.leapfrog(steps, c0, c1, c2, method=method, graph=graph)
+            bt.sync()
+            wall = time.perf_counter() - t0
+            ms, launches = bt.timer_read()
+            out[name] = {"gpu_ms_per_step": round(ms / steps, 4),
+                         "wall_ms_per_step": round(wall * 1e3 / steps, 4),
+                         "gpts": round(pts * steps / (ms * 1e-3) / 1e9, 2),
+                         "hbm_gbs_alg": round(16.0 * pts * steps / (ms * 1e-3) / 1e9, 1),
+                         "launches": launches}
+            log(f"[bench] batch {name}: {ms / steps:.3f} ms/step, "
+                f"{out[name]['gpts']} Gpts/s")
+    p, q = out["persistent_graph"]["gpu_ms_per_step"], out["sequential_eager"]["gpu_ms_per_step"]
+    out["persistent_speedup_vs_sequential"] = round(q / p, 3)
+    return out
+
+
 def cpu_baseline(n, sweeps):
     """hstencil3d (the reference, 25-point, iterate_3d, OpenMP static) on this
     host, or the repo's CPU restatement when the reference build is absent."""
@@ -153,6 +202,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-16bit", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -195,6 +245,10 @@ def main():
                                             "setup": "25-pt 256^3, c dt/dx=0.4, 1000 steps, "
                                                      "Ricker 15 Hz at the centre"}
                 log(f"[bench] {dt}: max-rel {mr:.3e}, L2-rel {lr:.3e}")
+
+    if not args.no_batch:
+        log("[bench] batch: 64 shots x 13 points, 100 steps")
+        extra["batch"] = run_batch(S)
 
     cpu = None
     if not args.no_cpu:

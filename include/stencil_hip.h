@@ -137,8 +137,8 @@ int sth_download(sth_ctx *ctx, sth_field f, float *host, const sth_host_layout *
  * value), download from any storage (16-bit values decoded exactly). */
 int sth_upload_f64(sth_ctx *ctx, sth_field f, const double *host, const sth_host_layout *hl);
 int sth_download_f64(sth_ctx *ctx, sth_field f, double *host, const sth_host_layout *hl);
-/* 16-bit storage: raw stored values and the field's scale exponent
- * (value = stored * 2^exp).  upload_raw sets the field's max from the data. */
+/* 16-bit storage: raw stored values q and the field's scale exponent
+ * (value = q * 2^exp); maxabs = max |q| (upload_raw sets it from the data). */
 int sth_upload_raw16(sth_ctx *ctx, sth_field f, const uint16_t *host, int exp,
                      const sth_host_layout *hl);
 int sth_download_raw16(sth_ctx *ctx, sth_field f, uint16_t *host, int *exp, float *maxabs,

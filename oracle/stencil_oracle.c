@@ -328,7 +328,7 @@ float orc_quantize(int fmt, const orc_geom *g, const float *x, uint16_t *q, int 
   float mx = 0.0f;
   for (int64_t t = 0; t < n; t++) {
     q[t] = orc_f32_to_16(fmt, ldexpf(x[t], -e));
-    float d = fabsf(ldexpf(orc_16_to_f32(fmt, q[t]), e));
+    float d = fabsf(orc_16_to_f32(fmt, q[t]));
     if (d > mx) mx = d;
   }
   return mx;
@@ -337,6 +337,17 @@ float orc_quantize(int fmt, const orc_geom *g, const float *x, uint16_t *q, int 
 void orc_dequantize(int fmt, const orc_geom *g, const uint16_t *q, int e, float *x) {
   const int64_t n = orc_gsize(g);
   for (int64_t t = 0; t < n; t++) x[t] = ldexpf(orc_16_to_f32(fmt, q[t]), e);
+}
+
+/* Output exponent of a 16-bit LEAPFROG step: orc_scale_exp of the bound
+ * (|c0| + |c1| gain) M_u + M_v + |c2n| M_s with M = max|q| 2^e (double). */
+static int q16_out_exp(const int exps[3], const float maxs[3], int cu, int cv, float c0,
+                       float c1, float c2n, double gain) {
+  const double Mu = ldexp((double)maxs[cu], exps[cu]);
+  const double Mv = ldexp((double)maxs[cv], exps[cv]);
+  const double Ms = ldexp((double)maxs[2], exps[2]);
+  return orc_scale_exp((fabs((double)c0) + fabs((double)c1) * gain) * Mu + Mv +
+                       fabs((double)c2n) * Ms);
 }
 
 int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
@@ -353,31 +364,30 @@ int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t
   for (int n = 0; n < steps; n++) {
     const uint16_t *u = bufs[cur];
     uint16_t *v = bufs[1 - cur];
-    const int eu = exps[cur], ev = exps[1 - cur], es = exps[2];
     const float c2n = c2 ? c2[n] : 0.0f;
-    const double B = (fabs((double)c0) + fabs((double)c1) * gain) * (double)maxs[cur] +
-                     (double)maxs[1 - cur] + fabs((double)c2n) * (double)maxs[2];
-    const int eo = orc_scale_exp(B);
+    const int eo = q16_out_exp(exps, maxs, cur, 1 - cur, c0, c1, c2n, gain);
+    /* the update on stored numbers: powers of two folded into the coefficients */
+    const float k0 = ldexpf(c0, exps[cur] - eo), k1 = ldexpf(c1, exps[cur] - eo);
+    const float k2 = ldexpf(c2n, exps[2] - eo), kv = ldexpf(1.0f, exps[1 - cur] - eo);
     float mx = 0.0f;
 #pragma omp parallel for schedule(static) reduction(max : mx)
     for (int64_t i = 0; i < g->xs; i++)
       for (int64_t j = 0; j < g->ys; j++)
         for (int64_t k = 0; k < g->zs; k++) {
           const int64_t c = o + i * im + j * jm + k;
-          float L = tb.w[0] * (ldexpf(orc_16_to_f32(fmt, u[c - tb.off[0]]), eu) +
-                               ldexpf(orc_16_to_f32(fmt, u[c + tb.off[0]]), eu));
+          float L = tb.w[0] * (orc_16_to_f32(fmt, u[c - tb.off[0]]) +
+                               orc_16_to_f32(fmt, u[c + tb.off[0]]));
           for (int t = 1; t < T; t++)
             L = fmaf(tb.w[t],
-                     ldexpf(orc_16_to_f32(fmt, u[c - tb.off[t]]), eu) +
-                         ldexpf(orc_16_to_f32(fmt, u[c + tb.off[t]]), eu),
+                     orc_16_to_f32(fmt, u[c - tb.off[t]]) + orc_16_to_f32(fmt, u[c + tb.off[t]]),
                      L);
-          const float u0 = ldexpf(orc_16_to_f32(fmt, u[c]), eu);
-          const float v0 = ldexpf(orc_16_to_f32(fmt, v[c]), ev);
-          const float s0 = ldexpf(orc_16_to_f32(fmt, s[c]), es);
-          const float out = fmaf(c2n, s0, fmaf(c1, L, fmaf(c0, u0, -v0)));
-          const uint16_t q = orc_f32_to_16(fmt, ldexpf(out, -eo));
+          const float u0 = orc_16_to_f32(fmt, u[c]);
+          const float v0 = orc_16_to_f32(fmt, v[c]);
+          const float s0 = orc_16_to_f32(fmt, s[c]);
+          const float out = fmaf(k2, s0, fmaf(k1, L, fmaf(k0, u0, -(v0 * kv))));
+          const uint16_t q = orc_f32_to_16(fmt, out);
           v[c] = q;
-          const float d = fabsf(ldexpf(orc_16_to_f32(fmt, q), eo));
+          const float d = fabsf(orc_16_to_f32(fmt, q));
           if (d > mx) mx = d;
         }
     exps[1 - cur] = eo;

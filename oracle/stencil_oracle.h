@@ -98,16 +98,20 @@ float orc_16_to_f32(int fmt, uint16_t h);
  * the smallest e with B*2^-e < 2^15 (e from frexp); B <= 0 -> 0. */
 int orc_scale_exp(double B);
 /* Quantize an FP32 field into 16-bit storage with exponent e
- * (q = rne16(x * 2^-e)) over the whole allocation; returns the max |decoded|. */
+ * (q = rne16(x * 2^-e)) over the whole allocation; returns max |q|. */
 float orc_quantize(int fmt, const orc_geom *g, const float *x, uint16_t *q, int e);
 void orc_dequantize(int fmt, const orc_geom *g, const uint16_t *q, int e, float *x);
-/* LEAPFROG with 16-bit storage, FP32 arithmetic.  exps[0..2] / maxs[0..2] are
- * the exponent and the max |decoded value| of buffers a, b and s (in/out).
- * Before each step the output exponent is orc_scale_exp(B) with
- *   B = (|c0| + |c1|*gain)*M_u + M_v + |c2[n]|*M_s   (double);
- * values decode as q*2^e, the update is the FP32 LEAPFROG order above, the
- * result is stored as rne16(out*2^-e_out) and the buffer's max is the max of
- * the decoded stored values over the interior.  Halos must be zero. */
+/* LEAPFROG with 16-bit storage, FP32 arithmetic.  A buffer holds numbers q
+ * with value q*2^e; exps[0..2] / maxs[0..2] are e and max |q| of buffers a, b
+ * and s (in/out).  Before each step the output exponent is orc_scale_exp(B),
+ *   B = (|c0| + |c1|*gain)*M_u + M_v + |c2[n]|*M_s,  M = max|q|*2^e (double);
+ * the update runs on the stored numbers with the powers of two folded into
+ * the coefficients (exact scalings):
+ *   L on q_u as in the FP32 order, out = fmaf(k2, q_s, fmaf(k1, L,
+ *   fmaf(k0, q_u, -(q_v*kv)))),  k0 = c0 2^(eu-eo), k1 = c1 2^(eu-eo),
+ *   k2 = c2[n] 2^(es-eo), kv = 2^(ev-eo);
+ * q_out = rne16(out) and the buffer's max is max |q_out| over the interior.
+ * Halos must be zero. */
 int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
                      const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
                      const float *c2, int steps, int nthreads);

@@ -223,8 +223,8 @@ int fork_shots(sth_batch *b) {
 // Queues `steps` steps on the batch stream (eagerly or while capturing).
 int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2, int method,
                   int parity0, uint64_t *launches) {
-  const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : 4;
-  const int nw = b->tune.waves ? (int)b->tune.waves : 4;
+  const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : kDefaultRows;
+  const int nw = b->tune.waves ? (int)b->tune.waves : kDefaultWaves;
   if (method == STH_BATCH_PERSISTENT) {
     const unsigned G = persistent_grid(b, mj, nw);
     const unsigned per_step = (unsigned)b->total_items + G;
@@ -341,8 +341,8 @@ uint32_t sth_batch_size(const sth_batch *b) { return b ? (uint32_t)b->shots.size
 
 int sth_batch_set_tuning(sth_batch *b, const sth_tuning *t, uint32_t planes, uint32_t grid_blocks) {
   if (!b || !t) return STH_ERR_ARG;
-  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : 4;
-  const int nw = t->waves ? (int)t->waves : 4;
+  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : kDefaultRows;
+  const int nw = t->waves ? (int)t->waves : kDefaultWaves;
   if (!tile_supported(mj, nw)) return STH_ERR_ARG;
   b->tune = *t;
   b->planes = planes;
@@ -357,13 +357,13 @@ int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,
   if (!b) return STH_ERR_ARG;
   if (t) {
     *t = b->tune;
-    t->rows_per_wave = b->tune.rows_per_wave ? b->tune.rows_per_wave : 4;
-    t->waves = b->tune.waves ? b->tune.waves : 4;
+    t->rows_per_wave = b->tune.rows_per_wave ? b->tune.rows_per_wave : kDefaultRows;
+    t->waves = b->tune.waves ? b->tune.waves : kDefaultWaves;
   }
   if (planes) *planes = b->planes;
   if (grid_blocks) {
-    const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : 4;
-    const int nw = b->tune.waves ? (int)b->tune.waves : 4;
+    const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : kDefaultRows;
+    const int nw = b->tune.waves ? (int)b->tune.waves : kDefaultWaves;
     *grid_blocks = persistent_grid(b, mj, nw);
   }
   return STH_OK;

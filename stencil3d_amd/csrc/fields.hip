@@ -89,9 +89,9 @@ __global__ void fill_kernel(FieldDesc d, FillSpec f, void *dst, int fmt, int e, 
     } else if (fmt == FMT_F32) {
       ((float *)dst)[idx] = x;
     } else if (fmt == FMT_F16) {
-      ((uint16_t *)dst)[idx] = Codec<FMT_F16>::enc(x, e);
+      ((uint16_t *)dst)[idx] = Codec<FMT_F16>::enc(ldexpf(x, -e));
     } else {
-      ((uint16_t *)dst)[idx] = Codec<FMT_BF16>::enc(x, e);
+      ((uint16_t *)dst)[idx] = Codec<FMT_BF16>::enc(ldexpf(x, -e));
     }
   }
   if (pass == 0) {
@@ -103,8 +103,8 @@ __global__ void fill_kernel(FieldDesc d, FillSpec f, void *dst, int fmt, int e, 
 __device__ inline double load_dec(const void *p, uint64_t idx, int fmt, int e) {
   if (fmt == FMT_F64) return ((const double *)p)[idx];
   if (fmt == FMT_F32) return ((const float *)p)[idx];
-  if (fmt == FMT_F16) return Codec<FMT_F16>::dec(((const uint16_t *)p)[idx], e);
-  return Codec<FMT_BF16>::dec(((const uint16_t *)p)[idx], e);
+  if (fmt == FMT_F16) return ldexpf(Codec<FMT_F16>::dec(((const uint16_t *)p)[idx]), e);
+  return ldexpf(Codec<FMT_BF16>::dec(((const uint16_t *)p)[idx]), e);
 }
 
 // check(): interior cells with |v - expected| > 0.1f (iterators.cpp:319).

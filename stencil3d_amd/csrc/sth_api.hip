@@ -203,9 +203,9 @@ int device_fill(sth_ctx *c, int buf, const FillSpec &f, bool halo_zero) {
   }
   STH_TRY(launch_fill(d, f, c->base[buf], c->fmt, e, 1, c->maxbits_dev, c->stream));
   if (is16(c)) {
-    // the buffer's max is that of the stored (rounded) values
+    // the buffer's max is max |q| of the stored numbers (exponent 0 decode)
     STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
-    STH_TRY(launch_norms(d, c->base[buf], c->fmt, e, c->partial_dev, c->maxbits_dev, c->sum_dev,
+    STH_TRY(launch_norms(d, c->base[buf], c->fmt, 0, c->partial_dev, c->maxbits_dev, c->sum_dev,
                          c->stream));
     STH_TRY(hipMemcpyAsync(&mbits, c->maxbits_dev, 4, hipMemcpyDeviceToHost, c->stream));
     STH_TRY(hipStreamSynchronize(c->stream));
@@ -234,18 +234,20 @@ void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
 
 Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   Plan p;
-  p.mj = t.rows_per_wave ? (int)t.rows_per_wave : 4;
-  p.nw = t.waves ? (int)t.waves : 4;
+  p.mj = t.rows_per_wave ? (int)t.rows_per_wave : kDefaultRows;
+  p.nw = t.waves ? (int)t.waves : kDefaultWaves;
   const int TJ = p.mj * p.nw;
   p.ntk = ((int)c->g.zs + 63) / 64;
   p.ntj = ((int)c->g.ys + TJ - 1) / TJ;
   const int xs = (int)c->g.xs;
   int ch = (int)t.chunks;
   if (ch == 0) {
-    // enough workgroups to cover the 256 CUs several times, but chunks no
-    // shorter than 4R planes (each chunk re-reads 2R planes of u)
+    // i-chunks of about 64 planes, and at least 8192 workgroups where the
+    // grid allows it: many short-lived workgroups drain evenly (no tail of a
+    // few long ones); a chunk re-reads 2R planes of u, so none is shorter
+    // than max(8, 4R) planes.
     const int tiles = p.ntk * p.ntj;
-    ch = (2048 + tiles - 1) / tiles;
+    ch = std::max((xs + 63) / 64, (8192 + tiles - 1) / tiles);
     ch = std::min(ch, std::max(1, xs / std::max(8, 4 * c->R)));
     ch = std::max(ch, 1);
   }
@@ -524,7 +526,7 @@ int upload_any(sth_ctx *c, sth_field fld, const HT *host, const sth_host_layout 
       uint16_t *d = (uint16_t *)srow;
       for (int64_t k = 0; k < n; k++) {
         d[k] = enc16(c->fmt, (float)src[k], e);
-        qmax = std::max(qmax, fabsf(dec16(c->fmt, d[k], e)));
+        qmax = std::max(qmax, fabsf(dec16(c->fmt, d[k], 0)));
       }
     }
   });
@@ -604,7 +606,7 @@ int sth_upload_raw16(sth_ctx *c, sth_field fld, const uint16_t *host, int exp,
     memcpy(srow, src, (size_t)(zs + 2 * R) * 2);
     const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
     for (int64_t k = 0; k < zs + 2 * R; k++) {
-      const float x = dec16(c->fmt, src[k], exp);
+      const float x = dec16(c->fmt, src[k], 0);
       qmax = std::max(qmax, fabsf(x));
       if (x != 0.0f && (!inrow || k < R || k >= zs + R)) hz = false;
     }
@@ -688,8 +690,8 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
   if (!c || !t) return STH_ERR_ARG;
-  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : 4;
-  const int nw = t->waves ? (int)t->waves : 4;
+  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : kDefaultRows;
+  const int nw = t->waves ? (int)t->waves : kDefaultWaves;
   if (!tile_supported(mj, nw) || t->chunks > 64) return STH_ERR_ARG;
   c->tune = *t;
   return STH_OK;
@@ -710,7 +712,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   if (!leapfrog && is16(c)) return STH_ERR_UNSUPPORTED;
   static const int tiles[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {1, 8}, {2, 8},
                                  {4, 8}, {8, 8}, {2, 2}, {4, 2}, {8, 2}};
-  static const int chunk_opts[] = {0, 1, 2, 4, 8};
+  static const int chunk_opts[] = {0, 4, 8, 16, 32};
   const sth_tuning saved = c->tune;
   const bool saved_timing = c->timing;
   c->timing = false;

@@ -42,6 +42,11 @@ struct sth_ctx {
 namespace sth {
 namespace detail {
 
+// Default tile: 2 rows per wave x 4 waves (8 x 64 points per workgroup),
+// the fastest family measured for 7-25 points at 1040^3 on MI355X.
+constexpr int kDefaultRows = 2;
+constexpr int kDefaultWaves = 4;
+
 // Launch plan of one sweep: tile (rows per wave x waves), i-chunks, grid.
 struct Plan {
   int mj, nw, chunks, chunk_len, ntk, ntj;

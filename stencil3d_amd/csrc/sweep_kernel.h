@@ -33,8 +33,9 @@ enum { FMT_F32 = 0, FMT_F16 = 1, FMT_BF16 = 2, FMT_F64 = 3 };
 
 // Per-step scale state of the 16-bit storage form (a ring of 3 slots: step n
 // reads slot n%3, publishes slot (n+1)%3 and clears the max of slot (n+2)%3).
+// A buffer holds values q * 2^exp; maxbits is max |q| (FP32 bits).
 struct Q16Slot {
-  uint32_t maxbits[3];  // max |decoded value| of buffers a, b, s as FP32 bits
+  uint32_t maxbits[3];  // max |stored value| of buffers a, b, s as FP32 bits
   int32_t exp[3];       // power-of-two exponents of buffers a, b, s
 };
 
@@ -57,33 +58,36 @@ struct SweepArgs {
 
 template <int FMT>
 struct Codec;
+// Storage codecs.  16-bit values are the stored numbers q themselves (exact
+// in FP32); a field's power-of-two scale 2^e is folded into the update's
+// coefficients instead of into every load and store.
 template <>
 struct Codec<FMT_F32> {
   using T = float;
   using A = float;
-  __device__ static inline float dec(T x, int) { return x; }
-  __device__ static inline T enc(float x, int) { return x; }
+  __device__ static inline float dec(T x) { return x; }
+  __device__ static inline T enc(float x) { return x; }
 };
 template <>
 struct Codec<FMT_F16> {
   using T = uint16_t;
   using A = float;
-  __device__ static inline float dec(T q, int e) {
-    return ldexpf((float)__builtin_bit_cast(_Float16, q), e);
-  }
-  __device__ static inline T enc(float x, int e) {
-    return __builtin_bit_cast(uint16_t, (_Float16)ldexpf(x, -e));  // v_cvt_f16_f32, RNE
+  __device__ static inline float dec(T q) { return (float)__builtin_bit_cast(_Float16, q); }
+  __device__ static inline T enc(float x) {
+    // The update rounds to FP32 first, then to binary16 (two roundings, as
+    // the oracle does).  The empty asm keeps the compiler from fusing the
+    // last fma with the conversion into v_fma_mix*_f16 (one rounding).
+    __asm__ volatile("" : "+v"(x));
+    return __builtin_bit_cast(uint16_t, (_Float16)x);  // v_cvt_f16_f32, RNE
   }
 };
 template <>
 struct Codec<FMT_BF16> {
   using T = uint16_t;
   using A = float;
-  __device__ static inline float dec(T q, int e) {
-    return ldexpf(__uint_as_float((uint32_t)q << 16), e);
-  }
-  __device__ static inline T enc(float x, int e) {
-    uint32_t b = __float_as_uint(ldexpf(x, -e));
+  __device__ static inline float dec(T q) { return __uint_as_float((uint32_t)q << 16); }
+  __device__ static inline T enc(float x) {
+    uint32_t b = __float_as_uint(x);
     if ((b & 0x7fffffffu) > 0x7f800000u) return (T)((b >> 16) | 0x40u);
     b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even
     return (T)(b >> 16);
@@ -94,8 +98,8 @@ template <>
 struct Codec<FMT_F64> {
   using T = double;
   using A = double;
-  __device__ static inline double dec(T x, int) { return x; }
-  __device__ static inline T enc(double x, int) { return x; }
+  __device__ static inline double dec(T x) { return x; }
+  __device__ static inline T enc(double x) { return x; }
 };
 
 __device__ inline float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -107,6 +111,19 @@ __host__ __device__ inline int scale_exp(double B) {
   int E;
   frexp(B, &E);
   return E - 15;
+}
+
+// Exponent of the field a LEAPFROG step writes: from the bound
+// |out| <= (|c0| + |c1| gain) M_u + M_v + |c2n| M_s, with M = max|q| * 2^e.
+__host__ __device__ inline int q16_out_exp(const Q16Slot &cur, int ubuf, int vbuf, float c0,
+                                           float c1, float c2n, double gain) {
+  auto M = [&](int b) {
+    float m;
+    memcpy(&m, &cur.maxbits[b], 4);
+    return ldexp((double)m, cur.exp[b]);
+  };
+  const double gmul = fabs((double)c0) + fabs((double)c1) * gain;
+  return scale_exp(gmul * M(ubuf) + M(vbuf) + fabs((double)c2n) * M(2));
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; this bijection gives
@@ -139,19 +156,19 @@ __device__ __forceinline__ void sweep_tile(
   const int k0 = tk * 64, j0 = tj * TJ;
   const int64_t P = a.plane, Pj = a.pitch;
 
-  // 16-bit storage: exponents of u, v, s and of the output.
-  int eu = 0, ev = 0, es = 0, eo = 0;
+  // LEAPFROG coefficients.  16-bit storage works on the stored numbers q
+  // (value = q 2^e): with the output exponent eo the update becomes
+  //   q_out = c2n 2^(es-eo) q_s + c1 2^(eu-eo) L(q_u) + c0 2^(eu-eo) q_u - 2^(ev-eo) q_v,
+  // the same FP32 operations on values scaled by exact powers of two.
+  AT kc0 = (AT)a.c0, kc1 = (AT)a.c1, kc2 = (AT)a.c2n, kv = (AT)1;
   if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
     const Q16Slot &cur = a.qs[a.slot];
-    eu = cur.exp[a.ubuf];
-    ev = cur.exp[a.vbuf];
-    es = cur.exp[2];
-    const double Mu = (double)__uint_as_float(cur.maxbits[a.ubuf]);
-    const double Mv = (double)__uint_as_float(cur.maxbits[a.vbuf]);
-    const double Ms = (double)__uint_as_float(cur.maxbits[2]);
-    const double gmul = fabs((double)a.c0) + fabs((double)a.c1) * a.gain;
-    const double B = gmul * Mu + Mv + fabs((double)a.c2n) * Ms;
-    eo = scale_exp(B);
+    const int eu = cur.exp[a.ubuf], ev = cur.exp[a.vbuf], es = cur.exp[2];
+    const int eo = q16_out_exp(cur, a.ubuf, a.vbuf, a.c0, a.c1, a.c2n, a.gain);
+    kc0 = ldexpf(a.c0, eu - eo);
+    kc1 = ldexpf(a.c1, eu - eo);
+    kc2 = ldexpf(a.c2n, es - eo);
+    kv = ldexpf(1.0f, ev - eo);
   }
 
   // Tile base: element (i = 0, j0 - R, k0 - R); all in-tile offsets >= 0.
@@ -197,7 +214,7 @@ __device__ __forceinline__ void sweep_tile(
   for (int t = 0; t < 2 * R; t++) {
     const ST *Up = Ub + (int64_t)(ib - R + t) * P;
 #pragma unroll
-    for (int m = 0; m < MJ; m++) q[m][t] = CD::dec(Up[off[m]], eu);
+    for (int m = 0; m < MJ; m++) q[m][t] = CD::dec(Up[off[m]]);
   }
   ST nu[MJ], ns[MJ], nv[MJ], nhj[HJ], nhk;
   auto prefetch = [&](int i) {
@@ -223,7 +240,7 @@ __device__ __forceinline__ void sweep_tile(
     ST cs[MJ], cv[MJ], chj[HJ], chk;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      q[m][2 * R] = CD::dec(nu[m], eu);
+      q[m][2 * R] = CD::dec(nu[m]);
       cs[m] = ns[m];
       cv[m] = nv[m];
     }
@@ -236,8 +253,8 @@ __device__ __forceinline__ void sweep_tile(
     for (int m = 0; m < MJ; m++) L[(w * MJ + m + R) * LW + lane + R] = q[m][R];
 #pragma unroll
     for (int h = 0; h < HJ; h++)
-      if (hrow[h] >= 0) L[hrow[h] * LW + lane + R] = CD::dec(chj[h], eu);
-    if (has_kh) L[khpos] = CD::dec(chk, eu);
+      if (hrow[h] >= 0) L[hrow[h] * LW + lane + R] = CD::dec(chj[h]);
+    if (has_kh) L[khpos] = CD::dec(chk);
     __syncthreads();
 
     ST *Vp = Vb + (int64_t)i * P;
@@ -252,8 +269,8 @@ __device__ __forceinline__ void sweep_tile(
         return lo + hi;
       };
       const AT u0 = q[m][R];
-      const AT s0 = CD::dec(cs[m], es);
-      const AT v0 = CD::dec(cv[m], ev);
+      const AT s0 = CD::dec(cs[m]);
+      const AT v0 = CD::dec(cv[m]);
       AT out;
       if constexpr (MODE == MODE_REF) {
         // stencil_N_ref_1: p = coef0*u0 + sum c*pair (the first product
@@ -266,12 +283,15 @@ __device__ __forceinline__ void sweep_tile(
         AT Lp = (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
         for (int t = 1; t < T; t++) Lp = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        out = fmaT((AT)a.c2n, s0, fmaT((AT)a.c1, Lp, fmaT((AT)a.c0, u0, -v0)));
+        if constexpr (FMT == FMT_F16 || FMT == FMT_BF16)
+          out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -(v0 * kv))));
+        else
+          out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -v0)));
       }
       if (valid[m]) {
-        const ST qo = CD::enc(out, eo);
+        const ST qo = CD::enc(out);
         Vp[off[m]] = qo;
-        if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) mx = fmaxf(mx, fabsf(CD::dec(qo, eo)));
+        if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) mx = fmaxf(mx, fabsf(CD::dec(qo)));
       }
     }
 #pragma unroll
@@ -302,11 +322,7 @@ __device__ inline void q16_publish(const SweepArgs &a) {
       const Q16Slot &cur = a.qs[a.slot];
       Q16Slot &nxt = a.qs[(a.slot + 1) % 3];
       Q16Slot &clr = a.qs[(a.slot + 2) % 3];
-      const double Mu = (double)__uint_as_float(cur.maxbits[a.ubuf]);
-      const double Mv = (double)__uint_as_float(cur.maxbits[a.vbuf]);
-      const double Ms = (double)__uint_as_float(cur.maxbits[2]);
-      const double gmul = fabs((double)a.c0) + fabs((double)a.c1) * a.gain;
-      nxt.exp[a.vbuf] = scale_exp(gmul * Mu + Mv + fabs((double)a.c2n) * Ms);
+      nxt.exp[a.vbuf] = q16_out_exp(cur, a.ubuf, a.vbuf, a.c0, a.c1, a.c2n, a.gain);
       nxt.exp[a.ubuf] = cur.exp[a.ubuf];
       nxt.exp[2] = cur.exp[2];
       nxt.maxbits[a.ubuf] = cur.maxbits[a.ubuf];
