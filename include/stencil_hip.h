@@ -90,9 +90,10 @@ typedef struct {
  * iterators.cpp:42-89).  Zero fields select the built-in defaults. */
 typedef struct {
   uint32_t rows_per_wave; /* j rows held in registers by one wave: 1, 2, 4, 8 */
-  uint32_t waves;         /* waves per workgroup: 1, 2, 4, 8 */
+  uint32_t waves;         /* waves per workgroup: 2, 4, 8 */
   uint32_t chunks;        /* i-range split: 1..64 chunks per (j,k) tile */
   uint32_t xcd_remap;     /* 1: keep neighbouring tiles on one XCD (default), 0: off */
+  uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
 } sth_tuning;
 
 typedef struct sth_ctx sth_ctx;
@@ -143,6 +144,15 @@ int sth_upload_raw16(sth_ctx *ctx, sth_field f, const uint16_t *host, int exp,
                      const sth_host_layout *hl);
 int sth_download_raw16(sth_ctx *ctx, sth_field f, uint16_t *host, int *exp, float *maxabs,
                        const sth_host_layout *hl);
+
+/* Field I/O (checkpoint / restart; no counterpart in the reference, whose
+ * fields live only in memory): writes dir/u.raw, v.raw, s.raw -- the storage
+ * elements of the dense box with an R-wide halo, C order, k fastest -- and
+ * dir/manifest.json (geometry, dtype, 16-bit exponents).  The directory must
+ * exist.  sth_load restores them into a context of the same geometry and
+ * dtype (STH_ERR_ARG otherwise). */
+int sth_save(sth_ctx *ctx, const char *dir);
+int sth_load(sth_ctx *ctx, const char *dir);
 
 /* ---- the sweep -------------------------------------------------------------- */
 /* ITERATE replacement (iterate_3d, iterators.cpp:130-157): `iterations` REF

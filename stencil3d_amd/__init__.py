@@ -50,7 +50,7 @@ class HostLayout(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("rows_per_wave", C.c_uint32), ("waves", C.c_uint32), ("chunks", C.c_uint32),
-                ("xcd_remap", C.c_uint32)]
+                ("xcd_remap", C.c_uint32), ("vec", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -81,6 +81,8 @@ SIGNATURES = {
     "sth_upload_raw16": (C.c_int, [_VP, C.c_int, _P(C.c_uint16), C.c_int, _P(HostLayout)]),
     "sth_download_raw16": (C.c_int, [_VP, C.c_int, _P(C.c_uint16), _P(C.c_int),
                                      _P(C.c_float), _P(HostLayout)]),
+    "sth_save": (C.c_int, [_VP, C.c_char_p]),
+    "sth_load": (C.c_int, [_VP, C.c_char_p]),
     "sth_iterate": (C.c_int, [_VP, C.c_uint32]),
     "sth_leapfrog": (C.c_int, [_VP, C.c_uint32, C.c_float, C.c_float, _P(C.c_float)]),
     "sth_sync": (C.c_int, [_VP]),
@@ -269,6 +271,13 @@ class Stencil:
                                      C.byref(mx), C.byref(hl)), "sth_download_raw16")
         return out, e.value, mx.value
 
+    def save(self, directory):
+        """Writes u, v, s and a JSON manifest into an existing directory."""
+        _ok(lib().sth_save(self.handle, os.fsencode(directory)), "sth_save")
+
+    def load(self, directory):
+        _ok(lib().sth_load(self.handle, os.fsencode(directory)), "sth_load")
+
     # -- the sweep
     def iterate(self, iterations):
         """ITERATE replacement: REF sweeps, v updated in place."""
@@ -296,8 +305,8 @@ class Stencil:
         return mx.value, l2.value
 
     # -- schedule and timing
-    def set_tuning(self, rows_per_wave=0, waves=0, chunks=0, xcd_remap=1):
-        t = Tuning(rows_per_wave, waves, chunks, xcd_remap)
+    def set_tuning(self, rows_per_wave=0, waves=0, chunks=0, xcd_remap=1, vec=0):
+        t = Tuning(rows_per_wave, waves, chunks, xcd_remap, vec)
         _ok(lib().sth_set_tuning(self.handle, C.byref(t)), "sth_set_tuning")
 
     def get_tuning(self):
@@ -364,8 +373,8 @@ class Batch:
     def shot(self, q) -> Stencil:
         return self._shots[q]
 
-    def set_tuning(self, rows_per_wave=0, waves=0, planes=0, grid_blocks=0, xcd_remap=1):
-        t = Tuning(rows_per_wave, waves, 0, xcd_remap)
+    def set_tuning(self, rows_per_wave=0, waves=0, planes=0, grid_blocks=0, xcd_remap=1, vec=0):
+        t = Tuning(rows_per_wave, waves, 0, xcd_remap, vec)
         _ok(lib().sth_batch_set_tuning(self._h, C.byref(t), planes, grid_blocks),
             "sth_batch_set_tuning")
 

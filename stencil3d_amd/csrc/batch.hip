@@ -41,10 +41,10 @@ struct BatchArgs {
 // Every block takes tickets until it draws one past the end, so a step
 // consumes exactly total_items + gridDim.x tickets: step n of a launch
 // sequence starts at ticket_base = n * (total_items + grid).
-template <int NPTS, int MODE, int FMT, int MJ, int NW>
+template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void batch_kernel(const BatchArgs b) {
   constexpr int R = Terms<NPTS>::R;
-  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 + 2 * R)];
+  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 * V + 2 * R)];
   __shared__ float red[NW];
   __shared__ int sh_item;
   for (;;) {
@@ -65,27 +65,29 @@ __global__ __launch_bounds__(64 * NW) void batch_kernel(const BatchArgs b) {
     a.c2n = b.c2n;
     const int ib = it.ch * a.chunk_len;
     const int ie = min(a.xs, ib + a.chunk_len);
-    sweep_tile<NPTS, MODE, FMT, MJ, NW>(a, it.tk, it.tj, ib, ie, lds, red);
+    sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, it.tk, it.tj, ib, ie, lds, red);
     __syncthreads();  // LDS is reused by the next item
   }
 }
 
 template <int NPTS>
-const void *batch_fn(int mj, int nw) {
-#define STH_BATCH_FN(MJ, NW) \
-  if (mj == MJ && nw == NW) return (const void *)&batch_kernel<NPTS, MODE_LEAP, FMT_F32, MJ, NW>;
+const void *batch_fn(int mj, int nw, int vec) {
+#define STH_BATCH_FN(MJ, NW, VV)                 \
+  if (mj == MJ && nw == NW && vec == VV)         \
+    return (const void *)&batch_kernel<NPTS, MODE_LEAP, FMT_F32, MJ, NW, VV>;
   STH_TILE_LIST(STH_BATCH_FN)
 #undef STH_BATCH_FN
   return nullptr;
 }
 
 template <int NPTS>
-hipError_t launch_batch_n(int mj, int nw, unsigned grid, hipStream_t st, const BatchArgs &b) {
-#define STH_BATCH_CASE(MJ, NW)                                                                  \
-  if (mj == MJ && nw == NW) {                                                                   \
-    hipLaunchKernelGGL((batch_kernel<NPTS, MODE_LEAP, FMT_F32, MJ, NW>), dim3(grid),            \
-                       dim3(64 * NW), 0, st, b);                                                \
-    return hipGetLastError();                                                                   \
+hipError_t launch_batch_n(int mj, int nw, int vec, unsigned grid, hipStream_t st,
+                          const BatchArgs &b) {
+#define STH_BATCH_CASE(MJ, NW, VV)                                                     \
+  if (mj == MJ && nw == NW && vec == VV) {                                             \
+    hipLaunchKernelGGL((batch_kernel<NPTS, MODE_LEAP, FMT_F32, MJ, NW, VV>), dim3(grid), \
+                       dim3(64 * NW), 0, st, b);                                       \
+    return hipGetLastError();                                                          \
   }
   STH_TILE_LIST(STH_BATCH_CASE)
 #undef STH_BATCH_CASE
@@ -118,19 +120,27 @@ struct sth_batch {
 
 namespace {
 
-hipError_t launch_batch(uint32_t npts, int mj, int nw, unsigned grid, hipStream_t st,
+hipError_t launch_batch(uint32_t npts, int mj, int nw, int vec, unsigned grid, hipStream_t st,
                         const BatchArgs &b) {
-  if (npts == 7) return launch_batch_n<7>(mj, nw, grid, st, b);
-  if (npts == 13) return launch_batch_n<13>(mj, nw, grid, st, b);
-  if (npts == 25) return launch_batch_n<25>(mj, nw, grid, st, b);
+  if (npts == 7) return launch_batch_n<7>(mj, nw, vec, grid, st, b);
+  if (npts == 13) return launch_batch_n<13>(mj, nw, vec, grid, st, b);
+  if (npts == 25) return launch_batch_n<25>(mj, nw, vec, grid, st, b);
   return hipErrorInvalidValue;
 }
 
-const void *batch_kernel_fn(uint32_t npts, int mj, int nw) {
-  if (npts == 7) return batch_fn<7>(mj, nw);
-  if (npts == 13) return batch_fn<13>(mj, nw);
-  if (npts == 25) return batch_fn<25>(mj, nw);
+const void *batch_kernel_fn(uint32_t npts, int mj, int nw, int vec) {
+  if (npts == 7) return batch_fn<7>(mj, nw, vec);
+  if (npts == 13) return batch_fn<13>(mj, nw, vec);
+  if (npts == 25) return batch_fn<25>(mj, nw, vec);
   return nullptr;
+}
+
+struct TileSel {
+  int mj, nw, vec;
+};
+TileSel tile_of(const sth_tuning &t) {
+  return {t.rows_per_wave ? (int)t.rows_per_wave : kDefaultRows,
+          t.waves ? (int)t.waves : kDefaultWaves, t.vec ? (int)t.vec : kDefaultVec};
 }
 
 #define STH_TRY(expr)                            \
@@ -149,10 +159,11 @@ Plan shot_plan(const sth_batch *b, const sth_ctx *c) {
   return make_plan(c, t);
 }
 
-unsigned persistent_grid(const sth_batch *b, int mj, int nw) {
+unsigned persistent_grid(const sth_batch *b, const TileSel &ts) {
   if (b->grid_blocks) return b->grid_blocks;
   int per_cu = 0;
-  const void *fn = batch_kernel_fn(b->npts, mj, nw);
+  const int nw = ts.nw;
+  const void *fn = batch_kernel_fn(b->npts, ts.mj, ts.nw, ts.vec);
   if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * nw, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
@@ -223,10 +234,9 @@ int fork_shots(sth_batch *b) {
 // Queues `steps` steps on the batch stream (eagerly or while capturing).
 int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2, int method,
                   int parity0, uint64_t *launches) {
-  const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : kDefaultRows;
-  const int nw = b->tune.waves ? (int)b->tune.waves : kDefaultWaves;
+  const TileSel ts = tile_of(b->tune);
   if (method == STH_BATCH_PERSISTENT) {
-    const unsigned G = persistent_grid(b, mj, nw);
+    const unsigned G = persistent_grid(b, ts);
     const unsigned per_step = (unsigned)b->total_items + G;
     if ((uint64_t)b->ticket_base + (uint64_t)per_step * steps >= 0xF0000000ull) {
       STH_TRY(launch_zero_u32(b->ticket, 1, b->stream));
@@ -243,7 +253,7 @@ int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float 
       a.ticket_base = b->ticket_base;
       a.parity = (parity0 + (int)n) & 1;
       a.c2n = c2 ? c2[n] : 0.0f;
-      STH_TRY(launch_batch(b->npts, mj, nw, G, b->stream, a));
+      STH_TRY(launch_batch(b->npts, ts.mj, ts.nw, ts.vec, G, b->stream, a));
       b->ticket_base += per_step;
       (*launches)++;
     }
@@ -267,7 +277,7 @@ int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float 
       args[s].c1 = c1;
       args[s].c2n = c2 ? c2[n] : 0.0f;
       STH_TRY(launch_sweep((int)b->npts, MODE_LEAP, FMT_F32, plans[s].mj, plans[s].nw,
-                           plans[s].grid, b->stream, args[s]));
+                           plans[s].vec, plans[s].grid, b->stream, args[s]));
       (*launches)++;
     }
   }
@@ -341,9 +351,8 @@ uint32_t sth_batch_size(const sth_batch *b) { return b ? (uint32_t)b->shots.size
 
 int sth_batch_set_tuning(sth_batch *b, const sth_tuning *t, uint32_t planes, uint32_t grid_blocks) {
   if (!b || !t) return STH_ERR_ARG;
-  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : kDefaultRows;
-  const int nw = t->waves ? (int)t->waves : kDefaultWaves;
-  if (!tile_supported(mj, nw)) return STH_ERR_ARG;
+  const TileSel ts = tile_of(*t);
+  if (!tile_supported(ts.mj, ts.nw, ts.vec)) return STH_ERR_ARG;
   b->tune = *t;
   b->planes = planes;
   b->grid_blocks = grid_blocks;
@@ -356,15 +365,15 @@ int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,
                          uint32_t *grid_blocks) {
   if (!b) return STH_ERR_ARG;
   if (t) {
+    const TileSel ts = tile_of(b->tune);
     *t = b->tune;
-    t->rows_per_wave = b->tune.rows_per_wave ? b->tune.rows_per_wave : kDefaultRows;
-    t->waves = b->tune.waves ? b->tune.waves : kDefaultWaves;
+    t->rows_per_wave = (uint32_t)ts.mj;
+    t->waves = (uint32_t)ts.nw;
+    t->vec = (uint32_t)ts.vec;
   }
   if (planes) *planes = b->planes;
   if (grid_blocks) {
-    const int mj = b->tune.rows_per_wave ? (int)b->tune.rows_per_wave : kDefaultRows;
-    const int nw = b->tune.waves ? (int)b->tune.waves : kDefaultWaves;
-    *grid_blocks = persistent_grid(b, mj, nw);
+    *grid_blocks = persistent_grid(b, tile_of(b->tune));
   }
   return STH_OK;
 }
@@ -449,7 +458,8 @@ int sth_batch_timer_read(sth_batch *b, double *ms, uint64_t *launches) {
 int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *best_planes,
                        float *best_ms) {
   if (!b || reps == 0) return STH_ERR_ARG;
-  static const int tiles[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {2, 8}, {4, 8}, {4, 2}, {8, 2}};
+  static const int tiles[][3] = {{1, 4, 1}, {2, 4, 1}, {4, 4, 1}, {2, 8, 1},
+                                 {4, 8, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2}};
   static const uint32_t plane_opts[] = {0, 128, 64, 32};
   const sth_tuning saved = b->tune;
   const uint32_t saved_planes = b->planes, saved_grid = b->grid_blocks;
@@ -462,6 +472,7 @@ int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *
       sth_tuning t = saved;
       t.rows_per_wave = (uint32_t)tl[0];
       t.waves = (uint32_t)tl[1];
+      t.vec = (uint32_t)tl[2];
       int st = sth_batch_set_tuning(b, &t, pl, saved_grid);
       if (st) return st;
       b->timing = false;

@@ -112,7 +112,7 @@ int main(int argc, char **argv) {
   unsigned xs = 945, ys = 945, zs = 880, xh = 4, yh = 4, zh = 16;
   unsigned xb = 1, yb = 1, zb = 16;
   double sbm = 12.0;
-  unsigned npts = 7, seed = 0, tune = 0, rows = 0, waves = 0, chunks = 0;
+  unsigned npts = 7, seed = 0, tune = 0, rows = 0, waves = 0, chunks = 0, vec = 0;
   int mode = 0, dtype = 0, init = -1;
   double c1 = 0.25, c2 = 1e-6, c0 = -1e30;
   static const char *const kModes[] = {"ref", "leapfrog"};
@@ -151,6 +151,7 @@ int main(int argc, char **argv) {
     args += get_int_arg(argv, iarg, argc, "--rows", rows, 1, 8, 1);
     args += get_int_arg(argv, iarg, argc, "--waves", waves, 1, 8, 1);
     args += get_int_arg(argv, iarg, argc, "--chunks", chunks, 1, 64, 1);
+    args += get_int_arg(argv, iarg, argc, "--vec", vec, 1, 2, 1);
     if (args > 0) {
       iarg += args;
     } else {
@@ -211,7 +212,7 @@ int main(int argc, char **argv) {
 
   sth_ctx *ctx = nullptr;
   if ((st = sth_create(&g, &ctx))) die("sth_create", st);
-  sth_tuning tn = {rows, waves, chunks, 1};
+  sth_tuning tn = {rows, waves, chunks, 1, vec};
   if ((st = sth_set_tuning(ctx, &tn))) die("tuning", st);
 
   printf("[rank0] Phase #%i: array initialization\n", ++phase);
@@ -236,8 +237,8 @@ int main(int argc, char **argv) {
     sth_tuning best;
     float ms = 0.0f;
     if ((st = sth_autotune(ctx, mode, 3, &best, &ms))) die("autotune", st);
-    printf(" -- autotune: rows/wave %u, waves %u, chunks %u: %.3f ms/sweep\n", best.rows_per_wave,
-           best.waves, best.chunks, ms);
+    printf(" -- autotune: rows/wave %u, waves %u, k/lane %u, chunks %u: %.3f ms/sweep\n",
+           best.rows_per_wave, best.waves, best.vec, best.chunks, ms);
     // the tuning sweeps modified the fields: initialise again
     if (init == 0) st = sth_init_const(ctx, 0.0f, 1.0f, 1.0f);
     if (st) die("re-initialization", st);

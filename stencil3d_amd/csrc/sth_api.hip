@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "fields.h"
@@ -235,9 +236,10 @@ void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
 Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   Plan p;
   p.mj = t.rows_per_wave ? (int)t.rows_per_wave : kDefaultRows;
-  p.nw = t.waves ? (int)t.waves : kDefaultWaves;
-  const int TJ = p.mj * p.nw;
-  p.ntk = ((int)c->g.zs + 63) / 64;
+  p.nw = t.waves ? (int)t.waves : default_waves(c->fmt);
+  p.vec = t.vec ? (int)t.vec : default_vec(c->fmt);
+  const int TJ = p.mj * p.nw, TK = 64 * p.vec;
+  p.ntk = ((int)c->g.zs + TK - 1) / TK;
   p.ntj = ((int)c->g.ys + TJ - 1) / TJ;
   const int xs = (int)c->g.xs;
   int ch = (int)t.chunks;
@@ -282,7 +284,8 @@ void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a) {
 }
 
 int launch_one(sth_ctx *c, const Plan &p, int mode, const SweepArgs &a) {
-  const hipError_t e = launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.grid, c->stream, a);
+  const hipError_t e =
+      launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (c->timing) c->launches++;
   return STH_OK;
@@ -636,6 +639,136 @@ int sth_download_raw16(sth_ctx *c, sth_field fld, uint16_t *host, int *exp, floa
   });
 }
 
+// ---- field I/O -------------------------------------------------------------
+// A directory with manifest.json and one raw file per field (u.raw, v.raw,
+// s.raw): the storage elements (float32, float64 or the 16-bit numbers) of the
+// dense box with an R-wide halo, C order, k fastest, little-endian.
+
+}  // extern "C"
+
+namespace {
+
+const char *dtype_name(uint32_t d) {
+  return d == STH_FP32 ? "fp32" : d == STH_FP64 ? "fp64" : d == STH_FP16 ? "fp16" : "bf16";
+}
+
+std::string path_join(const char *dir, const char *f) {
+  std::string p(dir);
+  if (!p.empty() && p.back() != '/') p += '/';
+  return p + f;
+}
+
+// Reads an integer value following "key": in a manifest written by sth_save.
+bool json_int(const std::string &txt, const char *key, long long &out, size_t from = 0) {
+  const std::string k = std::string("\"") + key + "\":";
+  const size_t at = txt.find(k, from);
+  if (at == std::string::npos) return false;
+  return sscanf(txt.c_str() + at + k.size(), " %lld", &out) == 1;
+}
+
+template <class T>
+int save_field(sth_ctx *c, sth_field f, const std::string &path, int *exp, float *mx) {
+  const sth_host_layout hl = {(uint32_t)c->R, (uint32_t)c->R, (uint32_t)c->R};
+  uint64_t n = 0;
+  sth_host_elems(&c->g, &hl, &n);
+  std::vector<T> buf(n);
+  int st;
+  if constexpr (sizeof(T) == 2)
+    st = sth_download_raw16(c, f, (uint16_t *)buf.data(), exp, mx, &hl);
+  else
+    st = download_any<T>(c, f, buf.data(), &hl);
+  if (st) return st;
+  FILE *fp = fopen(path.c_str(), "wb");
+  if (!fp) return STH_ERR_ARG;
+  const size_t w = fwrite(buf.data(), sizeof(T), buf.size(), fp);
+  fclose(fp);
+  return w == buf.size() ? STH_OK : STH_ERR_ARG;
+}
+
+template <class T>
+int load_field(sth_ctx *c, sth_field f, const std::string &path, int exp) {
+  const sth_host_layout hl = {(uint32_t)c->R, (uint32_t)c->R, (uint32_t)c->R};
+  uint64_t n = 0;
+  sth_host_elems(&c->g, &hl, &n);
+  std::vector<T> buf(n);
+  FILE *fp = fopen(path.c_str(), "rb");
+  if (!fp) return STH_ERR_ARG;
+  const size_t r = fread(buf.data(), sizeof(T), buf.size(), fp);
+  fclose(fp);
+  if (r != buf.size()) return STH_ERR_ARG;
+  if constexpr (sizeof(T) == 2)
+    return sth_upload_raw16(c, f, (const uint16_t *)buf.data(), exp, &hl);
+  else
+    return upload_any<T>(c, f, buf.data(), &hl);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sth_save(sth_ctx *c, const char *dir) {
+  if (!c || !dir) return STH_ERR_ARG;
+  static const char *names[3] = {"v", "s", "u"};
+  int exps[3] = {0, 0, 0};
+  float mxs[3] = {0, 0, 0};
+  for (int f = 0; f < 3; f++) {
+    const std::string path = path_join(dir, (std::string(names[f]) + ".raw").c_str());
+    int st;
+    if (c->fmt == FMT_F32) st = save_field<float>(c, (sth_field)f, path, &exps[f], &mxs[f]);
+    else if (c->fmt == FMT_F64) st = save_field<double>(c, (sth_field)f, path, &exps[f], &mxs[f]);
+    else st = save_field<uint16_t>(c, (sth_field)f, path, &exps[f], &mxs[f]);
+    if (st) return st;
+  }
+  FILE *fp = fopen(path_join(dir, "manifest.json").c_str(), "w");
+  if (!fp) return STH_ERR_ARG;
+  fprintf(fp,
+          "{\n \"format\": \"stencil3d_amd fields\",\n \"version\": 1,\n"
+          " \"xs\": %u,\n \"ys\": %u,\n \"zs\": %u,\n \"npts\": %u,\n \"halo\": %d,\n"
+          " \"dtype\": \"%s\",\n \"elem_bytes\": %d,\n"
+          " \"layout\": \"dense (xs+2*halo) x (ys+2*halo) x (zs+2*halo), C order, k fastest, "
+          "little-endian; 16-bit values are q with value = q * 2^exp\",\n"
+          " \"fields\": {\n",
+          c->g.xs, c->g.ys, c->g.zs, c->g.npts, c->R, dtype_name(c->g.dtype), c->eb);
+  for (int f = 0; f < 3; f++)
+    fprintf(fp, "  \"%s\": {\"file\": \"%s.raw\", \"exp\": %d, \"maxabs_q\": %.9g}%s\n", names[f],
+            names[f], exps[f], (double)mxs[f], f < 2 ? "," : "");
+  fprintf(fp, " }\n}\n");
+  fclose(fp);
+  return STH_OK;
+}
+
+int sth_load(sth_ctx *c, const char *dir) {
+  if (!c || !dir) return STH_ERR_ARG;
+  FILE *fp = fopen(path_join(dir, "manifest.json").c_str(), "r");
+  if (!fp) return STH_ERR_ARG;
+  std::string txt;
+  char chunk[4096];
+  size_t r;
+  while ((r = fread(chunk, 1, sizeof(chunk), fp)) > 0) txt.append(chunk, r);
+  fclose(fp);
+  long long xs, ys, zs, npts, halo;
+  if (!json_int(txt, "xs", xs) || !json_int(txt, "ys", ys) || !json_int(txt, "zs", zs) ||
+      !json_int(txt, "npts", npts) || !json_int(txt, "halo", halo))
+    return STH_ERR_ARG;
+  const std::string want = std::string("\"dtype\": \"") + dtype_name(c->g.dtype) + "\"";
+  if (xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || npts != c->g.npts || halo != c->R ||
+      txt.find(want) == std::string::npos)
+    return STH_ERR_ARG;
+  static const char *names[3] = {"v", "s", "u"};
+  for (int f = 0; f < 3; f++) {
+    long long e = 0;
+    const size_t at = txt.find(std::string("\"") + names[f] + "\": {");
+    if (at == std::string::npos || !json_int(txt, "exp", e, at)) return STH_ERR_ARG;
+    const std::string path = path_join(dir, (std::string(names[f]) + ".raw").c_str());
+    int st;
+    if (c->fmt == FMT_F32) st = load_field<float>(c, (sth_field)f, path, 0);
+    else if (c->fmt == FMT_F64) st = load_field<double>(c, (sth_field)f, path, 0);
+    else st = load_field<uint16_t>(c, (sth_field)f, path, (int)e);
+    if (st) return st;
+  }
+  return STH_OK;
+}
+
 int sth_iterate(sth_ctx *c, uint32_t iterations) {
   if (!c) return STH_ERR_ARG;
   if (is16(c)) return STH_ERR_UNSUPPORTED;
@@ -691,8 +824,9 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
   if (!c || !t) return STH_ERR_ARG;
   const int mj = t->rows_per_wave ? (int)t->rows_per_wave : kDefaultRows;
-  const int nw = t->waves ? (int)t->waves : kDefaultWaves;
-  if (!tile_supported(mj, nw) || t->chunks > 64) return STH_ERR_ARG;
+  const int nw = t->waves ? (int)t->waves : default_waves(c->fmt);
+  const int vec = t->vec ? (int)t->vec : default_vec(c->fmt);
+  if (!tile_supported(mj, nw, vec) || t->chunks > 64) return STH_ERR_ARG;
   c->tune = *t;
   return STH_OK;
 }
@@ -703,6 +837,7 @@ int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
   const Plan p = make_plan(c, c->tune);
   t->rows_per_wave = (uint32_t)p.mj;
   t->waves = (uint32_t)p.nw;
+  t->vec = (uint32_t)p.vec;
   t->chunks = (uint32_t)p.chunks;
   return STH_OK;
 }
@@ -710,9 +845,10 @@ int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
 int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, float *best_ms) {
   if (!c || reps == 0) return STH_ERR_ARG;
   if (!leapfrog && is16(c)) return STH_ERR_UNSUPPORTED;
-  static const int tiles[][2] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {1, 8}, {2, 8},
-                                 {4, 8}, {8, 8}, {2, 2}, {4, 2}, {8, 2}};
-  static const int chunk_opts[] = {0, 4, 8, 16, 32};
+  static const int tiles[][3] = {{1, 4, 1}, {2, 4, 1}, {4, 4, 1}, {8, 4, 1}, {2, 8, 1},
+                                 {4, 8, 1}, {4, 2, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2},
+                                 {2, 8, 2}, {4, 8, 2}};
+  static const int chunk_opts[] = {0, 8, 16, 32};
   const sth_tuning saved = c->tune;
   const bool saved_timing = c->timing;
   c->timing = false;
@@ -723,6 +859,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
       sth_tuning t = saved;
       t.rows_per_wave = (uint32_t)tl[0];
       t.waves = (uint32_t)tl[1];
+      t.vec = (uint32_t)tl[2];
       t.chunks = (uint32_t)ch;
       const Plan p = make_plan(c, t);
       if (ch != 0 && p.chunks != ch) continue;

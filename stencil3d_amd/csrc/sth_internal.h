@@ -42,14 +42,25 @@ struct sth_ctx {
 namespace sth {
 namespace detail {
 
-// Default tile: 2 rows per wave x 4 waves (8 x 64 points per workgroup),
-// the fastest family measured for 7-25 points at 1040^3 on MI355X.
+// Default tiles, the fastest measured at 1040^3 on MI355X: FP32/FP64 2 rows
+// per wave x 4 waves x 1 k point per lane (8 x 64 points per workgroup);
+// 16-bit storage 2 x 8 waves x 2 k points per lane (16 x 128), so each lane
+// moves 4 bytes per load as in FP32.
 constexpr int kDefaultRows = 2;
 constexpr int kDefaultWaves = 4;
+constexpr int kDefaultVec = 1;
+constexpr int kDefaultWaves16 = 8;
+constexpr int kDefaultVec16 = 2;
+inline int default_waves(int fmt) {
+  return (fmt == FMT_F16 || fmt == FMT_BF16) ? kDefaultWaves16 : kDefaultWaves;
+}
+inline int default_vec(int fmt) {
+  return (fmt == FMT_F16 || fmt == FMT_BF16) ? kDefaultVec16 : kDefaultVec;
+}
 
 // Launch plan of one sweep: tile (rows per wave x waves), i-chunks, grid.
 struct Plan {
-  int mj, nw, chunks, chunk_len, ntk, ntj;
+  int mj, nw, vec, chunks, chunk_len, ntk, ntj;
   unsigned grid;
 };
 

@@ -136,24 +136,45 @@ __device__ inline int xcd_remap(int bid, int nb) {
   return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
 }
 
-template <int NPTS, int MODE, int FMT, int MJ, int NW>
+// V consecutive storage elements of one lane (one aligned vector load).
+template <class ST, int V>
+struct VecT;
+template <class ST>
+struct VecT<ST, 1> {
+  using type = ST;
+  __device__ static inline ST get(const type &x, int) { return x; }
+};
+template <class ST>
+struct VecT<ST, 2> {
+  using type = ST __attribute__((ext_vector_type(2)));
+  __device__ static inline ST get(const type &x, int c) { return c ? x.y : x.x; }
+};
+
+// Tile engine shared by the single-grid and the batched kernels: a
+// (TJ = NW*MJ) x (TK = 64*V) tile of one grid, planes [ib, ie).  Lane l of
+// wave w holds k = k0 + V*l .. k0 + V*l + V-1 of rows j0 + w*MJ .. + MJ-1.
+template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __device__ __forceinline__ void sweep_tile(
     const SweepArgs &a, int tk, int tj, int ib, int ie,
-    typename Codec<FMT>::A (*lds)[(NW * MJ + 2 * Terms<NPTS>::R) * (64 + 2 * Terms<NPTS>::R)],
+    typename Codec<FMT>::A (*lds)[(NW * MJ + 2 * Terms<NPTS>::R) * (64 * V + 2 * Terms<NPTS>::R)],
     float *red) {
   using Tm = Terms<NPTS>;
   constexpr int R = Tm::R, T = Tm::T;
   using CD = Codec<FMT>;
   using ST = typename CD::T;
   using AT = typename CD::A;  // arithmetic type: FP32, or FP64 for FP64 storage
-  constexpr int TJ = NW * MJ, LW = 64 + 2 * R;
+  using VV = VecT<ST, V>;
+  using VT = typename VV::type;
+  constexpr int TJ = NW * MJ, TK = 64 * V, LW = TK + 2 * R;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // j-halo rows loaded per wave
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
+  static_assert(V == 1 || V == 2, "1 or 2 k points per lane");
+  static_assert(V - 1 <= R, "a lane's vector may reach past the last halo cell by V-1");
 
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int tid = threadIdx.x;
-  const int k0 = tk * 64, j0 = tj * TJ;
+  const int k0 = tk * TK, j0 = tj * TJ;
   const int64_t P = a.plane, Pj = a.pitch;
 
   // LEAPFROG coefficients.  16-bit storage works on the stored numbers q
@@ -178,16 +199,21 @@ __device__ __forceinline__ void sweep_tile(
 
   // Clamp reads to the halo-padded extent: lanes/rows past the interior still
   // load the halo values their neighbours need, and never leave the field.
+  // A lane's vector starts at or before the last readable column, so it ends
+  // at most V-1 elements past it: inside the row padding (pitch >= zs + 2R,
+  // and the field has 256 B of slack after its last row).
   const int jmax = a.ys + R - 1 - j0;  // last readable tile row
   const int kmax = a.zs + R - 1 - k0;  // last readable tile column
-  const int kc = min(lane, kmax);
+  const int kc = min(lane * V, (kmax / V) * V);
   uint32_t off[MJ];
-  bool valid[MJ];
+  bool valid[MJ];    // the lane's first point is inside the interior
+  bool valid2[MJ];   // ... and its last one too
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int row = w * MJ + m;
     off[m] = (uint32_t)((int64_t)(min(row, jmax) + R) * Pj + (kc + R));
-    valid[m] = (j0 + row < a.ys) && (k0 + lane < a.zs);
+    valid[m] = (j0 + row < a.ys) && (k0 + lane * V < a.zs);
+    valid2[m] = (j0 + row < a.ys) && (k0 + lane * V + V - 1 < a.zs);
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
@@ -203,20 +229,30 @@ __device__ __forceinline__ void sweep_tile(
   int khpos = 0;
   if (has_kh) {
     const int row = tid / (2 * R), c = tid % (2 * R);
-    const int tcol = c < R ? c - R : 64 + c - R;
+    const int tcol = c < R ? c - R : TK + c - R;
     khoff = (uint32_t)((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R));
-    khpos = (row + R) * LW + (c < R ? c : 64 + c);
+    khpos = (row + R) * LW + (c < R ? c : TK + c);
   }
 
-  // Register queue: q[m][t] = u at plane i - R + t.
-  AT q[MJ][2 * R + 1];
+  auto vload = [](const ST *p) { return *(const VT *)p; };
+  auto vload_nt = [](const ST *p) { return __builtin_nontemporal_load((const VT *)p); };
+
+  // Register queue: q[m][t][c] = u at plane i - R + t.
+  AT q[MJ][2 * R + 1][V];
 #pragma unroll
   for (int t = 0; t < 2 * R; t++) {
     const ST *Up = Ub + (int64_t)(ib - R + t) * P;
 #pragma unroll
-    for (int m = 0; m < MJ; m++) q[m][t] = CD::dec(Up[off[m]]);
+    for (int m = 0; m < MJ; m++) {
+      const VT x = vload(Up + off[m]);
+#pragma unroll
+      for (int c = 0; c < V; c++) q[m][t][c] = CD::dec(VV::get(x, c));
+    }
   }
-  ST nu[MJ], ns[MJ], nv[MJ], nhj[HJ], nhk;
+  VT nu[MJ], ns[MJ], nv[MJ], nhj[HJ];
+  ST nhk;
+  // s and v are touched once per sweep: nontemporal loads and stores keep
+  // them from displacing u's halo rows, which neighbouring tiles re-read.
   auto prefetch = [&](int i) {
     const ST *Up = Ub + (int64_t)(i + R) * P;
     const ST *Uc = Ub + (int64_t)i * P;
@@ -224,12 +260,12 @@ __device__ __forceinline__ void sweep_tile(
     const ST *Vp = Vb + (int64_t)i * P;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      nu[m] = Up[off[m]];
-      ns[m] = valid[m] ? Sp[off[m]] : ST(0);
-      nv[m] = valid[m] ? Vp[off[m]] : ST(0);
+      nu[m] = vload(Up + off[m]);
+      ns[m] = valid[m] ? vload_nt(Sp + off[m]) : VT(0);
+      nv[m] = valid[m] ? vload_nt(Vp + off[m]) : VT(0);
     }
 #pragma unroll
-    for (int h = 0; h < HJ; h++) nhj[h] = hrow[h] >= 0 ? Uc[hoff[h]] : ST(0);
+    for (int h = 0; h < HJ; h++) nhj[h] = hrow[h] >= 0 ? vload(Uc + hoff[h]) : VT(0);
     nhk = has_kh ? Uc[khoff] : ST(0);
   };
   prefetch(ib);
@@ -237,10 +273,12 @@ __device__ __forceinline__ void sweep_tile(
   float mx = 0.0f;
   for (int i = ib; i < ie; i++) {
     AT *L = lds[(i - ib) & 1];
-    ST cs[MJ], cv[MJ], chj[HJ], chk;
+    VT cs[MJ], cv[MJ], chj[HJ];
+    ST chk;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      q[m][2 * R] = CD::dec(nu[m]);
+#pragma unroll
+      for (int c = 0; c < V; c++) q[m][2 * R][c] = CD::dec(VV::get(nu[m], c));
       cs[m] = ns[m];
       cv[m] = nv[m];
     }
@@ -250,54 +288,90 @@ __device__ __forceinline__ void sweep_tile(
     if (i + 1 < ie) prefetch(i + 1);
 
 #pragma unroll
-    for (int m = 0; m < MJ; m++) L[(w * MJ + m + R) * LW + lane + R] = q[m][R];
+    for (int m = 0; m < MJ; m++)
+#pragma unroll
+      for (int c = 0; c < V; c++) L[(w * MJ + m + R) * LW + lane * V + c + R] = q[m][R][c];
 #pragma unroll
     for (int h = 0; h < HJ; h++)
-      if (hrow[h] >= 0) L[hrow[h] * LW + lane + R] = CD::dec(chj[h]);
+      if (hrow[h] >= 0)
+#pragma unroll
+        for (int c = 0; c < V; c++)
+          L[hrow[h] * LW + lane * V + c + R] = CD::dec(VV::get(chj[h], c));
     if (has_kh) L[khpos] = CD::dec(chk);
     __syncthreads();
 
     ST *Vp = Vb + (int64_t)i * P;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      const int lr = w * MJ + m + R, col = lane + R;
-      auto pair = [&](int ax, int r) -> AT {
-        if (ax == 0) return q[m][R - r] + q[m][R + r];
-        if (ax == 2) return L[lr * LW + col - r] + L[lr * LW + col + r];
-        const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R] : L[(lr - r) * LW + col];
-        const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R] : L[(lr + r) * LW + col];
-        return lo + hi;
-      };
-      const AT u0 = q[m][R];
-      const AT s0 = CD::dec(cs[m]);
-      const AT v0 = CD::dec(cv[m]);
-      AT out;
-      if constexpr (MODE == MODE_REF) {
-        // stencil_N_ref_1: p = coef0*u0 + sum c*pair (the first product
-        // rounded, the rest fused), then v0 - A*u0 + s0*p.
-        AT p = fmaT((AT)a.coef0, u0, (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]));
+      const int lr = w * MJ + m + R, col = lane * V + R;
+      AT res[V];
 #pragma unroll
-        for (int t = 1; t < T; t++) p = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
-        out = fmaT(s0, p, fmaT((AT)a.mA, u0, v0));
-      } else {
-        AT Lp = (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+      for (int c = 0; c < V; c++) {
+        auto pair = [&](int ax, int r) -> AT {
+          if (ax == 0) return q[m][R - r][c] + q[m][R + r][c];
+          if (ax == 2) {
+            // k-neighbours inside the lane's own vector come from registers
+            const AT lo = (c - r >= 0) ? q[m][R][(c - r) >= 0 ? c - r : 0]
+                                       : L[lr * LW + col + c - r];
+            const AT hi = (c + r < V) ? q[m][R][(c + r) < V ? c + r : 0]
+                                      : L[lr * LW + col + c + r];
+            return lo + hi;
+          }
+          const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R][c]
+                                     : L[(lr - r) * LW + col + c];
+          const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R][c]
+                                     : L[(lr + r) * LW + col + c];
+          return lo + hi;
+        };
+        const AT u0 = q[m][R][c];
+        const AT s0 = CD::dec(VV::get(cs[m], c));
+        const AT v0 = CD::dec(VV::get(cv[m], c));
+        AT out;
+        if constexpr (MODE == MODE_REF) {
+          // stencil_N_ref_1: p = coef0*u0 + sum c*pair (the first product
+          // rounded, the rest fused), then v0 - A*u0 + s0*p.
+          AT p = fmaT((AT)a.coef0, u0, (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]));
 #pragma unroll
-        for (int t = 1; t < T; t++) Lp = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        if constexpr (FMT == FMT_F16 || FMT == FMT_BF16)
-          out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -(v0 * kv))));
-        else
-          out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -v0)));
+          for (int t = 1; t < T; t++) p = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
+          out = fmaT(s0, p, fmaT((AT)a.mA, u0, v0));
+        } else {
+          AT Lp = (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+#pragma unroll
+          for (int t = 1; t < T; t++) Lp = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+          if constexpr (FMT == FMT_F16 || FMT == FMT_BF16)
+            out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -(v0 * kv))));
+          else
+            out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -v0)));
+        }
+        res[c] = out;
       }
-      if (valid[m]) {
-        const ST qo = CD::enc(out);
-        Vp[off[m]] = qo;
-        if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) mx = fmaxf(mx, fabsf(CD::dec(qo)));
+      ST qo[V];
+#pragma unroll
+      for (int c = 0; c < V; c++) qo[c] = CD::enc(res[c]);
+      if constexpr (V == 1) {
+        if (valid[m]) __builtin_nontemporal_store(qo[0], Vp + off[m]);
+      } else {
+        if (valid2[m]) {
+          VT o;
+          o.x = qo[0];
+          o.y = qo[1];
+          __builtin_nontemporal_store(o, (VT *)(Vp + off[m]));
+        } else if (valid[m]) {
+          __builtin_nontemporal_store(qo[0], Vp + off[m]);
+        }
+      }
+      if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
+        if (valid[m]) mx = fmaxf(mx, fabsf(CD::dec(qo[0])));
+        if constexpr (V == 2)
+          if (valid2[m]) mx = fmaxf(mx, fabsf(CD::dec(qo[1])));
       }
     }
 #pragma unroll
     for (int m = 0; m < MJ; m++)
 #pragma unroll
-      for (int t = 0; t < 2 * R; t++) q[m][t] = q[m][t + 1];
+      for (int t = 0; t < 2 * R; t++)
+#pragma unroll
+        for (int c = 0; c < V; c++) q[m][t][c] = q[m][t + 1][c];
   }
 
   if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
@@ -332,10 +406,10 @@ __device__ inline void q16_publish(const SweepArgs &a) {
   }
 }
 
-template <int NPTS, int MODE, int FMT, int MJ, int NW>
+template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
   constexpr int R = Terms<NPTS>::R;
-  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 + 2 * R)];
+  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 * V + 2 * R)];
   __shared__ float red[NW];
   q16_publish<FMT>(a);
   int bid = blockIdx.x;
@@ -347,7 +421,7 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
   const int ib = ch * a.chunk_len;
   const int ie = min(a.xs, ib + a.chunk_len);
   if (ib >= ie) return;
-  sweep_tile<NPTS, MODE, FMT, MJ, NW>(a, tk, tj, ib, ie, lds, red);
+  sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, tk, tj, ib, ie, lds, red);
 }
 
 }  // namespace sth

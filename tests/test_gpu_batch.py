@@ -97,13 +97,14 @@ def test_graph_replays_match_eager(gpu, orc):
         assert np.array_equal(outs[0][q].view(np.uint32), outs[1][q].view(np.uint32))
 
 
-@pytest.mark.parametrize("tile,planes", [((1, 4), 0), ((4, 4), 16), ((8, 2), 5), ((2, 8), 33)])
+@pytest.mark.parametrize("tile,planes", [((1, 4, 1), 0), ((4, 4, 1), 16), ((8, 2, 1), 5),
+                                         ((2, 8, 1), 33), ((2, 4, 2), 9), ((4, 8, 2), 0)])
 def test_batch_tunings_agree(gpu, orc, tile, planes):
     npts, R = 13, 2
     fields = shot_fields(orc, DIMS, npts, 70)
     c0, c1 = coeffs(orc, npts)
     with gpu.Batch(DIMS, npts=npts) as bt:
-        bt.set_tuning(*tile, planes=planes)
+        bt.set_tuning(tile[0], tile[1], planes=planes, vec=tile[2])
         load(bt, fields, R)
         bt.leapfrog(3, c0, c1)
         got = [bt.shot(q).download("u", R) for q in range(len(DIMS))]

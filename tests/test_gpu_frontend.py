@@ -1,0 +1,96 @@
+"""GPU tests of the hstencil3d_hip front end (the reference driver's CLI and
+report, main.cpp:94-332) and of the library's diagnostics."""
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def run_fe(sth, *args):
+    r = subprocess.run([sth.FRONTEND_PATH, *args], capture_output=True, text=True, timeout=120)
+    return r
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_frontend_reference_run_passes_its_check(gpu, npts):
+    """The reference's default run: init_arrays, warm-up, timed sweeps, two
+    checks with 0 errors, the same report line."""
+    r = run_fe(gpu, "--xs", "64", "--ys", "64", "--zs", "96", "--iter", "10",
+               "--stencil", str(npts))
+    assert r.returncode == 0, r.stderr
+    assert re.findall(r"check: (\d+) errors found", r.stdout) == ["0", "0"]
+    m = re.search(r"finish compute in ([0-9.]+) secs \(([0-9.]+) GFlop/s, mem: ([0-9.]+) GB/s, "
+                  r"cache: ([0-9.]+) GB/s\)", r.stdout)
+    assert m and float(m.group(1)) > 0
+    assert "Gpts/s" in r.stdout
+
+
+def test_frontend_leapfrog_fp16(gpu):
+    r = run_fe(gpu, "--xs", "48", "--ys", "40", "--zs", "70", "--iter", "20", "--stencil", "25",
+               "--mode", "leapfrog", "--init", "gauss", "--c1", "0.16", "--c0", "0.08",
+               "--dtype", "fp16")
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"norms: max \|u\| = ([0-9.e+-]+)", r.stdout)
+    assert m and 0.0 < float(m.group(1)) < 10.0
+
+
+def test_frontend_autotune(gpu):
+    r = run_fe(gpu, "--xs", "40", "--ys", "40", "--zs", "64", "--iter", "3", "--stencil", "13",
+               "--tune")
+    assert r.returncode == 0, r.stderr
+    assert "autotune:" in r.stdout
+    assert re.findall(r"check: (\d+) errors found", r.stdout) == ["0", "0"]
+
+
+def test_stream_copy_copies(gpu, orc):
+    g = orc.Grid(20, 22, 64, 1, 1, 1)
+    x = orc.fill_uniform(g, 3, 0)
+    with gpu.Stencil(20, 22, 64, npts=7) as ctx:
+        ctx.upload("u", x, 1)
+        ms = ctx.stream_copy(ctx.layout().bytes // 16 * 16, 2)
+        y = ctx.download("v", 1)
+    assert ms > 0
+    assert np.array_equal(y[g.interior], x[g.interior])
+
+
+def test_autotune_keeps_results(gpu, orc):
+    """The tuner picks a schedule; a run with it still matches the oracle."""
+    g = orc.Grid(40, 36, 100, 2, 2, 2)
+    u, s, v = (orc.fill_uniform(g, 4, f) for f in range(3))
+    with gpu.Stencil(40, 36, 100, npts=13) as ctx:
+        best, ms = ctx.autotune(leapfrog=False, reps=1)
+        assert ms > 0 and best["rows_per_wave"] in (1, 2, 4, 8)
+        for f, a in (("u", u), ("s", s), ("v", v)):
+            ctx.upload(f, a, 2)
+        ctx.iterate(2)
+        got = ctx.download("v", 2)
+    orc.ref_sweep_f32(13, g, v, s, u, 2)
+    assert np.array_equal(got[g.interior].view(np.uint32), v[g.interior].view(np.uint32))
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp16", "bf16", "fp64"])
+def test_save_load_restart_is_exact(gpu, tmp_path, dtype):
+    """Checkpoint after 5 steps, restore into a fresh context, run 4 more
+    steps on both: identical fields (and manifest geometry checks)."""
+    shape, npts = (18, 21, 70), 13
+    c1 = 0.16
+    c0 = 2.0 - 6.0 * c1
+    with gpu.Stencil(*shape, npts=npts, dtype=dtype) as a:
+        a.init_gaussian("u", 1.0, 3.0, 1e-3, 1, 0)
+        a.init_gaussian("v", 1.0, 3.0, 1e-3, 1, 1)
+        a.init_point("s", 9, 10, 35, 1.0)
+        a.leapfrog(5, c0, c1, [1e-2] * 5)
+        a.save(str(tmp_path))
+        a.leapfrog(4, c0, c1, [1e-2] * 4)
+        ua = a.download("u", 2, dtype=np.float64)
+    with gpu.Stencil(*shape, npts=npts, dtype=dtype) as b:
+        b.load(str(tmp_path))
+        b.leapfrog(4, c0, c1, [1e-2] * 4)
+        ub = b.download("u", 2, dtype=np.float64)
+    assert np.array_equal(ua, ub)
+    with gpu.Stencil(shape[0] + 1, shape[1], shape[2], npts=npts, dtype=dtype) as c:
+        with pytest.raises(gpu.SthError):
+            c.load(str(tmp_path))

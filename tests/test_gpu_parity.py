@@ -44,19 +44,25 @@ def test_ref_sweep_bit_exact(gpu, orc, npts, shape):
     assert same_bits(out[I], v[I]), f"max ulp {ulp_diff(out[I], v[I])}"
 
 
-@pytest.mark.parametrize("npts", [7, 25])
-@pytest.mark.parametrize("tile", [(1, 4, 1), (2, 4, 3), (4, 4, 2), (8, 4, 1), (1, 8, 5),
-                                  (2, 8, 1), (4, 8, 4), (8, 8, 2), (2, 2, 1), (4, 2, 7),
-                                  (8, 2, 3)])
+# (rows per wave, waves, k points per lane, i-chunks): every compiled tile
+TILES = [(1, 4, 1, 1), (2, 4, 1, 3), (4, 4, 1, 2), (8, 4, 1, 1), (1, 8, 1, 5), (2, 8, 1, 1),
+         (4, 8, 1, 4), (8, 8, 1, 2), (2, 2, 1, 1), (4, 2, 1, 7), (8, 2, 1, 3), (1, 4, 2, 2),
+         (2, 4, 2, 1), (4, 4, 2, 3), (2, 8, 2, 2), (4, 8, 2, 1), (2, 2, 2, 4), (4, 2, 2, 2)]
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("tile", TILES, ids=[f"{t[0]}x{t[1]}v{t[2]}c{t[3]}" for t in TILES])
 def test_ref_every_tile_schedule(gpu, orc, npts, tile):
     """Every compiled tile/chunk schedule gives the same bits (the schedule is
-    result-neutral, like the reference's iterators)."""
-    shape = (21, 37, 75)
+    result-neutral, like the reference's iterators).  zs = 75 is odd, so the
+    two-points-per-lane tiles end on a half-valid lane; zs > 128 gives them a
+    second, partial k-tile."""
+    shape = (21, 37, 75) if tile[2] == 1 else (21, 19, 139)
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     v, s, u = seeded(orc, g, seed=3)
     with gpu.Stencil(*shape, npts=npts) as ctx:
-        ctx.set_tuning(*tile, xcd_remap=tile[2] % 2)
+        ctx.set_tuning(tile[0], tile[1], tile[3], xcd_remap=tile[3] % 2, vec=tile[2])
         for f, a in (("v", v), ("s", s), ("u", u)):
             ctx.upload(f, a, R)
         ctx.iterate(3)
@@ -121,7 +127,8 @@ def test_kat_exact_recurrence(gpu, orc, npts):
 
 @pytest.mark.parametrize("npts", [7, 13, 25])
 @pytest.mark.parametrize("shape", [(16, 16, 16), (19, 33, 70)])
-def test_leapfrog_bit_exact(gpu, orc, npts, shape):
+@pytest.mark.parametrize("vec", [1, 2])
+def test_leapfrog_bit_exact(gpu, orc, npts, shape, vec):
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     v, s, u = seeded(orc, g, seed=0)  # zero halos
@@ -129,6 +136,7 @@ def test_leapfrog_bit_exact(gpu, orc, npts, shape):
     c0 = np.float32(2.0 - orc.lap_gain(npts) * float(c1))
     c2 = (np.arange(1, 8, dtype=np.float32) * np.float32(1e-3)).astype(np.float32)
     with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.set_tuning(vec=vec)
         ctx.upload("u", u, R)
         ctx.upload("v", v, R)
         ctx.upload("s", s, R)
@@ -145,8 +153,9 @@ def test_leapfrog_bit_exact(gpu, orc, npts, shape):
 
 @pytest.mark.parametrize("fmt,dtype", [(1, "fp16"), (2, "bf16")])
 @pytest.mark.parametrize("npts", [7, 13, 25])
-def test_q16_leapfrog_bit_exact(gpu, orc, fmt, dtype, npts):
-    shape = (14, 17, 70)
+@pytest.mark.parametrize("vec", [1, 2])
+def test_q16_leapfrog_bit_exact(gpu, orc, fmt, dtype, npts, vec):
+    shape = (14, 17, 71)
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     v, s, u = seeded(orc, g, seed=5)
@@ -161,6 +170,7 @@ def test_q16_leapfrog_bit_exact(gpu, orc, fmt, dtype, npts):
     c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
     c2 = np.linspace(0.5, -0.5, 9).astype(np.float32)
     with gpu.Stencil(*shape, npts=npts, dtype=dtype) as ctx:
+        ctx.set_tuning(vec=vec)
         ctx.upload_raw16("u", qs[0], exps[0], R)
         ctx.upload_raw16("v", qs[1], exps[1], R)
         ctx.upload_raw16("s", qs[2], exps[2], R)
