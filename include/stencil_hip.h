@@ -177,7 +177,10 @@ int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
 int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
 /* Times the candidate schedules on this context's geometry (REF or LEAPFROG
  * sweeps, `reps` launches each, fields are modified), keeps the fastest and
- * returns it with its mean kernel time in ms. */
+ * returns it with its mean kernel time in ms.  If the environment variable
+ * STH_TUNE_CACHE names a file, a result stored there for the same (stencil,
+ * dtype, mode, xs, ys, zs) is applied without timing, and new results are
+ * appended to it. */
 int sth_autotune(sth_ctx *ctx, int leapfrog, uint32_t reps, sth_tuning *best, float *best_ms);
 /* Kernel timing: HIP events recorded on the context's stream around every
  * sweep launch between timer_start and timer_read.  timer_read synchronises

@@ -235,21 +235,22 @@ void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
 
 Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   Plan p;
-  p.mj = t.rows_per_wave ? (int)t.rows_per_wave : kDefaultRows;
-  p.nw = t.waves ? (int)t.waves : default_waves(c->fmt);
-  p.vec = t.vec ? (int)t.vec : default_vec(c->fmt);
+  const TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, t);
+  p.mj = ts.mj;
+  p.nw = ts.nw;
+  p.vec = ts.vec;
   const int TJ = p.mj * p.nw, TK = 64 * p.vec;
   p.ntk = ((int)c->g.zs + TK - 1) / TK;
   p.ntj = ((int)c->g.ys + TJ - 1) / TJ;
   const int xs = (int)c->g.xs;
   int ch = (int)t.chunks;
   if (ch == 0) {
-    // i-chunks of about 64 planes, and at least 8192 workgroups where the
+    // i-chunks of about 64 planes, and at least 4096 workgroups where the
     // grid allows it: many short-lived workgroups drain evenly (no tail of a
     // few long ones); a chunk re-reads 2R planes of u, so none is shorter
     // than max(8, 4R) planes.
     const int tiles = p.ntk * p.ntj;
-    ch = std::max((xs + 63) / 64, (8192 + tiles - 1) / tiles);
+    ch = std::max((xs + 63) / 64, (4096 + tiles - 1) / tiles);
     ch = std::min(ch, std::max(1, xs / std::max(8, 4 * c->R)));
     ch = std::max(ch, 1);
   }
@@ -823,10 +824,8 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
   if (!c || !t) return STH_ERR_ARG;
-  const int mj = t->rows_per_wave ? (int)t->rows_per_wave : kDefaultRows;
-  const int nw = t->waves ? (int)t->waves : default_waves(c->fmt);
-  const int vec = t->vec ? (int)t->vec : default_vec(c->fmt);
-  if (!tile_supported(mj, nw, vec) || t->chunks > 64) return STH_ERR_ARG;
+  const TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, *t);
+  if (!tile_supported(ts.mj, ts.nw, ts.vec) || t->chunks > 64) return STH_ERR_ARG;
   c->tune = *t;
   return STH_OK;
 }
@@ -842,9 +841,65 @@ int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
   return STH_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Tuning cache: one line per measured geometry,
+//   sth-tune 1 <npts> <dtype> <leapfrog> <xs> <ys> <zs> <rows> <waves> <vec> <chunks> <ms>
+// the last matching line wins.
+bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning &t, float &ms) {
+  FILE *fp = fopen(path, "r");
+  if (!fp) return false;
+  char line[256];
+  bool found = false;
+  while (fgets(line, sizeof(line), fp)) {
+    unsigned ver, npts, dt, lf, xs, ys, zs, mj, nw, vec, ch;
+    float m;
+    if (sscanf(line, "sth-tune %u %u %u %u %u %u %u %u %u %u %u %f", &ver, &npts, &dt, &lf, &xs,
+               &ys, &zs, &mj, &nw, &vec, &ch, &m) != 12)
+      continue;
+    if (ver != 1 || npts != c->g.npts || dt != c->g.dtype || lf != (unsigned)leap ||
+        xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || !tile_supported((int)mj, (int)nw, (int)vec))
+      continue;
+    t.rows_per_wave = mj;
+    t.waves = nw;
+    t.vec = vec;
+    t.chunks = ch;
+    ms = m;
+    found = true;
+  }
+  fclose(fp);
+  return found;
+}
+
+void tune_cache_store(const char *path, const sth_ctx *c, int leap, const sth_tuning &t, float ms) {
+  FILE *fp = fopen(path, "a");
+  if (!fp) return;
+  fprintf(fp, "sth-tune 1 %u %u %d %u %u %u %u %u %u %u %.5f\n", c->g.npts, c->g.dtype, leap,
+          c->g.xs, c->g.ys, c->g.zs, t.rows_per_wave, t.waves, t.vec, t.chunks, ms);
+  fclose(fp);
+}
+
+}  // namespace
+
+extern "C" {
+
 int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, float *best_ms) {
   if (!c || reps == 0) return STH_ERR_ARG;
   if (!leapfrog && is16(c)) return STH_ERR_UNSUPPORTED;
+  leapfrog = leapfrog ? 1 : 0;
+  const char *cache = getenv("STH_TUNE_CACHE");
+  if (cache && *cache) {
+    sth_tuning t = c->tune;
+    float ms = 0.0f;
+    if (tune_cache_lookup(cache, c, leapfrog, t, ms)) {
+      c->tune = t;
+      if (best) *best = t;
+      if (best_ms) *best_ms = ms;
+      return STH_OK;
+    }
+  }
   static const int tiles[][3] = {{1, 4, 1}, {2, 4, 1}, {4, 4, 1}, {8, 4, 1}, {2, 8, 1},
                                  {4, 8, 1}, {4, 2, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2},
                                  {2, 8, 2}, {4, 8, 2}};
@@ -883,6 +938,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   }
   c->tune = bt;
   c->timing = saved_timing;
+  if (cache && *cache) tune_cache_store(cache, c, leapfrog, bt, bms);
   if (best) *best = bt;
   if (best_ms) *best_ms = bms;
   return STH_OK;

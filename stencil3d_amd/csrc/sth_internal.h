@@ -42,10 +42,12 @@ struct sth_ctx {
 namespace sth {
 namespace detail {
 
-// Default tiles, the fastest measured at 1040^3 on MI355X: FP32/FP64 2 rows
-// per wave x 4 waves x 1 k point per lane (8 x 64 points per workgroup);
-// 16-bit storage 2 x 8 waves x 2 k points per lane (16 x 128), so each lane
-// moves 4 bytes per load as in FP32.
+// Default tiles.  A field of at least one tile per CU takes the fastest
+// tile measured at 1040^3 on MI355X: FP32 4 rows per wave x 8 waves x 2 k
+// points per lane (32 x 128 points per workgroup), 16-bit storage 2 x 8 x 2
+// (16 x 128; each lane moves 4 bytes per load as in FP32).  Smaller fields
+// (and FP64) take 2 x 4 x 1 (FP32/FP64) or 2 x 8 x 2 (16-bit).  A tuning
+// that names some of rows/waves/vec takes the small tile's values for the rest.
 constexpr int kDefaultRows = 2;
 constexpr int kDefaultWaves = 4;
 constexpr int kDefaultVec = 1;
@@ -56,6 +58,16 @@ inline int default_waves(int fmt) {
 }
 inline int default_vec(int fmt) {
   return (fmt == FMT_F16 || fmt == FMT_BF16) ? kDefaultVec16 : kDefaultVec;
+}
+struct TileShape {
+  int mj, nw, vec;
+};
+inline TileShape default_tile(int fmt, uint32_t ys, uint32_t zs, const sth_tuning &t) {
+  if (t.rows_per_wave || t.waves || t.vec)
+    return {t.rows_per_wave ? (int)t.rows_per_wave : kDefaultRows,
+            t.waves ? (int)t.waves : default_waves(fmt), t.vec ? (int)t.vec : default_vec(fmt)};
+  if (fmt == FMT_F32 && ((ys + 31) / 32) * ((zs + 127) / 128) >= 256) return {4, 8, 2};
+  return {kDefaultRows, default_waves(fmt), default_vec(fmt)};
 }
 
 // Launch plan of one sweep: tile (rows per wave x waves), i-chunks, grid.

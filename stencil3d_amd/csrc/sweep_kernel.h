@@ -211,7 +211,7 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int row = w * MJ + m;
-    off[m] = (uint32_t)((int64_t)(min(row, jmax) + R) * Pj + (kc + R));
+    off[m] = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (kc + R)) * sizeof(ST));
     valid[m] = (j0 + row < a.ys) && (k0 + lane * V < a.zs);
     valid2[m] = (j0 + row < a.ys) && (k0 + lane * V + V - 1 < a.zs);
   }
@@ -222,7 +222,7 @@ __device__ __forceinline__ void sweep_tile(
     const int hh = w + NW * h;  // halo row id 0..2R-1
     const int trow = hh < R ? hh - R : TJ + hh - R;
     hrow[h] = hh < 2 * R ? (hh < R ? hh : TJ + hh) : -1;  // LDS row, -1: none
-    hoff[h] = (uint32_t)((int64_t)(min(trow, jmax) + R) * Pj + (kc + R));
+    hoff[h] = (uint32_t)(((int64_t)(min(trow, jmax) + R) * Pj + (kc + R)) * sizeof(ST));
   }
   const bool has_kh = tid < TJ * 2 * R;
   uint32_t khoff = 0;
@@ -230,12 +230,17 @@ __device__ __forceinline__ void sweep_tile(
   if (has_kh) {
     const int row = tid / (2 * R), c = tid % (2 * R);
     const int tcol = c < R ? c - R : TK + c - R;
-    khoff = (uint32_t)((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R));
+    khoff = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R)) * sizeof(ST));
     khpos = (row + R) * LW + (c < R ? c : TK + c);
   }
 
-  auto vload = [](const ST *p) { return *(const VT *)p; };
-  auto vload_nt = [](const ST *p) { return __builtin_nontemporal_load((const VT *)p); };
+  // In-tile offsets are 32-bit byte offsets from a workgroup-uniform plane
+  // base, so loads and stores take the SGPR-base + VGPR-offset form.
+  auto vload = [](const ST *p, uint32_t ob) { return *(const VT *)((const char *)p + ob); };
+  auto vload_nt = [](const ST *p, uint32_t ob) {
+    return __builtin_nontemporal_load((const VT *)((const char *)p + ob));
+  };
+  auto sload = [](const ST *p, uint32_t ob) { return *(const ST *)((const char *)p + ob); };
 
   // Register queue: q[m][t][c] = u at plane i - R + t.
   AT q[MJ][2 * R + 1][V];
@@ -244,37 +249,52 @@ __device__ __forceinline__ void sweep_tile(
     const ST *Up = Ub + (int64_t)(ib - R + t) * P;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      const VT x = vload(Up + off[m]);
+      const VT x = vload(Up, off[m]);
 #pragma unroll
       for (int c = 0; c < V; c++) q[m][t][c] = CD::dec(VV::get(x, c));
     }
   }
+  // Every u value of a plane -- the tile's own rows, its j-halo rows and its
+  // k-halo columns -- is loaded in the same step, when the plane enters as
+  // the leading plane i+R; the halo values then wait R steps in register
+  // queues (hq, kq: planes i..i+R).  Neighbouring workgroups run in step, so
+  // each line is fetched once and the other tiles' reads of it hit in L2
+  // (loading the halo when the plane becomes the centre, R steps later, finds
+  // it evicted).
+  VT hq[HJ][R + 1];
+  ST kq[R + 1];
+#pragma unroll
+  for (int t = 0; t < R; t++) {
+    const ST *Up = Ub + (int64_t)(ib + t) * P;
+#pragma unroll
+    for (int h = 0; h < HJ; h++) hq[h][t] = hrow[h] >= 0 ? vload(Up, hoff[h]) : VT(0);
+    kq[t] = has_kh ? sload(Up, khoff) : ST(0);
+  }
   VT nu[MJ], ns[MJ], nv[MJ], nhj[HJ];
   ST nhk;
   // s and v are touched once per sweep: nontemporal loads and stores keep
-  // them from displacing u's halo rows, which neighbouring tiles re-read.
+  // them from displacing u's lines, which neighbouring tiles re-read.
   auto prefetch = [&](int i) {
     const ST *Up = Ub + (int64_t)(i + R) * P;
-    const ST *Uc = Ub + (int64_t)i * P;
     const ST *Sp = Sb + (int64_t)i * P;
     const ST *Vp = Vb + (int64_t)i * P;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      nu[m] = vload(Up + off[m]);
-      ns[m] = valid[m] ? vload_nt(Sp + off[m]) : VT(0);
-      nv[m] = valid[m] ? vload_nt(Vp + off[m]) : VT(0);
+      // masked lanes read their clamped (in-bounds) cells: no branches
+      nu[m] = vload(Up, off[m]);
+      ns[m] = vload_nt(Sp, off[m]);
+      nv[m] = vload_nt(Vp, off[m]);
     }
 #pragma unroll
-    for (int h = 0; h < HJ; h++) nhj[h] = hrow[h] >= 0 ? vload(Uc + hoff[h]) : VT(0);
-    nhk = has_kh ? Uc[khoff] : ST(0);
+    for (int h = 0; h < HJ; h++) nhj[h] = hrow[h] >= 0 ? vload(Up, hoff[h]) : VT(0);
+    nhk = has_kh ? sload(Up, khoff) : ST(0);
   };
   prefetch(ib);
 
   float mx = 0.0f;
   for (int i = ib; i < ie; i++) {
     AT *L = lds[(i - ib) & 1];
-    VT cs[MJ], cv[MJ], chj[HJ];
-    ST chk;
+    VT cs[MJ], cv[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
 #pragma unroll
@@ -283,8 +303,8 @@ __device__ __forceinline__ void sweep_tile(
       cv[m] = nv[m];
     }
 #pragma unroll
-    for (int h = 0; h < HJ; h++) chj[h] = nhj[h];
-    chk = nhk;
+    for (int h = 0; h < HJ; h++) hq[h][R] = nhj[h];
+    kq[R] = nhk;
     if (i + 1 < ie) prefetch(i + 1);
 
 #pragma unroll
@@ -296,8 +316,8 @@ __device__ __forceinline__ void sweep_tile(
       if (hrow[h] >= 0)
 #pragma unroll
         for (int c = 0; c < V; c++)
-          L[hrow[h] * LW + lane * V + c + R] = CD::dec(VV::get(chj[h], c));
-    if (has_kh) L[khpos] = CD::dec(chk);
+          L[hrow[h] * LW + lane * V + c + R] = CD::dec(VV::get(hq[h][0], c));
+    if (has_kh) L[khpos] = CD::dec(kq[0]);
     __syncthreads();
 
     ST *Vp = Vb + (int64_t)i * P;
@@ -349,15 +369,15 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
       for (int c = 0; c < V; c++) qo[c] = CD::enc(res[c]);
       if constexpr (V == 1) {
-        if (valid[m]) __builtin_nontemporal_store(qo[0], Vp + off[m]);
+        if (valid[m]) __builtin_nontemporal_store(qo[0], (ST *)((char *)Vp + off[m]));
       } else {
         if (valid2[m]) {
           VT o;
           o.x = qo[0];
           o.y = qo[1];
-          __builtin_nontemporal_store(o, (VT *)(Vp + off[m]));
+          __builtin_nontemporal_store(o, (VT *)((char *)Vp + off[m]));
         } else if (valid[m]) {
-          __builtin_nontemporal_store(qo[0], Vp + off[m]);
+          __builtin_nontemporal_store(qo[0], (ST *)((char *)Vp + off[m]));
         }
       }
       if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
@@ -372,6 +392,12 @@ __device__ __forceinline__ void sweep_tile(
       for (int t = 0; t < 2 * R; t++)
 #pragma unroll
         for (int c = 0; c < V; c++) q[m][t][c] = q[m][t + 1][c];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+#pragma unroll
+      for (int h = 0; h < HJ; h++) hq[h][t] = hq[h][t + 1];
+      kq[t] = kq[t + 1];
+    }
   }
 
   if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {

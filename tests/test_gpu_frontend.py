@@ -94,3 +94,20 @@ def test_save_load_restart_is_exact(gpu, tmp_path, dtype):
     with gpu.Stencil(shape[0] + 1, shape[1], shape[2], npts=npts, dtype=dtype) as c:
         with pytest.raises(gpu.SthError):
             c.load(str(tmp_path))
+
+
+def test_autotune_cache_round_trip(gpu, tmp_path, monkeypatch):
+    """With STH_TUNE_CACHE set, the first autotune stores its result and the
+    second one on the same geometry applies it without timing."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("STH_TUNE_CACHE", str(cache))
+    with gpu.Stencil(30, 34, 90, npts=25) as ctx:
+        ctx.init_const(0.0, 1.0, 1.0)
+        best, ms = ctx.autotune(leapfrog=False, reps=1)
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("sth-tune 1 25 0 0 30 34 90")
+    with gpu.Stencil(30, 34, 90, npts=25) as ctx:
+        best2, ms2 = ctx.autotune(leapfrog=False, reps=1)
+        assert ctx.get_tuning()["rows_per_wave"] == best["rows_per_wave"]
+    assert best2 == best and abs(ms2 - ms) < 1e-4
+    assert len(cache.read_text().splitlines()) == 1
