@@ -94,6 +94,8 @@ typedef struct {
   uint32_t chunks;        /* i-range split: 1..64 chunks per (j,k) tile */
   uint32_t xcd_remap;     /* 1: keep neighbouring tiles on one XCD (default), 0: off */
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
+  uint32_t steps_per_pass; /* LEAPFROG, FP32: 2 = two steps per pass over memory
+                              (temporal blocking), 1 = one; 0: default */
 } sth_tuning;
 
 typedef struct sth_ctx sth_ctx;
@@ -176,8 +178,9 @@ int sth_norms(sth_ctx *ctx, sth_field f, double *maxabs, double *l2);
 int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
 int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
 /* Times the candidate schedules on this context's geometry (REF or LEAPFROG
- * sweeps, `reps` launches each, fields are modified), keeps the fastest and
- * returns it with its mean kernel time in ms.  If the environment variable
+ * sweeps, 2*reps steps each, fields are modified; FP32 LEAPFROG also tries
+ * two steps per pass), keeps the fastest and returns it with its mean time
+ * per step in ms.  If the environment variable
  * STH_TUNE_CACHE names a file, a result stored there for the same (stencil,
  * dtype, mode, xs, ys, zs) is applied without timing, and new results are
  * appended to it. */

@@ -44,7 +44,7 @@ struct BatchArgs {
 template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void batch_kernel(const BatchArgs b) {
   constexpr int R = Terms<NPTS>::R;
-  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 * V + 2 * R)];
+  __shared__ __attribute__((aligned(16))) typename Codec<FMT>::A lds[2][TileLds<NPTS, MJ, NW, V>::SIZE];
   __shared__ float red[NW];
   __shared__ int sh_item;
   for (;;) {
@@ -332,7 +332,10 @@ int sth_batch_create(const uint32_t *dims, uint32_t nshots, uint32_t npts, uint3
     sth_geom g = {dims[3 * s], dims[3 * s + 1], dims[3 * s + 2], npts, dtype};
     sth_ctx *c = nullptr;
     st = sth_create(&g, &c);
-    if (st == STH_OK) b->shots.push_back(c);
+    if (st == STH_OK) {
+      c->no_fuse = true;  // the batch's tables hold the shot's buffers
+      b->shots.push_back(c);
+    }
   }
   if (st != STH_OK) {
     sth_batch_destroy(b);

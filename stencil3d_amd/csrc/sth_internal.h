@@ -19,6 +19,12 @@ struct sth_ctx {
   hipStream_t stream = nullptr;
   void *base[3] = {nullptr, nullptr, nullptr};  // buffers a, b, s
   int ubuf = 0;                                 // buffer holding logical u
+  // Two-steps-per-pass sweep: a twin of buffers a and b with the same halo
+  // (allocated on first use); a pass writes the twins, then each buffer and
+  // its twin swap.  twins_ok: the twins' halos equal a's and b's.
+  void *twin[2] = {nullptr, nullptr};
+  bool twins_ok = false;
+  bool no_fuse = false;  // shots of a batch: their buffers sit in its tables
   // 16-bit storage scale state
   sth::Q16Slot *qs_dev = nullptr;
   sth::Q16Slot host_slot{};
@@ -76,8 +82,16 @@ struct Plan {
   unsigned grid;
 };
 
+// Launch plan of a two-steps-per-pass sweep.
+struct Plan2 {
+  int mj, nw, oj, ok, chunks, chunk_len, ntk, ntj;
+  unsigned grid;
+};
+
 int hip_status(hipError_t e);
 Plan make_plan(const sth_ctx *c, const sth_tuning &t);
+bool fused_applies(const sth_ctx *c, const sth_tuning &t);
+Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t);
 void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a);
 void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a);
 int push_slot(sth_ctx *c);

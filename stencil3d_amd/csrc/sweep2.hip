@@ -1,0 +1,34 @@
+// sweep2.hip -- launches of the two-steps-per-pass LEAPFROG sweep (FP32).
+#include "sweep2_kernel.h"
+#include "sweep2_dispatch.h"
+
+namespace sth {
+
+#define STH_FUSED2_LIST(X) X(7, 4, 16) X(13, 4, 16) X(13, 3, 16) X(25, 3, 16) X(25, 4, 12)
+
+bool fused2_supported(int npts, int mj, int nw) {
+#define STH_F2_SUP(N, MJ, NW) \
+  if (npts == N && mj == MJ && nw == NW) return true;
+  STH_FUSED2_LIST(STH_F2_SUP)
+#undef STH_F2_SUP
+  return false;
+}
+
+Fused2Geom fused2_geom(int npts, int mj, int nw) {
+  const int R = npts == 7 ? 1 : npts == 13 ? 2 : 4;
+  return {nw * mj - 2 * R, 64 - 2 * R, 64 * nw};
+}
+
+hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+                         const Sweep2Args &a) {
+#define STH_F2_LAUNCH(N, MJ, NW)                                                  \
+  if (npts == N && mj == MJ && nw == NW) {                                        \
+    hipLaunchKernelGGL((sweep2_kernel<N, MJ, NW>), dim3(grid), dim3(64 * NW), 0, st, a); \
+    return hipGetLastError();                                                     \
+  }
+  STH_FUSED2_LIST(STH_F2_LAUNCH)
+#undef STH_F2_LAUNCH
+  return hipErrorInvalidValue;
+}
+
+}  // namespace sth

@@ -1,0 +1,20 @@
+// sweep2_dispatch.h -- host-side entry to the two-steps-per-pass sweep.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace sth {
+
+struct Sweep2Args;
+
+// Output tile of a compiled (stencil, rows per wave, waves) shape.
+struct Fused2Geom {
+  int oj, ok, threads;
+};
+
+bool fused2_supported(int npts, int mj, int nw);
+Fused2Geom fused2_geom(int npts, int mj, int nw);
+hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+                         const Sweep2Args &a);
+
+}  // namespace sth

@@ -150,13 +150,25 @@ struct VecT<ST, 2> {
   __device__ static inline ST get(const type &x, int c) { return c ? x.y : x.x; }
 };
 
+// LDS plane of a tile: the centre plane plus an R-wide halo.  Column CO
+// holds the tile's first k; with two k points per lane CO is even, so every
+// lane's pair sits on an 8-byte boundary and LDS traffic moves as 2-element
+// vectors (no 2-way bank conflicts from lanes 8 bytes apart).
+template <int NPTS, int MJ, int NW, int V>
+struct TileLds {
+  static constexpr int R = Terms<NPTS>::R;
+  static constexpr int CO = V == 2 ? 2 * ((R + 1) / 2) : R;
+  static constexpr int LW = 64 * V + 2 * CO;
+  static constexpr int SIZE = (NW * MJ + 2 * R) * LW;
+};
+
 // Tile engine shared by the single-grid and the batched kernels: a
 // (TJ = NW*MJ) x (TK = 64*V) tile of one grid, planes [ib, ie).  Lane l of
 // wave w holds k = k0 + V*l .. k0 + V*l + V-1 of rows j0 + w*MJ .. + MJ-1.
 template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __device__ __forceinline__ void sweep_tile(
     const SweepArgs &a, int tk, int tj, int ib, int ie,
-    typename Codec<FMT>::A (*lds)[(NW * MJ + 2 * Terms<NPTS>::R) * (64 * V + 2 * Terms<NPTS>::R)],
+    typename Codec<FMT>::A (*lds)[TileLds<NPTS, MJ, NW, V>::SIZE],
     float *red) {
   using Tm = Terms<NPTS>;
   constexpr int R = Tm::R, T = Tm::T;
@@ -165,7 +177,13 @@ __device__ __forceinline__ void sweep_tile(
   using AT = typename CD::A;  // arithmetic type: FP32, or FP64 for FP64 storage
   using VV = VecT<ST, V>;
   using VT = typename VV::type;
-  constexpr int TJ = NW * MJ, TK = 64 * V, LW = TK + 2 * R;
+  // global (address space 1) views: plane bases are byte pointers
+  using GB = __attribute__((address_space(1))) char;
+  using GPtr = GB *;
+  using GV = __attribute__((address_space(1))) VT;
+  using GS = __attribute__((address_space(1))) ST;
+  constexpr int TJ = NW * MJ, TK = 64 * V;
+  constexpr int CO = TileLds<NPTS, MJ, NW, V>::CO, LW = TileLds<NPTS, MJ, NW, V>::LW;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // j-halo rows loaded per wave
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
   static_assert(V == 1 || V == 2, "1 or 2 k points per lane");
@@ -231,22 +249,29 @@ __device__ __forceinline__ void sweep_tile(
     const int row = tid / (2 * R), c = tid % (2 * R);
     const int tcol = c < R ? c - R : TK + c - R;
     khoff = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R)) * sizeof(ST));
-    khpos = (row + R) * LW + (c < R ? c : TK + c);
+    khpos = (row + R) * LW + (c < R ? CO - R + c : CO + TK + c - R);
   }
 
   // In-tile offsets are 32-bit byte offsets from a workgroup-uniform plane
-  // base, so loads and stores take the SGPR-base + VGPR-offset form.
-  auto vload = [](const ST *p, uint32_t ob) { return *(const VT *)((const char *)p + ob); };
-  auto vload_nt = [](const ST *p, uint32_t ob) {
-    return __builtin_nontemporal_load((const VT *)((const char *)p + ob));
+  // base, so loads and stores take the SGPR-base + VGPR-offset form.  The
+  // base goes through readfirstlane each plane: it stays a scalar, and the
+  // compiler does not turn base + offset into per-lane 64-bit pointers
+  // stepped by the plane stride.
+  auto uni = [](const void *p) {
+    const uint64_t x = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return (GPtr)(((uint64_t)hi << 32) | lo);
   };
-  auto sload = [](const ST *p, uint32_t ob) { return *(const ST *)((const char *)p + ob); };
+  auto vload = [](GPtr p, uint32_t ob) { return *(const GV *)(p + ob); };
+  auto vload_nt = [](GPtr p, uint32_t ob) { return __builtin_nontemporal_load((const GV *)(p + ob)); };
+  auto sload = [](GPtr p, uint32_t ob) { return *(const GS *)(p + ob); };
 
   // Register queue: q[m][t][c] = u at plane i - R + t.
   AT q[MJ][2 * R + 1][V];
 #pragma unroll
   for (int t = 0; t < 2 * R; t++) {
-    const ST *Up = Ub + (int64_t)(ib - R + t) * P;
+    const GPtr Up = uni(Ub + (int64_t)(ib - R + t) * P);
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       const VT x = vload(Up, off[m]);
@@ -265,70 +290,116 @@ __device__ __forceinline__ void sweep_tile(
   ST kq[R + 1];
 #pragma unroll
   for (int t = 0; t < R; t++) {
-    const ST *Up = Ub + (int64_t)(ib + t) * P;
+    const GPtr Up = uni(Ub + (int64_t)(ib + t) * P);
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][t] = hrow[h] >= 0 ? vload(Up, hoff[h]) : VT(0);
     kq[t] = has_kh ? sload(Up, khoff) : ST(0);
   }
-  VT nu[MJ], ns[MJ], nv[MJ], nhj[HJ];
-  ST nhk;
+  // Loads run two planes ahead: two sets of staging registers, used in
+  // turn by the even and odd steps of a plane pair, so a set is refilled
+  // only after it has been consumed (no copy of registers still in flight).
+  struct Staged {
+    VT u[MJ], s[MJ], v[MJ], hj[HJ];
+    ST hk;
+  };
+  Staged PA, PB;
   // s and v are touched once per sweep: nontemporal loads and stores keep
   // them from displacing u's lines, which neighbouring tiles re-read.
-  auto prefetch = [&](int i) {
-    const ST *Up = Ub + (int64_t)(i + R) * P;
-    const ST *Sp = Sb + (int64_t)i * P;
-    const ST *Vp = Vb + (int64_t)i * P;
+  auto prefetch = [&](int i, Staged &n) __attribute__((always_inline)) {
+    const GPtr Up = uni(Ub + (int64_t)(i + R) * P);
+    const GPtr Sp = uni(Sb + (int64_t)i * P);
+    const GPtr Vp = uni(Vb + (int64_t)i * P);
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       // masked lanes read their clamped (in-bounds) cells: no branches
-      nu[m] = vload(Up, off[m]);
-      ns[m] = vload_nt(Sp, off[m]);
-      nv[m] = vload_nt(Vp, off[m]);
+      n.u[m] = vload(Up, off[m]);
+      n.s[m] = vload_nt(Sp, off[m]);
+      n.v[m] = vload_nt(Vp, off[m]);
     }
 #pragma unroll
-    for (int h = 0; h < HJ; h++) nhj[h] = hrow[h] >= 0 ? vload(Up, hoff[h]) : VT(0);
-    nhk = has_kh ? sload(Up, khoff) : ST(0);
+    for (int h = 0; h < HJ; h++) n.hj[h] = hrow[h] >= 0 ? vload(Up, hoff[h]) : VT(0);
+    n.hk = has_kh ? sload(Up, khoff) : ST(0);
   };
-  prefetch(ib);
+  prefetch(ib, PA);
+  if (ib + 1 < ie) prefetch(ib + 1, PB);
 
   float mx = 0.0f;
-  for (int i = ib; i < ie; i++) {
+  auto step = [&](int i, Staged &n) __attribute__((always_inline)) {
     AT *L = lds[(i - ib) & 1];
     VT cs[MJ], cv[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
 #pragma unroll
-      for (int c = 0; c < V; c++) q[m][2 * R][c] = CD::dec(VV::get(nu[m], c));
-      cs[m] = ns[m];
-      cv[m] = nv[m];
+      for (int c = 0; c < V; c++) q[m][2 * R][c] = CD::dec(VV::get(n.u[m], c));
+      cs[m] = n.s[m];
+      cv[m] = n.v[m];
     }
 #pragma unroll
-    for (int h = 0; h < HJ; h++) hq[h][R] = nhj[h];
-    kq[R] = nhk;
-    if (i + 1 < ie) prefetch(i + 1);
+    for (int h = 0; h < HJ; h++) hq[h][R] = n.hj[h];
+    kq[R] = n.hk;
+    if (i + 2 < ie) prefetch(i + 2, n);
 
 #pragma unroll
     for (int m = 0; m < MJ; m++)
 #pragma unroll
-      for (int c = 0; c < V; c++) L[(w * MJ + m + R) * LW + lane * V + c + R] = q[m][R][c];
+      for (int c = 0; c < V; c++) L[(w * MJ + m + R) * LW + lane * V + c + CO] = q[m][R][c];
 #pragma unroll
     for (int h = 0; h < HJ; h++)
       if (hrow[h] >= 0)
 #pragma unroll
         for (int c = 0; c < V; c++)
-          L[hrow[h] * LW + lane * V + c + R] = CD::dec(VV::get(hq[h][0], c));
+          L[hrow[h] * LW + lane * V + c + CO] = CD::dec(VV::get(hq[h][0], c));
     if (has_kh) L[khpos] = CD::dec(kq[0]);
     __syncthreads();
 
-    ST *Vp = Vb + (int64_t)i * P;
+    GPtr Vp = uni(Vb + (int64_t)i * P);
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      const int lr = w * MJ + m + R, col = lane * V + R;
+      const int lr = w * MJ + m + R, col = lane * V + CO;
+      // Two k points per lane: the k- and j-neighbours come from LDS as
+      // aligned 2-element vectors, read once for both points.
+      // kn[t] = column col - 2 KH + t; jl[r] / jh[r] = rows lr -/+ r.
+      using AT2 = AT __attribute__((ext_vector_type(2)));
+      constexpr int KH = (R + 1) / 2;
+      AT kn[V == 2 ? 4 * KH + 2 : 1];
+      AT jl[R + 1][V], jh[R + 1][V];
+      if constexpr (V == 2) {
+#pragma unroll
+        for (int p = -KH; p <= KH; p++) {
+          if (p == 0) {
+            kn[2 * KH] = q[m][R][0];
+            kn[2 * KH + 1] = q[m][R][1];
+          } else {
+            const AT2 x = *(const AT2 *)&L[lr * LW + col + 2 * p];
+            kn[2 * (p + KH)] = x.x;
+            kn[2 * (p + KH) + 1] = x.y;
+          }
+        }
+#pragma unroll
+        for (int r = 1; r <= R; r++) {
+          if (m - r < 0) {
+            const AT2 x = *(const AT2 *)&L[(lr - r) * LW + col];
+            jl[r][0] = x.x;
+            jl[r][1] = x.y;
+          }
+          if (m + r >= MJ) {
+            const AT2 x = *(const AT2 *)&L[(lr + r) * LW + col];
+            jh[r][0] = x.x;
+            jh[r][1] = x.y;
+          }
+        }
+      }
       AT res[V];
 #pragma unroll
       for (int c = 0; c < V; c++) {
         auto pair = [&](int ax, int r) -> AT {
           if (ax == 0) return q[m][R - r][c] + q[m][R + r][c];
+          if constexpr (V == 2) {
+            if (ax == 2) return kn[2 * KH + c - r] + kn[2 * KH + c + r];
+            const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R][c] : jl[r][c];
+            const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R][c] : jh[r][c];
+            return lo + hi;
+          }
           if (ax == 2) {
             // k-neighbours inside the lane's own vector come from registers
             const AT lo = (c - r >= 0) ? q[m][R][(c - r) >= 0 ? c - r : 0]
@@ -369,15 +440,15 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
       for (int c = 0; c < V; c++) qo[c] = CD::enc(res[c]);
       if constexpr (V == 1) {
-        if (valid[m]) __builtin_nontemporal_store(qo[0], (ST *)((char *)Vp + off[m]));
+        if (valid[m]) __builtin_nontemporal_store(qo[0], (GS *)(Vp + off[m]));
       } else {
         if (valid2[m]) {
           VT o;
           o.x = qo[0];
           o.y = qo[1];
-          __builtin_nontemporal_store(o, (VT *)((char *)Vp + off[m]));
+          __builtin_nontemporal_store(o, (GV *)(Vp + off[m]));
         } else if (valid[m]) {
-          __builtin_nontemporal_store(qo[0], (ST *)((char *)Vp + off[m]));
+          __builtin_nontemporal_store(qo[0], (GS *)(Vp + off[m]));
         }
       }
       if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
@@ -398,6 +469,10 @@ __device__ __forceinline__ void sweep_tile(
       for (int h = 0; h < HJ; h++) hq[h][t] = hq[h][t + 1];
       kq[t] = kq[t + 1];
     }
+  };
+  for (int i = ib; i < ie; i += 2) {
+    step(i, PA);
+    if (i + 1 < ie) step(i + 1, PB);
   }
 
   if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
@@ -435,7 +510,7 @@ __device__ inline void q16_publish(const SweepArgs &a) {
 template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
   constexpr int R = Terms<NPTS>::R;
-  __shared__ typename Codec<FMT>::A lds[2][(NW * MJ + 2 * R) * (64 * V + 2 * R)];
+  __shared__ __attribute__((aligned(16))) typename Codec<FMT>::A lds[2][TileLds<NPTS, MJ, NW, V>::SIZE];
   __shared__ float red[NW];
   q16_publish<FMT>(a);
   int bid = blockIdx.x;

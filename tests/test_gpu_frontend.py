@@ -105,7 +105,7 @@ def test_autotune_cache_round_trip(gpu, tmp_path, monkeypatch):
         ctx.init_const(0.0, 1.0, 1.0)
         best, ms = ctx.autotune(leapfrog=False, reps=1)
     lines = cache.read_text().splitlines()
-    assert len(lines) == 1 and lines[0].startswith("sth-tune 1 25 0 0 30 34 90")
+    assert len(lines) == 1 and lines[0].startswith("sth-tune 2 25 0 0 30 34 90")
     with gpu.Stencil(30, 34, 90, npts=25) as ctx:
         best2, ms2 = ctx.autotune(leapfrog=False, reps=1)
         assert ctx.get_tuning()["rows_per_wave"] == best["rows_per_wave"]

@@ -1,0 +1,123 @@
+"""GPU parity of the two-steps-per-pass LEAPFROG sweep (FP32, temporal
+blocking).  A pass must be bit-identical to two one-step sweeps of the
+oracle, halos included: the fields carry nonzero halo values, which the
+one-step scheme never writes and which a pass must leave where the one-step
+scheme leaves them.  Covered: every compiled fused shape, i-chunks, odd step
+counts (the last step runs one-step), one-step calls and uploads between
+calls, and shapes that are not multiples of the output tile."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FUSED = {7: [(4, 16)], 13: [(4, 16), (3, 16)], 25: [(3, 16), (4, 12)]}
+SHAPES = [(20, 24, 36), (37, 45, 70), (9, 70, 130)]
+
+
+def seeded(orc, g, seed, halo):
+    u = orc.fill_uniform(g, seed, 0, halo=halo)
+    v = orc.fill_uniform(g, seed, 1, halo=-halo)
+    s = orc.fill_uniform(g, seed, 2, halo=2 * halo)
+    return u, v, s
+
+
+def coeffs(orc, npts):
+    c1 = np.float32(0.25 if npts != 25 else 0.16)
+    return float(np.float32(2.0 - orc.lap_gain(npts) * float(c1))), float(c1)
+
+
+def same_bits(a, b):
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def oracle_run(orc, npts, g, u, v, s, c0, c1, c2, steps):
+    a, b = u.copy(), v.copy()
+    newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, steps)
+    return (a, b) if newest == 0 else (b, a)
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_two_step_passes_bit_exact(gpu, orc, npts, shape):
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 3, 0.375)
+    c0, c1 = coeffs(orc, npts)
+    steps = 5
+    c2 = (np.arange(1, steps + 1, dtype=np.float32) * np.float32(1e-3)).astype(np.float32)
+    want_u, want_v = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, steps)
+    for mj, nw in FUSED[npts]:
+        for chunks in (0, 2, 3):
+            with gpu.Stencil(*shape, npts=npts) as ctx:
+                ctx.set_tuning(mj, nw, chunks, steps_per_pass=2)
+                t = ctx.get_tuning()
+                assert t["steps_per_pass"] == 2 and (t["rows_per_wave"], t["waves"]) == (mj, nw)
+                ctx.upload("u", u, R)
+                ctx.upload("v", v, R)
+                ctx.upload("s", s, R)
+                ctx.timer_start()
+                ctx.leapfrog(steps, c0, c1, c2)
+                _, launches = ctx.timer_read()
+                assert launches == 3  # two passes + one one-step sweep
+                gu = ctx.download("u", R)
+                gv = ctx.download("v", R)
+            assert same_bits(gu, want_u), (mj, nw, chunks)
+            assert same_bits(gv, want_v), (mj, nw, chunks)
+
+
+@pytest.mark.parametrize("npts", [13, 25])
+def test_passes_mixed_with_single_steps_and_uploads(gpu, orc, npts):
+    """Two passes, one one-step sweep, a new v (other halo values), two more
+    passes: the twin buffers must pick up the new halo."""
+    shape = (26, 31, 67)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 11, 0.25)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.full(4, 2e-3, np.float32)
+    # oracle: 4 steps, 1 step, replace v, 4 steps
+    a, b = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 4)
+    a, b = oracle_run(orc, npts, g, a, b, s, c0, c1, c2[:1], 1)
+    v2 = orc.fill_uniform(g, 12, 1, halo=0.625)
+    want_u, want_v = oracle_run(orc, npts, g, a, v2, s, c0, c1, c2, 4)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.set_tuning(steps_per_pass=2)
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.leapfrog(4, c0, c1, c2)
+        ctx.set_tuning(steps_per_pass=1)
+        ctx.leapfrog(1, c0, c1, c2[:1])
+        ctx.set_tuning(steps_per_pass=2)
+        ctx.upload("v", v2, R)
+        ctx.leapfrog(4, c0, c1, c2)
+        gu = ctx.download("u", R)
+        gv = ctx.download("v", R)
+    assert same_bits(gu, want_u)
+    assert same_bits(gv, want_v)
+
+
+def test_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
+    """A larger grid (several tiles and chunks in every direction): 8 steps
+    as passes and as one-step sweeps give the same bits."""
+    shape = (96, 100, 200)
+    npts, R = 25, 4
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-3, 1e-3, 8).astype(np.float32)
+    out = []
+    for spp in (1, 2):
+        with gpu.Stencil(*shape, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            ctx.init_uniform("u", 5, 0, -1.0, 1.0, 0.5)
+            ctx.init_uniform("v", 5, 1, -1.0, 1.0, -0.5)
+            ctx.init_uniform("s", 5, 2, -1.0, 1.0, 0.25)
+            ctx.leapfrog(8, c0, c1, c2)
+            out.append((ctx.download("u", R), ctx.download("v", R)))
+    assert same_bits(out[0][0], out[1][0])
+    assert same_bits(out[0][1], out[1][1])
+
+
+def test_two_steps_rejected_for_16bit(gpu):
+    with gpu.Stencil(16, 16, 64, npts=25, dtype="fp16") as ctx:
+        with pytest.raises(gpu.SthError):
+            ctx.set_tuning(steps_per_pass=2)
