@@ -150,6 +150,26 @@ struct VecT<ST, 2> {
   __device__ static inline ST get(const type &x, int c) { return c ? x.y : x.x; }
 };
 
+// One register-queue entry: a lane's V values of one plane.  With V = 2 the
+// pair is one 2-element vector, so shifting the queue moves 64-bit registers.
+template <class A, int V>
+struct QCell {
+  A e[1];
+  __device__ A get(int) const { return e[0]; }
+  __device__ void set(int, A x) { e[0] = x; }
+};
+template <class A>
+struct QCell<A, 2> {
+  A __attribute__((ext_vector_type(2))) e;
+  __device__ A get(int c) const { return c ? e.y : e.x; }
+  __device__ void set(int c, A x) {
+    if (c)
+      e.y = x;
+    else
+      e.x = x;
+  }
+};
+
 // LDS plane of a tile: the centre plane plus an R-wide halo.  Column CO
 // holds the tile's first k; with two k points per lane CO is even, so every
 // lane's pair sits on an 8-byte boundary and LDS traffic moves as 2-element
@@ -267,8 +287,8 @@ __device__ __forceinline__ void sweep_tile(
   auto vload_nt = [](GPtr p, uint32_t ob) { return __builtin_nontemporal_load((const GV *)(p + ob)); };
   auto sload = [](GPtr p, uint32_t ob) { return *(const GS *)(p + ob); };
 
-  // Register queue: q[m][t][c] = u at plane i - R + t.
-  AT q[MJ][2 * R + 1][V];
+  // Register queue: q[m][t].get(c) = u at plane i - R + t.
+  QCell<AT, V> q[MJ][2 * R + 1];
 #pragma unroll
   for (int t = 0; t < 2 * R; t++) {
     const GPtr Up = uni(Ub + (int64_t)(ib - R + t) * P);
@@ -276,7 +296,7 @@ __device__ __forceinline__ void sweep_tile(
     for (int m = 0; m < MJ; m++) {
       const VT x = vload(Up, off[m]);
 #pragma unroll
-      for (int c = 0; c < V; c++) q[m][t][c] = CD::dec(VV::get(x, c));
+      for (int c = 0; c < V; c++) q[m][t].set(c, CD::dec(VV::get(x, c)));
     }
   }
   // Every u value of a plane -- the tile's own rows, its j-halo rows and its
@@ -330,7 +350,7 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
 #pragma unroll
-      for (int c = 0; c < V; c++) q[m][2 * R][c] = CD::dec(VV::get(n.u[m], c));
+      for (int c = 0; c < V; c++) q[m][2 * R].set(c, CD::dec(VV::get(n.u[m], c)));
       cs[m] = n.s[m];
       cv[m] = n.v[m];
     }
@@ -342,7 +362,7 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
     for (int m = 0; m < MJ; m++)
 #pragma unroll
-      for (int c = 0; c < V; c++) L[(w * MJ + m + R) * LW + lane * V + c + CO] = q[m][R][c];
+      for (int c = 0; c < V; c++) L[(w * MJ + m + R) * LW + lane * V + c + CO] = q[m][R].get(c);
 #pragma unroll
     for (int h = 0; h < HJ; h++)
       if (hrow[h] >= 0)
@@ -367,8 +387,8 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
         for (int p = -KH; p <= KH; p++) {
           if (p == 0) {
-            kn[2 * KH] = q[m][R][0];
-            kn[2 * KH + 1] = q[m][R][1];
+            kn[2 * KH] = q[m][R].get(0);
+            kn[2 * KH + 1] = q[m][R].get(1);
           } else {
             const AT2 x = *(const AT2 *)&L[lr * LW + col + 2 * p];
             kn[2 * (p + KH)] = x.x;
@@ -393,28 +413,28 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
       for (int c = 0; c < V; c++) {
         auto pair = [&](int ax, int r) -> AT {
-          if (ax == 0) return q[m][R - r][c] + q[m][R + r][c];
+          if (ax == 0) return q[m][R - r].get(c) + q[m][R + r].get(c);
           if constexpr (V == 2) {
             if (ax == 2) return kn[2 * KH + c - r] + kn[2 * KH + c + r];
-            const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R][c] : jl[r][c];
-            const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R][c] : jh[r][c];
+            const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R].get(c) : jl[r][c];
+            const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R].get(c) : jh[r][c];
             return lo + hi;
           }
           if (ax == 2) {
             // k-neighbours inside the lane's own vector come from registers
-            const AT lo = (c - r >= 0) ? q[m][R][(c - r) >= 0 ? c - r : 0]
+            const AT lo = (c - r >= 0) ? q[m][R].get((c - r) >= 0 ? c - r : 0)
                                        : L[lr * LW + col + c - r];
-            const AT hi = (c + r < V) ? q[m][R][(c + r) < V ? c + r : 0]
+            const AT hi = (c + r < V) ? q[m][R].get((c + r) < V ? c + r : 0)
                                       : L[lr * LW + col + c + r];
             return lo + hi;
           }
-          const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R][c]
+          const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R].get(c)
                                      : L[(lr - r) * LW + col + c];
-          const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R][c]
+          const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R].get(c)
                                      : L[(lr + r) * LW + col + c];
           return lo + hi;
         };
-        const AT u0 = q[m][R][c];
+        const AT u0 = q[m][R].get(c);
         const AT s0 = CD::dec(VV::get(cs[m], c));
         const AT v0 = CD::dec(VV::get(cv[m], c));
         AT out;
@@ -460,9 +480,7 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
     for (int m = 0; m < MJ; m++)
 #pragma unroll
-      for (int t = 0; t < 2 * R; t++)
-#pragma unroll
-        for (int c = 0; c < V; c++) q[m][t][c] = q[m][t + 1][c];
+      for (int t = 0; t < 2 * R; t++) q[m][t] = q[m][t + 1];
 #pragma unroll
     for (int t = 0; t < R; t++) {
 #pragma unroll
