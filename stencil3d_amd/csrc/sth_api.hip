@@ -310,10 +310,11 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
 }
 
 // Two-steps-per-pass sweep (FP32 LEAPFROG).  Default for 7 and 13 points
-// on grids of at least 64 output tiles (measured at 1040^3 on MI355X: +11 %
-// and +17 % over the one-step sweep); 25 points keep the one-step sweep, as
-// the 25-point pass (one 16-wave workgroup per CU, 38 LDS reads per output
-// point) is latency-bound below it.  Never for the shots of a batch.
+// on grids of at least 64 output tiles (measured at 1040^3 on MI355X: +31 %
+// and +18 % over the one-step sweep); 25 points keep the one-step sweep, as
+// the 25-point pass (one 16-wave workgroup per CU, 1.37x recomputed first
+// step, 38 LDS reads per output point) stays below it.  Never for the shots
+// of a batch.
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (c->fmt != FMT_F32 || c->no_fuse) return false;
   if (t.steps_per_pass) return t.steps_per_pass == 2;
@@ -342,10 +343,10 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   const int xs = (int)c->g.xs, tiles = p.ntk * p.ntj;
   int ch = (int)t.chunks;
   if (ch == 0) {
-    // one workgroup per CU fits (LDS): about eight rounds of workgroups, so
+    // one workgroup per CU fits (LDS): about twelve rounds of workgroups, so
     // the last round is a small tail, and chunks of at least 8R planes (each
     // chunk recomputes 2R planes of the first step, re-reads 4R planes of u).
-    ch = std::max(1, (8 * 256 + tiles - 1) / tiles);
+    ch = std::max(1, (12 * 256 + tiles - 1) / tiles);
     ch = std::min(ch, std::max(1, xs / (8 * c->R)));
   }
   ch = std::max(1, std::min(ch, xs));
@@ -1043,7 +1044,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
                                  {4, 8, 1}, {4, 2, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2},
                                  {2, 8, 2}, {4, 8, 2}};
   static const int chunk_opts[] = {0, 8, 16, 32};
-  static const int fused_shapes[][2] = {{3, 16}, {4, 16}, {4, 12}};
+  static const int fused_shapes[][2] = {{3, 16}, {4, 16}, {4, 12}, {4, 8}, {3, 8}};
   static const int fused_chunks[] = {0, 2, 4, 8};
   const sth_tuning saved = c->tune;
   std::vector<sth_tuning> cand;

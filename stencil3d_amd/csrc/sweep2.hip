@@ -4,7 +4,8 @@
 
 namespace sth {
 
-#define STH_FUSED2_LIST(X) X(7, 4, 16) X(13, 4, 16) X(13, 3, 16) X(25, 3, 16) X(25, 4, 12)
+#define STH_FUSED2_LIST(X) \
+  X(7, 4, 16) X(7, 4, 8) X(13, 4, 16) X(13, 3, 16) X(13, 4, 8) X(25, 3, 16) X(25, 4, 12) X(25, 3, 8) X(25, 4, 8)
 
 bool fused2_supported(int npts, int mj, int nw) {
 #define STH_F2_SUP(N, MJ, NW) \
@@ -15,8 +16,12 @@ bool fused2_supported(int npts, int mj, int nw) {
 }
 
 Fused2Geom fused2_geom(int npts, int mj, int nw) {
-  const int R = npts == 7 ? 1 : npts == 13 ? 2 : 4;
-  return {nw * mj - 2 * R, 64 - 2 * R, 64 * nw};
+#define STH_F2_GEOM(N, MJ, NW)                                         \
+  if (npts == N && mj == MJ && nw == NW)                               \
+    return {Fused2Shape<N, MJ, NW>::OJ, Fused2Shape<N, MJ, NW>::OK, 64 * NW};
+  STH_FUSED2_LIST(STH_F2_GEOM)
+#undef STH_F2_GEOM
+  return {0, 0, 0};
 }
 
 hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st,

@@ -10,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-FUSED = {7: [(4, 16)], 13: [(4, 16), (3, 16)], 25: [(3, 16), (4, 12)]}
+FUSED = {7: [(4, 16), (4, 8)], 13: [(4, 16), (3, 16), (4, 8)], 25: [(3, 16), (4, 12), (3, 8), (4, 8)]}
 SHAPES = [(20, 24, 36), (37, 45, 70), (9, 70, 130)]
 
 
