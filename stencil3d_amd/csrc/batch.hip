@@ -65,7 +65,7 @@ __global__ __launch_bounds__(64 * NW) void batch_kernel(const BatchArgs b) {
     a.c2n = b.c2n;
     const int ib = it.ch * a.chunk_len;
     const int ie = min(a.xs, ib + a.chunk_len);
-    sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, it.tk, it.tj, ib, ie, lds, red);
+    sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, it.tk * 64 * V, it.tj * NW * MJ, ib, ie, lds, red);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -153,6 +153,12 @@ TileSel tile_of(const sth_tuning &t) {
 // `planes` planes.
 Plan shot_plan(const sth_batch *b, const sth_ctx *c) {
   sth_tuning t = b->tune;
+  // the persistent kernel's tile (tile_of): name it, so the plan does not
+  // take the single-grid size-dependent default
+  const TileSel ts = tile_of(b->tune);
+  t.rows_per_wave = (uint32_t)ts.mj;
+  t.waves = (uint32_t)ts.nw;
+  t.vec = (uint32_t)ts.vec;
   const int xs = (int)c->g.xs;
   const int planes = b->planes ? (int)b->planes : xs;
   t.chunks = (uint32_t)std::min(64, std::max(1, (xs + planes - 1) / planes));
@@ -277,7 +283,7 @@ int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float 
       args[s].c1 = c1;
       args[s].c2n = c2 ? c2[n] : 0.0f;
       STH_TRY(launch_sweep((int)b->npts, MODE_LEAP, FMT_F32, plans[s].mj, plans[s].nw,
-                           plans[s].vec, plans[s].grid, b->stream, args[s]));
+                           plans[s].vec, plans[s].edge, plans[s].grid, b->stream, args[s]));
       (*launches)++;
     }
   }

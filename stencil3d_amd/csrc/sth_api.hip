@@ -263,6 +263,18 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   p.chunk_len = (xs + ch - 1) / ch;
   p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
   p.grid = (unsigned)(p.ntk * p.ntj * p.chunks);
+  // A last k-tile with at most 64/f columns runs folded (f rows per wave):
+  // at 1040 columns and 128-column tiles it holds 16 columns, and unfolded it
+  // would cost as much as a full tile.
+  p.edge = 0;
+  const int we = (int)c->g.zs - (p.ntk - 1) * 64 * p.vec;
+  if (c->fmt != FMT_F64 && we < 64 * p.vec)
+    for (int f : {4, 2, 1})
+      if (64 / f >= we && p.mj % f == 0 && (f > 1 || p.vec == 2) &&
+          edge_supported(p.mj, p.nw, p.vec, f)) {
+        p.edge = f;
+        break;
+      }
   return p;
 }
 
@@ -291,7 +303,7 @@ void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a) {
 
 int launch_one(sth_ctx *c, const Plan &p, int mode, const SweepArgs &a) {
   const hipError_t e =
-      launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.grid, c->stream, a);
+      launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.edge, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (c->timing) c->launches++;
   return STH_OK;

@@ -79,6 +79,7 @@ inline TileShape default_tile(int fmt, uint32_t ys, uint32_t zs, const sth_tunin
 // Launch plan of one sweep: tile (rows per wave x waves), i-chunks, grid.
 struct Plan {
   int mj, nw, vec, chunks, chunk_len, ntk, ntj;
+  int edge;  // > 0: the last k-tile runs folded, `edge` rows per wave
   unsigned grid;
 };
 

@@ -3,9 +3,9 @@
 
 namespace sth {
 
-hipError_t launch_sweep_13(int mode, int fmt, int mj, int nw, int vec, unsigned grid,
+hipError_t launch_sweep_13(int mode, int fmt, int mj, int nw, int vec, int ef, unsigned grid,
                           hipStream_t st, const SweepArgs &a) {
-  return launch_sweep_n<13>(mode, fmt, mj, nw, vec, grid, st, a);
+  return launch_sweep_n<13>(mode, fmt, mj, nw, vec, ef, grid, st, a);
 }
 
 }  // namespace sth

@@ -11,6 +11,17 @@ namespace sth {
   X(2, 2, 1) X(4, 2, 1) X(8, 2, 1) X(1, 4, 2) X(2, 4, 2) X(4, 4, 2) X(2, 8, 2) X(4, 8, 2) \
   X(2, 2, 2) X(4, 2, 2)
 
+// Folded last k-tile (rows_per_wave, waves, k points per lane, fold) variants
+// of the default tiles.
+#define STH_EDGE_LIST(X) X(4, 8, 2, 1) X(4, 8, 2, 2) X(4, 8, 2, 4) X(2, 8, 2, 1) X(2, 8, 2, 2)
+
+inline bool edge_supported(int mj, int nw, int vec, int ef) {
+#define STH_EDGE_OK(MJ, NW, VV, EF) if (mj == MJ && nw == NW && vec == VV && ef == EF) return true;
+  STH_EDGE_LIST(STH_EDGE_OK)
+#undef STH_EDGE_OK
+  return false;
+}
+
 inline bool tile_supported(int mj, int nw, int vec) {
 #define STH_TILE_OK(MJ, NW, VV) if (mj == MJ && nw == NW && vec == VV) return true;
   STH_TILE_LIST(STH_TILE_OK)
@@ -19,8 +30,19 @@ inline bool tile_supported(int mj, int nw, int vec) {
 }
 
 template <int NPTS, int MODE, int FMT>
-hipError_t launch_tile(int mj, int nw, int vec, unsigned grid, hipStream_t st,
+hipError_t launch_tile(int mj, int nw, int vec, int ef, unsigned grid, hipStream_t st,
                        const SweepArgs &a) {
+  if constexpr (FMT != FMT_F64) {
+#define STH_EDGE_CASE(MJ, NW, VV, EF)                                                 \
+  if (ef == EF && mj == MJ && nw == NW && vec == VV) {                                \
+    hipLaunchKernelGGL((sweep_kernel<NPTS, MODE, FMT, MJ, NW, VV, EF>), dim3(grid),   \
+                       dim3(64 * NW), 0, st, a);                                      \
+    return hipGetLastError();                                                         \
+  }
+    STH_EDGE_LIST(STH_EDGE_CASE)
+#undef STH_EDGE_CASE
+  }
+  if (ef != 0) return hipErrorInvalidValue;
 #define STH_TILE_CASE(MJ, NW, VV)                                                     \
   if (mj == MJ && nw == NW && vec == VV) {                                            \
     hipLaunchKernelGGL((sweep_kernel<NPTS, MODE, FMT, MJ, NW, VV>), dim3(grid),       \
@@ -33,34 +55,34 @@ hipError_t launch_tile(int mj, int nw, int vec, unsigned grid, hipStream_t st,
 }
 
 template <int NPTS>
-hipError_t launch_sweep_n(int mode, int fmt, int mj, int nw, int vec, unsigned grid,
+hipError_t launch_sweep_n(int mode, int fmt, int mj, int nw, int vec, int ef, unsigned grid,
                           hipStream_t st, const SweepArgs &a) {
   if (mode == MODE_REF) {
-    if (fmt == FMT_F32) return launch_tile<NPTS, MODE_REF, FMT_F32>(mj, nw, vec, grid, st, a);
-    if (fmt == FMT_F64) return launch_tile<NPTS, MODE_REF, FMT_F64>(mj, nw, vec, grid, st, a);
+    if (fmt == FMT_F32) return launch_tile<NPTS, MODE_REF, FMT_F32>(mj, nw, vec, ef, grid, st, a);
+    if (fmt == FMT_F64) return launch_tile<NPTS, MODE_REF, FMT_F64>(mj, nw, vec, ef, grid, st, a);
   }
   if (mode == MODE_LEAP) {
-    if (fmt == FMT_F32) return launch_tile<NPTS, MODE_LEAP, FMT_F32>(mj, nw, vec, grid, st, a);
-    if (fmt == FMT_F64) return launch_tile<NPTS, MODE_LEAP, FMT_F64>(mj, nw, vec, grid, st, a);
-    if (fmt == FMT_F16) return launch_tile<NPTS, MODE_LEAP, FMT_F16>(mj, nw, vec, grid, st, a);
-    if (fmt == FMT_BF16) return launch_tile<NPTS, MODE_LEAP, FMT_BF16>(mj, nw, vec, grid, st, a);
+    if (fmt == FMT_F32) return launch_tile<NPTS, MODE_LEAP, FMT_F32>(mj, nw, vec, ef, grid, st, a);
+    if (fmt == FMT_F64) return launch_tile<NPTS, MODE_LEAP, FMT_F64>(mj, nw, vec, ef, grid, st, a);
+    if (fmt == FMT_F16) return launch_tile<NPTS, MODE_LEAP, FMT_F16>(mj, nw, vec, ef, grid, st, a);
+    if (fmt == FMT_BF16) return launch_tile<NPTS, MODE_LEAP, FMT_BF16>(mj, nw, vec, ef, grid, st, a);
   }
   return hipErrorInvalidValue;
 }
 
 // Defined in sweep_7.hip, sweep_13.hip, sweep_25.hip (one compile unit each).
-hipError_t launch_sweep_7(int mode, int fmt, int mj, int nw, int vec, unsigned grid,
+hipError_t launch_sweep_7(int mode, int fmt, int mj, int nw, int vec, int ef, unsigned grid,
                           hipStream_t st, const SweepArgs &a);
-hipError_t launch_sweep_13(int mode, int fmt, int mj, int nw, int vec, unsigned grid,
+hipError_t launch_sweep_13(int mode, int fmt, int mj, int nw, int vec, int ef, unsigned grid,
                            hipStream_t st, const SweepArgs &a);
-hipError_t launch_sweep_25(int mode, int fmt, int mj, int nw, int vec, unsigned grid,
+hipError_t launch_sweep_25(int mode, int fmt, int mj, int nw, int vec, int ef, unsigned grid,
                            hipStream_t st, const SweepArgs &a);
 
-inline hipError_t launch_sweep(int npts, int mode, int fmt, int mj, int nw, int vec,
+inline hipError_t launch_sweep(int npts, int mode, int fmt, int mj, int nw, int vec, int ef,
                                unsigned grid, hipStream_t st, const SweepArgs &a) {
-  if (npts == 7) return launch_sweep_7(mode, fmt, mj, nw, vec, grid, st, a);
-  if (npts == 13) return launch_sweep_13(mode, fmt, mj, nw, vec, grid, st, a);
-  if (npts == 25) return launch_sweep_25(mode, fmt, mj, nw, vec, grid, st, a);
+  if (npts == 7) return launch_sweep_7(mode, fmt, mj, nw, vec, ef, grid, st, a);
+  if (npts == 13) return launch_sweep_13(mode, fmt, mj, nw, vec, ef, grid, st, a);
+  if (npts == 25) return launch_sweep_25(mode, fmt, mj, nw, vec, ef, grid, st, a);
   return hipErrorInvalidValue;
 }
 

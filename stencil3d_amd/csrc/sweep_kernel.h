@@ -174,21 +174,25 @@ struct QCell<A, 2> {
 // holds the tile's first k; with two k points per lane CO is even, so every
 // lane's pair sits on an 8-byte boundary and LDS traffic moves as 2-element
 // vectors (no 2-way bank conflicts from lanes 8 bytes apart).
-template <int NPTS, int MJ, int NW, int V>
+template <int NPTS, int MJ, int NW, int V, int F = 1>
 struct TileLds {
   static constexpr int R = Terms<NPTS>::R;
   static constexpr int CO = V == 2 ? 2 * ((R + 1) / 2) : R;
-  static constexpr int LW = 64 * V + 2 * CO;
-  static constexpr int SIZE = (NW * MJ + 2 * R) * LW;
+  static constexpr int LW = 64 / F * V + 2 * CO;
+  static constexpr int SIZE = (NW * MJ * F + 2 * R) * LW;
 };
 
 // Tile engine shared by the single-grid and the batched kernels: a
-// (TJ = NW*MJ) x (TK = 64*V) tile of one grid, planes [ib, ie).  Lane l of
-// wave w holds k = k0 + V*l .. k0 + V*l + V-1 of rows j0 + w*MJ .. + MJ-1.
-template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
+// (TJ = NW*MJ*F) x (TK = 64/F*V) tile of one grid at (j0, k0), planes
+// [ib, ie).  A wave's 64 lanes form F rows of LL = 64/F lanes; lane l of wave
+// w holds k = k0 + V*(l % LL) .. + V-1 of rows j0 + (w*MJ + m)*F + l / LL,
+// m < MJ.  F = 1 is the regular tile (one row per lane group of 64); F > 1
+// folds a narrow tile (the last k-tile of a row whose length is not a
+// multiple of TK) onto whole waves.
+template <int NPTS, int MODE, int FMT, int MJ, int NW, int V, int F = 1>
 __device__ __forceinline__ void sweep_tile(
-    const SweepArgs &a, int tk, int tj, int ib, int ie,
-    typename Codec<FMT>::A (*lds)[TileLds<NPTS, MJ, NW, V>::SIZE],
+    const SweepArgs &a, int k0, int j0, int ib, int ie,
+    typename Codec<FMT>::A (*lds)[TileLds<NPTS, MJ, NW, V, F>::SIZE],
     float *red) {
   using Tm = Terms<NPTS>;
   constexpr int R = Tm::R, T = Tm::T;
@@ -202,17 +206,19 @@ __device__ __forceinline__ void sweep_tile(
   using GPtr = GB *;
   using GV = __attribute__((address_space(1))) VT;
   using GS = __attribute__((address_space(1))) ST;
-  constexpr int TJ = NW * MJ, TK = 64 * V;
-  constexpr int CO = TileLds<NPTS, MJ, NW, V>::CO, LW = TileLds<NPTS, MJ, NW, V>::LW;
-  constexpr int HJ = (2 * R + NW - 1) / NW;  // j-halo rows loaded per wave
-  static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
+  constexpr int LL = 64 / F;  // lanes per row
+  constexpr int TJ = NW * MJ * F, TK = LL * V;
+  constexpr int CO = TileLds<NPTS, MJ, NW, V, F>::CO, LW = TileLds<NPTS, MJ, NW, V, F>::LW;
+  constexpr int HJ = (2 * R + NW * F - 1) / (NW * F);  // j-halo row loads per wave
+  static_assert(MJ * F * 2 * R <= 64, "k-halo needs at most one entry per thread");
   static_assert(V == 1 || V == 2, "1 or 2 k points per lane");
+  static_assert(F == 1 || V == 1, "folded tiles have one k point per lane");
   static_assert(V - 1 <= R, "a lane's vector may reach past the last halo cell by V-1");
 
   const int lane = threadIdx.x & 63;
+  const int lc = lane % LL, lf = lane / LL;  // column lane, row in the fold
   const int w = threadIdx.x >> 6;
   const int tid = threadIdx.x;
-  const int k0 = tk * TK, j0 = tj * TJ;
   const int64_t P = a.plane, Pj = a.pitch;
 
   // LEAPFROG coefficients.  16-bit storage works on the stored numbers q
@@ -242,22 +248,22 @@ __device__ __forceinline__ void sweep_tile(
   // and the field has 256 B of slack after its last row).
   const int jmax = a.ys + R - 1 - j0;  // last readable tile row
   const int kmax = a.zs + R - 1 - k0;  // last readable tile column
-  const int kc = min(lane * V, (kmax / V) * V);
+  const int kc = min(lc * V, (kmax / V) * V);
   uint32_t off[MJ];
   bool valid[MJ];    // the lane's first point is inside the interior
   bool valid2[MJ];   // ... and its last one too
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
-    const int row = w * MJ + m;
+    const int row = (w * MJ + m) * F + lf;
     off[m] = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (kc + R)) * sizeof(ST));
-    valid[m] = (j0 + row < a.ys) && (k0 + lane * V < a.zs);
-    valid2[m] = (j0 + row < a.ys) && (k0 + lane * V + V - 1 < a.zs);
+    valid[m] = (j0 + row < a.ys) && (k0 + lc * V < a.zs);
+    valid2[m] = (j0 + row < a.ys) && (k0 + lc * V + V - 1 < a.zs);
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
 #pragma unroll
   for (int h = 0; h < HJ; h++) {
-    const int hh = w + NW * h;  // halo row id 0..2R-1
+    const int hh = (w + NW * h) * F + lf;  // halo row id 0..2R-1
     const int trow = hh < R ? hh - R : TJ + hh - R;
     hrow[h] = hh < 2 * R ? (hh < R ? hh : TJ + hh) : -1;  // LDS row, -1: none
     hoff[h] = (uint32_t)(((int64_t)(min(trow, jmax) + R) * Pj + (kc + R)) * sizeof(ST));
@@ -362,20 +368,21 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
     for (int m = 0; m < MJ; m++)
 #pragma unroll
-      for (int c = 0; c < V; c++) L[(w * MJ + m + R) * LW + lane * V + c + CO] = q[m][R].get(c);
+      for (int c = 0; c < V; c++)
+        L[((w * MJ + m) * F + lf + R) * LW + lc * V + c + CO] = q[m][R].get(c);
 #pragma unroll
     for (int h = 0; h < HJ; h++)
       if (hrow[h] >= 0)
 #pragma unroll
         for (int c = 0; c < V; c++)
-          L[hrow[h] * LW + lane * V + c + CO] = CD::dec(VV::get(hq[h][0], c));
+          L[hrow[h] * LW + lc * V + c + CO] = CD::dec(VV::get(hq[h][0], c));
     if (has_kh) L[khpos] = CD::dec(kq[0]);
     __syncthreads();
 
     GPtr Vp = uni(Vb + (int64_t)i * P);
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      const int lr = w * MJ + m + R, col = lane * V + CO;
+      const int lr = (w * MJ + m) * F + lf + R, col = lc * V + CO;
       // Two k points per lane: the k- and j-neighbours come from LDS as
       // aligned 2-element vectors, read once for both points.
       // kn[t] = column col - 2 KH + t; jl[r] / jh[r] = rows lr -/+ r.
@@ -428,10 +435,11 @@ __device__ __forceinline__ void sweep_tile(
                                       : L[lr * LW + col + c + r];
             return lo + hi;
           }
-          const AT lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R].get(c)
-                                     : L[(lr - r) * LW + col + c];
-          const AT hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R].get(c)
-                                     : L[(lr + r) * LW + col + c];
+          // rows lr -/+ r are this lane's own rows m -/+ r/F when F divides r
+          const bool lreg = r % F == 0 && m - r / F >= 0;
+          const bool hreg = r % F == 0 && m + r / F < MJ;
+          const AT lo = lreg ? q[lreg ? m - r / F : 0][R].get(c) : L[(lr - r) * LW + col + c];
+          const AT hi = hreg ? q[hreg ? m + r / F : 0][R].get(c) : L[(lr + r) * LW + col + c];
           return lo + hi;
         };
         const AT u0 = q[m][R].get(c);
@@ -525,10 +533,13 @@ __device__ inline void q16_publish(const SweepArgs &a) {
   }
 }
 
-template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
+// EF > 0: the last k-tile of every row is the narrow remainder of the row
+// (at most 64/EF points) and runs folded (EF rows per wave, one k point per
+// lane) instead of as a full, mostly masked tile.
+template <int NPTS, int MODE, int FMT, int MJ, int NW, int V, int EF = 0>
 __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
-  constexpr int R = Terms<NPTS>::R;
-  __shared__ __attribute__((aligned(16))) typename Codec<FMT>::A lds[2][TileLds<NPTS, MJ, NW, V>::SIZE];
+  using AT = typename Codec<FMT>::A;
+  __shared__ __attribute__((aligned(16))) AT lds[2][TileLds<NPTS, MJ, NW, V>::SIZE];
   __shared__ float red[NW];
   q16_publish<FMT>(a);
   int bid = blockIdx.x;
@@ -540,7 +551,18 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
   const int ib = ch * a.chunk_len;
   const int ie = min(a.xs, ib + a.chunk_len);
   if (ib >= ie) return;
-  sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, tk, tj, ib, ie, lds, red);
+  const int k0 = tk * 64 * V, j0 = tj * NW * MJ;
+  if constexpr (EF > 0) {
+    static_assert(MJ % EF == 0, "a folded tile keeps the tile's rows");
+    using E = TileLds<NPTS, MJ / EF, NW, 1, EF>;
+    static_assert(E::SIZE <= TileLds<NPTS, MJ, NW, V>::SIZE, "");
+    if (tk == a.ntk - 1) {
+      sweep_tile<NPTS, MODE, FMT, MJ / EF, NW, 1, EF>(a, k0, j0, ib, ie,
+                                                       (AT(*)[E::SIZE])&lds[0][0], red);
+      return;
+    }
+  }
+  sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, k0, j0, ib, ie, lds, red);
 }
 
 }  // namespace sth

@@ -112,3 +112,31 @@ def test_batch_tunings_agree(gpu, orc, tile, planes):
         newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, None, 3)
         want = a if newest == 0 else b
         assert np.array_equal(got[q][g.interior].view(np.uint32), want[g.interior].view(np.uint32))
+
+
+def test_default_tuning_large_shot_matches_single_grid(gpu, orc):
+    """A shot large enough for the single-grid size-dependent default tile
+    (here 8 x 1040 x 1040) under the batch's default tuning: persistent and
+    sequential batch steps equal a plain single-grid run, bit for bit."""
+    dims = [(8, 1040, 1040), (9, 20, 70)]
+    npts, R = 13, 2
+    c0, c1 = coeffs(orc, npts)
+    want = []
+    for d in dims:
+        with gpu.Stencil(*d, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=1)
+            ctx.init_uniform("u", 9, 0, -1.0, 1.0, 0.0)
+            ctx.init_uniform("v", 9, 1, -1.0, 1.0, 0.0)
+            ctx.init_uniform("s", 9, 2, -1.0, 1.0, 0.0)
+            ctx.leapfrog(3, c0, c1, 1e-3)
+            want.append(ctx.download("u", R))
+    for method in (0, 1):
+        with gpu.Batch(dims, npts=npts) as bt:
+            for q in range(len(dims)):
+                bt.shot(q).init_uniform("u", 9, 0, -1.0, 1.0, 0.0)
+                bt.shot(q).init_uniform("v", 9, 1, -1.0, 1.0, 0.0)
+                bt.shot(q).init_uniform("s", 9, 2, -1.0, 1.0, 0.0)
+            bt.leapfrog(3, c0, c1, np.full(3, 1e-3, np.float32), method=method)
+            for q in range(len(dims)):
+                got = bt.shot(q).download("u", R)
+                assert np.array_equal(got.view(np.uint32), want[q].view(np.uint32)), (method, q)
