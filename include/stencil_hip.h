@@ -96,6 +96,8 @@ typedef struct {
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
   uint32_t steps_per_pass; /* LEAPFROG, FP32: 2 = two steps per pass over memory
                               (temporal blocking), 1 = one; 0: default */
+  uint32_t launch;        /* one-step sweep: 1 = one workgroup per tile, 2 = persistent
+                             workgroups drawing tiles from a work queue; 0: default (2) */
 } sth_tuning;
 
 typedef struct sth_ctx sth_ctx;

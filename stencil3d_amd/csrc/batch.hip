@@ -159,6 +159,7 @@ Plan shot_plan(const sth_batch *b, const sth_ctx *c) {
   t.rows_per_wave = (uint32_t)ts.mj;
   t.waves = (uint32_t)ts.nw;
   t.vec = (uint32_t)ts.vec;
+  t.launch = 1;  // one workgroup per tile (the batch has its own work queue)
   const int xs = (int)c->g.xs;
   const int planes = b->planes ? (int)b->planes : xs;
   t.chunks = (uint32_t)std::min(64, std::max(1, (xs + planes - 1) / planes));

@@ -263,12 +263,38 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   p.chunk_len = (xs + ch - 1) / ch;
   p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
   p.grid = (unsigned)(p.ntk * p.ntj * p.chunks);
+  p.items = p.grid;
+  p.persist = t.launch != 1 && c->tickets_dev != nullptr;
+  if (p.persist) {
+    // persistent: as many workgroups as stay resident (measured at 1040^3:
+    // 10 % faster than one workgroup per tile -- no partly filled last round,
+    // and a workgroup's next tile starts without a launch)
+    int per_cu = 0;
+    const void *fn = sweep_fn((int)c->g.npts, MODE_LEAP, c->fmt, p.mj, p.nw, p.vec, 0);
+    if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * p.nw, 0) !=
+                   hipSuccess || per_cu < 1) {
+      (void)hipGetLastError();
+      per_cu = 1;
+    }
+    p.grid = std::min<unsigned>(p.items, (unsigned)(per_cu * std::max(1, c->cus)));
+    // 16-bit storage: bands of about the tiles one XCD holds at once (its
+    // blocks), rounded to whole rows of k-tiles, dealt round robin to the
+    // XCDs: an XCD's tiles in flight are j-neighbours (their halo rows meet
+    // in its L2) and the eight XCDs work on adjacent bands of one i-chunk.
+    // FP32: a single queue in item order (measured at 1040^3: FP32 4x8x2
+    // 3.26 ms single queue, 3.45 banded; FP16 2x8x2 2.13 banded, 2.17
+    // single).  xcd_remap = 0: a single queue for every storage type.
+    const int per_xcd = std::max(1, (int)p.grid / kXcds);
+    const int rows = std::max(1, (per_xcd + p.ntk / 2) / p.ntk);
+    const bool banded = t.xcd_remap && (c->fmt == FMT_F16 || c->fmt == FMT_BF16);
+    p.band = banded ? rows * p.ntk : (int)p.items;
+  }
   // A last k-tile with at most 64/f columns runs folded (f rows per wave):
   // at 1040 columns and 128-column tiles it holds 16 columns, and unfolded it
   // would cost as much as a full tile.
   p.edge = 0;
   const int we = (int)c->g.zs - (p.ntk - 1) * 64 * p.vec;
-  if (c->fmt != FMT_F64 && we < 64 * p.vec)
+  if (c->fmt == FMT_F32 && we < 64 * p.vec)
     for (int f : {4, 2, 1})
       if (64 / f >= we && p.mj % f == 0 && (f > 1 || p.vec == 2) &&
           edge_supported(p.mj, p.nw, p.vec, f)) {
@@ -301,10 +327,34 @@ void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a) {
   a.c0 = a.c1 = a.c2n = 0.0f;
 }
 
-int launch_one(sth_ctx *c, const Plan &p, int mode, const SweepArgs &a) {
+int launch_one(sth_ctx *c, const Plan &p, int mode, SweepArgs &a) {
+  a.tickets = nullptr;
+  if (p.persist) {
+    // counter x advances by its range size + the blocks on XCD x per launch;
+    // restart the counts well before one could wrap
+    bool wrap = false;
+    for (int x = 0; x < kXcds; x++) wrap = wrap || c->tick_next[x] > 0x7fffffffu;
+    if (wrap) {
+      const hipError_t z = launch_zero_u32(c->tickets_dev, 16 * kXcds, c->stream);
+      if (z != hipSuccess) return hip_status(z);
+      for (int x = 0; x < kXcds; x++) c->tick_next[x] = 0;
+    }
+    a.tickets = c->tickets_dev;
+    for (int x = 0; x < kXcds; x++) a.tick_base[x] = c->tick_next[x];
+    a.items = (int)p.items;
+    a.band = p.band;
+  }
   const hipError_t e =
       launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.edge, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
+  if (p.persist && p.band >= (int)p.items) {
+    c->tick_next[0] += p.items + p.grid;  // one queue
+  } else if (p.persist) {
+    for (int x = 0; x < kXcds; x++) {
+      const int blocks = ((int)p.grid - x + kXcds - 1) / kXcds;
+      c->tick_next[x] += (uint32_t)(xcd_band_items((int)p.items, p.band, x) + std::max(0, blocks));
+    }
+  }
   if (c->timing) c->launches++;
   return STH_OK;
 }
@@ -508,6 +558,7 @@ int sth_destroy(sth_ctx *c) {
   if (c->errors_dev) (void)hipFree(c->errors_dev);
   if (c->partial_dev) (void)hipFree(c->partial_dev);
   if (c->sum_dev) (void)hipFree(c->sum_dev);
+  if (c->tickets_dev) (void)hipFree(c->tickets_dev);
   if (c->staging) (void)hipHostFree(c->staging);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -542,6 +593,10 @@ int sth_create(const sth_geom *g, sth_ctx **out) {
   if (e == hipSuccess) e = hipMalloc((void **)&c->errors_dev, 256);
   if (e == hipSuccess) e = hipMalloc((void **)&c->partial_dev, kReduceBlocks * sizeof(double));
   if (e == hipSuccess) e = hipMalloc((void **)&c->sum_dev, 256);
+  if (e == hipSuccess) e = hipMalloc((void **)&c->tickets_dev, 64 * kXcds);
+  if (e == hipSuccess) e = hipMemsetAsync(c->tickets_dev, 0, 64 * kXcds, c->stream);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, c->device);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e == hipSuccess) e = hipMemsetAsync(c->qs_dev, 0, 3 * sizeof(Q16Slot), c->stream);
@@ -956,7 +1011,7 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 }
 
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
-  if (!c || !t || t->steps_per_pass > 2 || t->chunks > 64) return STH_ERR_ARG;
+  if (!c || !t || t->steps_per_pass > 2 || t->launch > 2 || t->chunks > 64) return STH_ERR_ARG;
   if (t->steps_per_pass == 2 && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
   const bool fused_shape = t->steps_per_pass == 2 &&
                            fused2_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves);
@@ -984,6 +1039,7 @@ int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
   t->vec = (uint32_t)p.vec;
   t->chunks = (uint32_t)p.chunks;
   t->steps_per_pass = 1;
+  t->launch = p.persist ? 2 : 1;
   return STH_OK;
 }
 

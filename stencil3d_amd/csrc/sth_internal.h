@@ -25,6 +25,10 @@ struct sth_ctx {
   void *twin[2] = {nullptr, nullptr};
   bool twins_ok = false;
   bool no_fuse = false;  // shots of a batch: their buffers sit in its tables
+  // persistent launches: the work-queue counter and the next launch's base
+  uint32_t *tickets_dev = nullptr;     // kXcds counters, 64 B apart
+  uint32_t tick_next[sth::kXcds] = {};
+  int cus = 0;  // compute units of the device
   // 16-bit storage scale state
   sth::Q16Slot *qs_dev = nullptr;
   sth::Q16Slot host_slot{};
@@ -80,6 +84,9 @@ inline TileShape default_tile(int fmt, uint32_t ys, uint32_t zs, const sth_tunin
 struct Plan {
   int mj, nw, vec, chunks, chunk_len, ntk, ntj;
   int edge;  // > 0: the last k-tile runs folded, `edge` rows per wave
+  bool persist;   // persistent workgroups drawing tiles from a work queue
+  unsigned items;  // tiles x chunks
+  int band;        // persistent: tiles per band (bands go round robin to the XCDs)
   unsigned grid;
 };
 

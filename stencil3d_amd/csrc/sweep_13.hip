@@ -8,4 +8,8 @@ hipError_t launch_sweep_13(int mode, int fmt, int mj, int nw, int vec, int ef, u
   return launch_sweep_n<13>(mode, fmt, mj, nw, vec, ef, grid, st, a);
 }
 
+const void *sweep_fn_13(int mode, int fmt, int mj, int nw, int vec, int ef) {
+  return sweep_fn_n<13>(mode, fmt, mj, nw, vec, ef);
+}
+
 }  // namespace sth

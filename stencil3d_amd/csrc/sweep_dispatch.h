@@ -12,7 +12,8 @@ namespace sth {
   X(2, 2, 2) X(4, 2, 2)
 
 // Folded last k-tile (rows_per_wave, waves, k points per lane, fold) variants
-// of the default tiles.
+// of the default tiles, FP32 only: the 16-bit tiles gain little from the fold
+// and its second tile body would cost them occupancy.
 #define STH_EDGE_LIST(X) X(4, 8, 2, 1) X(4, 8, 2, 2) X(4, 8, 2, 4) X(2, 8, 2, 1) X(2, 8, 2, 2)
 
 inline bool edge_supported(int mj, int nw, int vec, int ef) {
@@ -32,7 +33,7 @@ inline bool tile_supported(int mj, int nw, int vec) {
 template <int NPTS, int MODE, int FMT>
 hipError_t launch_tile(int mj, int nw, int vec, int ef, unsigned grid, hipStream_t st,
                        const SweepArgs &a) {
-  if constexpr (FMT != FMT_F64) {
+  if constexpr (FMT == FMT_F32) {
 #define STH_EDGE_CASE(MJ, NW, VV, EF)                                                 \
   if (ef == EF && mj == MJ && nw == NW && vec == VV) {                                \
     hipLaunchKernelGGL((sweep_kernel<NPTS, MODE, FMT, MJ, NW, VV, EF>), dim3(grid),   \
@@ -52,6 +53,40 @@ hipError_t launch_tile(int mj, int nw, int vec, int ef, unsigned grid, hipStream
   STH_TILE_LIST(STH_TILE_CASE)
 #undef STH_TILE_CASE
   return hipErrorInvalidValue;
+}
+
+// The kernel instance a launch would use (for occupancy queries).
+template <int NPTS, int MODE, int FMT>
+const void *tile_fn(int mj, int nw, int vec, int ef) {
+  if constexpr (FMT == FMT_F32) {
+#define STH_EDGE_FN(MJ, NW, VV, EF)                        \
+  if (ef == EF && mj == MJ && nw == NW && vec == VV)       \
+    return (const void *)&sweep_kernel<NPTS, MODE, FMT, MJ, NW, VV, EF>;
+    STH_EDGE_LIST(STH_EDGE_FN)
+#undef STH_EDGE_FN
+  }
+  if (ef != 0) return nullptr;
+#define STH_TILE_FN(MJ, NW, VV)                            \
+  if (mj == MJ && nw == NW && vec == VV)                   \
+    return (const void *)&sweep_kernel<NPTS, MODE, FMT, MJ, NW, VV>;
+  STH_TILE_LIST(STH_TILE_FN)
+#undef STH_TILE_FN
+  return nullptr;
+}
+
+template <int NPTS>
+const void *sweep_fn_n(int mode, int fmt, int mj, int nw, int vec, int ef) {
+  if (mode == MODE_REF) {
+    if (fmt == FMT_F32) return tile_fn<NPTS, MODE_REF, FMT_F32>(mj, nw, vec, ef);
+    if (fmt == FMT_F64) return tile_fn<NPTS, MODE_REF, FMT_F64>(mj, nw, vec, ef);
+  }
+  if (mode == MODE_LEAP) {
+    if (fmt == FMT_F32) return tile_fn<NPTS, MODE_LEAP, FMT_F32>(mj, nw, vec, ef);
+    if (fmt == FMT_F64) return tile_fn<NPTS, MODE_LEAP, FMT_F64>(mj, nw, vec, ef);
+    if (fmt == FMT_F16) return tile_fn<NPTS, MODE_LEAP, FMT_F16>(mj, nw, vec, ef);
+    if (fmt == FMT_BF16) return tile_fn<NPTS, MODE_LEAP, FMT_BF16>(mj, nw, vec, ef);
+  }
+  return nullptr;
 }
 
 template <int NPTS>
@@ -77,6 +112,17 @@ hipError_t launch_sweep_13(int mode, int fmt, int mj, int nw, int vec, int ef, u
                            hipStream_t st, const SweepArgs &a);
 hipError_t launch_sweep_25(int mode, int fmt, int mj, int nw, int vec, int ef, unsigned grid,
                            hipStream_t st, const SweepArgs &a);
+
+const void *sweep_fn_7(int mode, int fmt, int mj, int nw, int vec, int ef);
+const void *sweep_fn_13(int mode, int fmt, int mj, int nw, int vec, int ef);
+const void *sweep_fn_25(int mode, int fmt, int mj, int nw, int vec, int ef);
+
+inline const void *sweep_fn(int npts, int mode, int fmt, int mj, int nw, int vec, int ef) {
+  if (npts == 7) return sweep_fn_7(mode, fmt, mj, nw, vec, ef);
+  if (npts == 13) return sweep_fn_13(mode, fmt, mj, nw, vec, ef);
+  if (npts == 25) return sweep_fn_25(mode, fmt, mj, nw, vec, ef);
+  return nullptr;
+}
 
 inline hipError_t launch_sweep(int npts, int mode, int fmt, int mj, int nw, int vec, int ef,
                                unsigned grid, hipStream_t st, const SweepArgs &a) {

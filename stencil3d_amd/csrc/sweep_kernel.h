@@ -54,6 +54,16 @@ struct SweepArgs {
   int slot;             // slot read by this step
   int ubuf, vbuf;       // ring indices (0 = a, 1 = b) of u and v
   double gain;          // sum over terms of 2*|w| (LEAPFROG weights)
+  // Persistent launch (tickets != null): the tiles, in item order, form
+  // bands of `band` tiles; band q belongs to XCD q % kXcds (blocks are
+  // dispatched to XCDs round robin, so block b runs on XCD b % kXcds).  The
+  // blocks of XCD x draw its bands' tiles from counter x (tickets[16 x])
+  // until they draw one past them; a launch consumes exactly (tiles of XCD x
+  // + blocks on XCD x) tickets of counter x, so the launches of a sequence
+  // start at tick_base[x].
+  unsigned *tickets;
+  unsigned tick_base[kXcds];
+  int items, band;
 };
 
 template <int FMT>
@@ -124,6 +134,16 @@ __host__ __device__ inline int q16_out_exp(const Q16Slot &cur, int ubuf, int vbu
   };
   const double gmul = fabs((double)c0) + fabs((double)c1) * gain;
   return scale_exp(gmul * M(ubuf) + M(vbuf) + fabs((double)c2n) * M(2));
+}
+
+// Tiles owned by XCD x when `items` tiles form bands of `band`, band q
+// going to XCD q % kXcds (only the last band may be partial).
+__host__ __device__ inline int xcd_band_items(int items, int band, int x) {
+  const int nb = (items + band - 1) / band;
+  if (x >= nb) return 0;
+  const int owned = (nb - 1 - x) / kXcds + 1;
+  const bool has_last = (nb - 1) % kXcds == x;
+  return has_last ? (owned - 1) * band + (items - (nb - 1) * band) : owned * band;
 }
 
 // Blocks are dispatched round-robin over the 8 XCDs; this bijection gives
@@ -541,28 +561,54 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
   using AT = typename Codec<FMT>::A;
   __shared__ __attribute__((aligned(16))) AT lds[2][TileLds<NPTS, MJ, NW, V>::SIZE];
   __shared__ float red[NW];
+  __shared__ int sh_item;
   q16_publish<FMT>(a);
-  int bid = blockIdx.x;
-  if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
-  const int tk = bid % a.ntk;
-  const int t = bid / a.ntk;
-  const int tj = t % a.ntj;
-  const int ch = t / a.ntj;
-  const int ib = ch * a.chunk_len;
-  const int ie = min(a.xs, ib + a.chunk_len);
-  if (ib >= ie) return;
-  const int k0 = tk * 64 * V, j0 = tj * NW * MJ;
-  if constexpr (EF > 0) {
-    static_assert(MJ % EF == 0, "a folded tile keeps the tile's rows");
-    using E = TileLds<NPTS, MJ / EF, NW, 1, EF>;
-    static_assert(E::SIZE <= TileLds<NPTS, MJ, NW, V>::SIZE, "");
-    if (tk == a.ntk - 1) {
-      sweep_tile<NPTS, MODE, FMT, MJ / EF, NW, 1, EF>(a, k0, j0, ib, ie,
-                                                       (AT(*)[E::SIZE])&lds[0][0], red);
-      return;
+  // One tile: item = tk + ntk * (tj + ntj * chunk).
+  auto tile = [&](int item) __attribute__((always_inline)) {
+    const int tk = item % a.ntk;
+    const int t = item / a.ntk;
+    const int tj = t % a.ntj;
+    const int ch = t / a.ntj;
+    const int ib = ch * a.chunk_len;
+    const int ie = min(a.xs, ib + a.chunk_len);
+    if (ib >= ie) return;
+    const int k0 = tk * 64 * V, j0 = tj * NW * MJ;
+    if constexpr (EF > 0) {
+      static_assert(MJ % EF == 0, "a folded tile keeps the tile's rows");
+      using E = TileLds<NPTS, MJ / EF, NW, 1, EF>;
+      static_assert(E::SIZE <= TileLds<NPTS, MJ, NW, V>::SIZE, "");
+      if (tk == a.ntk - 1) {
+        sweep_tile<NPTS, MODE, FMT, MJ / EF, NW, 1, EF>(a, k0, j0, ib, ie,
+                                                         (AT(*)[E::SIZE])&lds[0][0], red);
+        return;
+      }
     }
+    sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, k0, j0, ib, ie, lds, red);
+  };
+  // Persistent launch: within an XCD's bands the tiles go in item order (k
+  // fastest, then j, then the i-chunk).  One call site of the tile body for
+  // both launch forms keeps the kernel's register budget that of one tile.
+  const bool persistent = a.tickets != nullptr;
+  const bool one = a.band >= a.items;  // a single queue (counter 0)
+  const int x = one ? 0 : (int)(blockIdx.x % kXcds);
+  const int n = persistent ? (one ? a.items : xcd_band_items(a.items, a.band, x)) : 0;
+  unsigned *cnt = persistent ? a.tickets + 16 * x : nullptr;
+  for (;;) {
+    int item;
+    if (persistent) {
+      if (threadIdx.x == 0) sh_item = (int)(atomicAdd(cnt, 1u) - a.tick_base[x]);
+      __syncthreads();
+      const int t = sh_item;
+      __syncthreads();
+      if (t < 0 || t >= n) break;
+      item = one ? t : (x + kXcds * (t / a.band)) * a.band + t % a.band;
+    } else {
+      item = a.remap ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    }
+    tile(item);
+    if (!persistent) break;
+    __syncthreads();  // LDS is reused by the next tile
   }
-  sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, k0, j0, ib, ie, lds, red);
 }
 
 }  // namespace sth

@@ -8,7 +8,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-ZS = [144, 170, 192, 200]
+ZS = [144, 170, 192, 200]  # q16 tiles run unfolded (FP32 only folds)
 TILES = [(4, 8, 2), (2, 8, 2)]
 
 

@@ -230,3 +230,27 @@ def test_fp64_ref_and_leapfrog_bit_exact(gpu, orc, npts):
     newest = orc.leapfrog_f64(npts, g, a, b, s, c0, c1, c2.astype(np.float64), 3)
     want = a if newest == 0 else b
     assert np.array_equal(got[g.interior], want[g.interior])
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp16"])
+@pytest.mark.parametrize("npts", [7, 25])
+def test_launch_modes_agree(gpu, orc, dtype, npts):
+    """One workgroup per tile and the persistent work-queue launch (several
+    calls in a row, so the queue's ticket base advances between launches)
+    give the same bits."""
+    shape = (40, 70, 300)
+    R = orc.radius(npts)
+    c1 = 0.16
+    c0 = 2.0 - orc.lap_gain(npts) * c1
+    out = []
+    for launch in (1, 2):
+        with gpu.Stencil(*shape, npts=npts, dtype=dtype) as ctx:
+            ctx.set_tuning(0, 0, 3, launch=launch, steps_per_pass=1 if dtype == "fp32" else 0)
+            assert ctx.get_tuning()["launch"] == launch
+            ctx.init_gaussian("u", 1.0, 6.0, 1e-3, 4, 0)
+            ctx.init_gaussian("v", 1.0, 6.0, 1e-3, 4, 1)
+            ctx.init_point("s", 20, 35, 150, 1.0)
+            for rep in range(3):
+                ctx.leapfrog(3, c0, c1, 1e-3)
+            out.append(ctx.download("u", R, dtype=np.float64))
+    assert np.array_equal(out[0], out[1])
