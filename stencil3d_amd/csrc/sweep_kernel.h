@@ -170,6 +170,47 @@ struct VecT<ST, 2> {
   __device__ static inline ST get(const type &x, int c) { return c ? x.y : x.x; }
 };
 
+// Buffer loads and stores: a plane's base lives in a scalar buffer resource
+// (range 4 GiB, checked against the 32-bit lane offset), so per access the
+// lane supplies only its 32-bit byte offset -- no 64-bit address arithmetic
+// in vector registers.  AUX = 2 marks the access nontemporal.
+using BufRes = __amdgpu_buffer_rsrc_t;
+template <class Ptr>
+__device__ __forceinline__ BufRes buf_res(Ptr base) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)base, (short)0, (int)0xffffffff,
+                                           0x00020000);
+}
+template <class T, int AUX>
+__device__ __forceinline__ T buf_load(BufRes r, uint32_t off) {
+  using I2 = int __attribute__((ext_vector_type(2)));
+  using I4 = int __attribute__((ext_vector_type(4)));
+  if constexpr (sizeof(T) == 2) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, AUX));
+  } else if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
+  } else if constexpr (sizeof(T) == 8) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX));
+  } else {
+    static_assert(sizeof(T) == 16, "2, 4, 8 or 16 bytes");
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX));
+  }
+}
+template <class T, int AUX>
+__device__ __forceinline__ void buf_store(T x, BufRes r, uint32_t off) {
+  using I2 = int __attribute__((ext_vector_type(2)));
+  using I4 = int __attribute__((ext_vector_type(4)));
+  if constexpr (sizeof(T) == 2) {
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(short, x), r, off, 0, AUX);
+  } else if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, x), r, off, 0, AUX);
+  } else if constexpr (sizeof(T) == 8) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(I2, x), r, off, 0, AUX);
+  } else {
+    static_assert(sizeof(T) == 16, "2, 4, 8 or 16 bytes");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(I4, x), r, off, 0, AUX);
+  }
+}
+
 // One register-queue entry: a lane's V values of one plane.  With V = 2 the
 // pair is one 2-element vector, so shifting the queue moves 64-bit registers.
 template <class A, int V>
@@ -309,15 +350,15 @@ __device__ __forceinline__ void sweep_tile(
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
     return (GPtr)(((uint64_t)hi << 32) | lo);
   };
-  auto vload = [](GPtr p, uint32_t ob) { return *(const GV *)(p + ob); };
-  auto vload_nt = [](GPtr p, uint32_t ob) { return __builtin_nontemporal_load((const GV *)(p + ob)); };
-  auto sload = [](GPtr p, uint32_t ob) { return *(const GS *)(p + ob); };
+  auto vload = [](BufRes p, uint32_t ob) { return buf_load<VT, 0>(p, ob); };
+  auto vload_nt = [](BufRes p, uint32_t ob) { return buf_load<VT, 2>(p, ob); };
+  auto sload = [](BufRes p, uint32_t ob) { return buf_load<ST, 0>(p, ob); };
 
   // Register queue: q[m][t].get(c) = u at plane i - R + t.
   QCell<AT, V> q[MJ][2 * R + 1];
 #pragma unroll
   for (int t = 0; t < 2 * R; t++) {
-    const GPtr Up = uni(Ub + (int64_t)(ib - R + t) * P);
+    const BufRes Up = buf_res(uni(Ub + (int64_t)(ib - R + t) * P));
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       const VT x = vload(Up, off[m]);
@@ -336,7 +377,7 @@ __device__ __forceinline__ void sweep_tile(
   ST kq[R + 1];
 #pragma unroll
   for (int t = 0; t < R; t++) {
-    const GPtr Up = uni(Ub + (int64_t)(ib + t) * P);
+    const BufRes Up = buf_res(uni(Ub + (int64_t)(ib + t) * P));
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][t] = hrow[h] >= 0 ? vload(Up, hoff[h]) : VT(0);
     kq[t] = has_kh ? sload(Up, khoff) : ST(0);
@@ -352,9 +393,9 @@ __device__ __forceinline__ void sweep_tile(
   // s and v are touched once per sweep: nontemporal loads and stores keep
   // them from displacing u's lines, which neighbouring tiles re-read.
   auto prefetch = [&](int i, Staged &n) __attribute__((always_inline)) {
-    const GPtr Up = uni(Ub + (int64_t)(i + R) * P);
-    const GPtr Sp = uni(Sb + (int64_t)i * P);
-    const GPtr Vp = uni(Vb + (int64_t)i * P);
+    const BufRes Up = buf_res(uni(Ub + (int64_t)(i + R) * P));
+    const BufRes Sp = buf_res(uni(Sb + (int64_t)i * P));
+    const BufRes Vp = buf_res(uni(Vb + (int64_t)i * P));
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       // masked lanes read their clamped (in-bounds) cells: no branches
@@ -399,7 +440,7 @@ __device__ __forceinline__ void sweep_tile(
     if (has_kh) L[khpos] = CD::dec(kq[0]);
     __syncthreads();
 
-    GPtr Vp = uni(Vb + (int64_t)i * P);
+    const BufRes Vp = buf_res(uni(Vb + (int64_t)i * P));
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       const int lr = (w * MJ + m) * F + lf + R, col = lc * V + CO;
@@ -488,21 +529,23 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
       for (int c = 0; c < V; c++) qo[c] = CD::enc(res[c]);
       if constexpr (V == 1) {
-        if (valid[m]) __builtin_nontemporal_store(qo[0], (GS *)(Vp + off[m]));
+        if (valid[m]) buf_store<ST, 2>(qo[0], Vp, off[m]);
       } else {
         if (valid2[m]) {
           VT o;
           o.x = qo[0];
           o.y = qo[1];
-          __builtin_nontemporal_store(o, (GV *)(Vp + off[m]));
+          buf_store<VT, 2>(o, Vp, off[m]);
         } else if (valid[m]) {
-          __builtin_nontemporal_store(qo[0], (GS *)(Vp + off[m]));
+          buf_store<ST, 2>(qo[0], Vp, off[m]);
         }
       }
       if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
-        if (valid[m]) mx = fmaxf(mx, fabsf(CD::dec(qo[0])));
-        if constexpr (V == 2)
-          if (valid2[m]) mx = fmaxf(mx, fabsf(CD::dec(qo[1])));
+        // max |stored value| = round(max |FP32 result|): rounding to the
+        // 16-bit format is odd and monotone, so it commutes with max |.|;
+        // the rounding is applied once, after the reduction
+        mx = fmaxf(mx, valid[m] ? fabsf((float)res[0]) : 0.0f);
+        if constexpr (V == 2) mx = fmaxf(mx, valid2[m] ? fabsf((float)res[1]) : 0.0f);
       }
     }
 #pragma unroll
@@ -522,6 +565,7 @@ __device__ __forceinline__ void sweep_tile(
   }
 
   if constexpr (FMT == FMT_F16 || FMT == FMT_BF16) {
+    mx = CD::dec(CD::enc(mx));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
     if (lane == 0) red[w] = mx;
