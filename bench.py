@@ -262,13 +262,15 @@ def measured_traffic(n, dtype_code, tune):
     if t.get("grid") != [n, n, n]:
         return None, None
     if tune.get("steps_per_pass") == 2:
-        name = (f"void sth::sweep2_kernel<{NPTS}, {tune['rows_per_wave']}, "
-                f"{tune['waves']}>(sth::Sweep2Args)")
+        prefix = (f"void sth::sweep2_kernel<{NPTS}, {tune['rows_per_wave']}, "
+                  f"{tune['waves']}>")
     else:
-        name = (f"void sth::sweep_kernel<{NPTS}, 1, {dtype_code}, {tune['rows_per_wave']}, "
-                f"{tune['waves']}, {tune['vec']}>(sth::SweepArgs)")
-    k = t["kernels"].get(name)
-    return (k["traffic_bytes"], t["source"]) if k else (None, None)
+        # the last template argument (fold of the last k-tile) follows the grid
+        prefix = (f"void sth::sweep_kernel<{NPTS}, 1, {dtype_code}, {tune['rows_per_wave']}, "
+                  f"{tune['waves']}, {tune['vec']}")
+    hits = [v for kname, v in t["kernels"].items()
+            if kname.startswith(prefix) and kname[len(prefix)] in ",>"]
+    return (hits[0]["traffic_bytes"], t["source"]) if len(hits) == 1 else (None, None)
 
 
 def main():
