@@ -113,6 +113,7 @@ int main(int argc, char **argv) {
   unsigned xb = 1, yb = 1, zb = 16;
   double sbm = 12.0;
   unsigned npts = 7, seed = 0, tune = 0, rows = 0, waves = 0, chunks = 0, vec = 0;
+  unsigned spp = 0, launch = 0;
   int mode = 0, dtype = 0, init = -1;
   double c1 = 0.25, c2 = 1e-6, c0 = -1e30;
   static const char *const kModes[] = {"ref", "leapfrog"};
@@ -149,9 +150,11 @@ int main(int argc, char **argv) {
     args += get_real_arg(argv, iarg, argc, "--c0", c0, -1e6, 1e6);
     args += get_flag(argv, iarg, "--tune", tune);
     args += get_int_arg(argv, iarg, argc, "--rows", rows, 1, 8, 1);
-    args += get_int_arg(argv, iarg, argc, "--waves", waves, 1, 8, 1);
+    args += get_int_arg(argv, iarg, argc, "--waves", waves, 1, 16, 1);
     args += get_int_arg(argv, iarg, argc, "--chunks", chunks, 1, 64, 1);
     args += get_int_arg(argv, iarg, argc, "--vec", vec, 1, 2, 1);
+    args += get_int_arg(argv, iarg, argc, "--steps-per-pass", spp, 1, 2, 1);
+    args += get_int_arg(argv, iarg, argc, "--launch", launch, 1, 2, 1);
     if (args > 0) {
       iarg += args;
     } else {
@@ -208,11 +211,11 @@ int main(int argc, char **argv) {
          ys, zs, R, R, R, 1e-9 * (double)L.bytes);
   printf(" -- device layout: row pitch %llu, plane %llu elements, %s storage, 256-B aligned rows\n",
          (unsigned long long)L.pitch, (unsigned long long)L.plane, kTypes[dtype]);
-  printf(" -- scheduling: 2.5D sweep along x, one workgroup per (y,z) tile, HIP on gfx950\n");
+  printf(" -- scheduling: 2.5D sweep along x over (y,z) tiles, HIP on gfx950\n");
 
   sth_ctx *ctx = nullptr;
   if ((st = sth_create(&g, &ctx))) die("sth_create", st);
-  sth_tuning tn = {rows, waves, chunks, 1, vec};
+  sth_tuning tn = {rows, waves, chunks, 1, vec, spp, launch};
   if ((st = sth_set_tuning(ctx, &tn))) die("tuning", st);
 
   printf("[rank0] Phase #%i: array initialization\n", ++phase);

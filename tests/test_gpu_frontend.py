@@ -37,6 +37,21 @@ def test_frontend_leapfrog_fp16(gpu):
     assert m and 0.0 < float(m.group(1)) < 10.0
 
 
+def test_frontend_leapfrog_steps_per_pass_and_launch(gpu):
+    """FP32 LEAPFROG from the CLI with one or two steps per pass and either
+    launch form: the printed norms agree to the last digit."""
+    outs = []
+    for extra in (["--steps-per-pass", "1", "--launch", "1"], ["--steps-per-pass", "2"],
+                  ["--steps-per-pass", "1", "--launch", "2"]):
+        r = run_fe(gpu, "--xs", "40", "--ys", "56", "--zs", "150", "--iter", "6", "--stencil",
+                   "13", "--mode", "leapfrog", "--init", "gauss", "--c1", "0.2", *extra)
+        assert r.returncode == 0, r.stderr
+        m = re.search(r"norms: max \|u\| = ([0-9.e+-]+).*", r.stdout)
+        assert m
+        outs.append(m.group(0))
+    assert outs[0] == outs[1] == outs[2]
+
+
 def test_frontend_autotune(gpu):
     r = run_fe(gpu, "--xs", "40", "--ys", "40", "--zs", "64", "--iter", "3", "--stencil", "13",
                "--tune")
