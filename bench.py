@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=1040)
-    ap.add_argument("--cpu-sweeps", type=int, default=8)
+    ap.add_argument("--cpu-sweeps", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--no-16bit", action="store_true")
