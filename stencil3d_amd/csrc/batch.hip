@@ -468,8 +468,8 @@ int sth_batch_timer_read(sth_batch *b, double *ms, uint64_t *launches) {
 int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *best_planes,
                        float *best_ms) {
   if (!b || reps == 0) return STH_ERR_ARG;
-  static const int tiles[][3] = {{1, 4, 1}, {2, 4, 1}, {4, 4, 1}, {2, 8, 1},
-                                 {4, 8, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2}};
+  static const int tiles[][3] = {{1, 4, 1}, {2, 4, 1}, {4, 4, 1}, {2, 8, 1}, {4, 8, 1},
+                                 {1, 4, 2}, {2, 4, 2}, {4, 4, 2}, {2, 8, 2}, {4, 8, 2}};
   static const uint32_t plane_opts[] = {0, 128, 64, 32};
   const sth_tuning saved = b->tune;
   const uint32_t saved_planes = b->planes, saved_grid = b->grid_blocks;
