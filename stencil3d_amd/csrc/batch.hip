@@ -269,7 +269,13 @@ int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float 
   std::vector<Plan> plans(b->shots.size());
   std::vector<SweepArgs> args(b->shots.size());
   for (size_t s = 0; s < b->shots.size(); s++) {
-    plans[s] = shot_plan(b, b->shots[s]);
+    // one plain launch per shot: the shot's own tuning (the single-grid
+    // default unless set on the shot), one workgroup per tile (a persistent
+    // launch carries its queue base on the host, which a replayed graph
+    // would not advance)
+    sth_tuning t = b->shots[s]->tune;
+    t.launch = 1;
+    plans[s] = make_plan(b->shots[s], t);
     memset(&args[s], 0, sizeof(SweepArgs));
     fill_weights(b->shots[s], true, args[s]);
   }
