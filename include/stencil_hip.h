@@ -55,7 +55,8 @@ typedef enum { STH_FIELD_V = 0, STH_FIELD_S = 1, STH_FIELD_U = 2 } sth_field;
 
 /* Geometry of a run (s_grid, grid.h:7-71).  The device keeps a halo of
  * exactly the stencil radius; the caller's halo widths describe its own host
- * buffers (sth_host_layout).  zs, ys, xs: 1..65535. */
+ * buffers (sth_host_layout).  zs, ys, xs: 1..65535, and one device plane
+ * ((ys+2R) rows of the padded pitch) below 2 GiB. */
 typedef struct {
   uint32_t xs, ys, zs; /* interior extents (--xs --ys --zs) */
   uint32_t npts;       /* 7, 13 or 25 (STENCIL_SIZE, config.h:37-39) */

@@ -413,6 +413,11 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   }
   ch = std::max(1, std::min(ch, xs));
   p.chunk_len = (xs + ch - 1) / ch;
+  // A chunk of n planes takes n + 2R + 1 steps and the kernel runs them in
+  // whole groups of 2R + 2: n = 1 mod 2R+2 leaves no idle step (except in
+  // a shorter last chunk).
+  const int ns = 2 * c->R + 2;
+  p.chunk_len = std::min(xs, p.chunk_len + ((1 - p.chunk_len) % ns + ns) % ns);
   p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
   p.grid = (unsigned)(tiles * p.chunks);
   return p;

@@ -35,6 +35,9 @@ inline int compute_layout(const sth_geom &g, sth_layout_info &L) {
   L.elem_bytes = (uint32_t)eb;
   L.pitch = ((uint64_t)g.zs + 2 * R + align - 1) / align * align;
   L.plane = L.pitch * ((uint64_t)g.ys + 2 * R);
+  // The kernels address a plane with 32-bit lane byte offsets (and use
+  // offsets from 2^31 up as "no load"): a plane must stay below 2 GiB.
+  if (L.plane * (uint64_t)eb >= (1ull << 31)) return STH_ERR_ARG;
   L.origin = align + (uint64_t)R * L.pitch + (uint64_t)R * L.plane;
   L.elems = L.plane * ((uint64_t)g.xs + 2 * R) + align;
   L.bytes = (L.elems * eb + 255) / 256 * 256;

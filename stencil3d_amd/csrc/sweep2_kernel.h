@@ -12,9 +12,11 @@
 //    neighbouring tiles too (overlapped tiling).
 //  * Both steps stream along i: step one at plane p (a register queue of u
 //    planes p-R-1 .. p+R, the centre plane plus an R halo in LDS, double
-//    buffered, as in the one-step sweep), step two at plane p-R-1, whose
-//    u_{n+1} planes p-2R-1 .. p-1 are held in an LDS ring of 2R+2 planes.
-//    One barrier per plane covers both.
+//    buffered, as in the one-step sweep), step two at plane i = p-R-1.  Each
+//    lane keeps its own u_{n+1} values of planes i-R .. i+R in a register
+//    queue (the i neighbours of step two, and its j neighbours inside the
+//    wave's rows); plane i of u_{n+1} also goes to LDS, double buffered, for
+//    the other j and k neighbours.  One barrier per plane covers both steps.
 //  * Both steps evaluate the one-step sweep's expression in its order, so a
 //    pass is bit-identical to two one-step sweeps.  Outside the interior
 //    u_{n+1} takes the value the v field holds there: the one-step scheme
@@ -23,6 +25,9 @@
 //    u_{n+2} to one whose halo equals u's (the host keeps a halo-identical
 //    twin of each of the two field buffers and swaps them after a pass).
 #pragma once
+
+#include <type_traits>
+#include <utility>
 
 #include "sweep_kernel.h"
 
@@ -51,25 +56,37 @@ struct Fused2Shape {
   // more tiles of the (compute-bound) 25-point pass, so it keeps 56.
   static constexpr int OJ = EJ - 2 * R;
   static constexpr int OK = R <= 2 ? ((EK - 2 * R) / 16) * 16 : EK - 2 * R;
-  static constexpr int LW = EK + 2 * R;                   // u plane + R halo
+  static constexpr int LW = EK + 2 * R;  // u plane + R halo
   static constexpr int LU = (EJ + 2 * R) * LW;
-  static constexpr int NS = 2 * R + 2;  // ring planes of u_{n+1}
-  static constexpr int LR = EJ * EK;
-  static constexpr int LDS_BYTES = (2 * LU + NS * LR) * 4;
+  static constexpr int LR = EJ * EK;  // one plane of u_{n+1} (or s) on E1
+  static constexpr int LDS_BYTES = (2 * LU + 2 * LR + (R + 1) * LR) * 4;
 };
+
+// Calls f(integral_constant<int, 0>) .. f(integral_constant<int, N-1>).
+template <class Fn, int... K>
+__device__ __forceinline__ void static_for_seq(Fn &&f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 template <int NPTS, int MJ, int NW>
 __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   using Tm = Terms<NPTS>;
   using F = Fused2Shape<NPTS, MJ, NW>;
   constexpr int R = Tm::R, T = Tm::T;
-  constexpr int EJ = F::EJ, LW = F::LW, NS = F::NS, OJ = F::OJ, OK = F::OK;
+  constexpr int EJ = F::EJ, LW = F::LW, OJ = F::OJ, OK = F::OK;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // u halo rows per wave
   constexpr int Q = 2 * R + 2;               // u queue: planes p-R-1 .. p+R
+  constexpr int U = Q;                       // steps per loop iteration
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
+  static_assert(EJ * 2 * R % 64 == 0, "k-halo entries fill whole waves");
   static_assert(F::LDS_BYTES <= 160 * 1024, "LDS");
-  __shared__ float LU[2][F::LU];
-  __shared__ float RING[NS][F::LR];
+  // One LDS array: the two u planes (LU), the two u_{n+1} planes (W), then
+  // R+1 planes of s, each lane's own values (step two needs s(p-R-1)).
+  __shared__ float SM[2 * F::LU + (R + 3) * F::LR];
 
   int bid = blockIdx.x;
   if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
@@ -82,7 +99,9 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   if (ib >= ie) return;
 
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  // The wave index as a scalar: every condition on rows is wave-uniform and
+  // compiles to a branch rather than to exec-mask updates.
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tid = threadIdx.x;
   const int je = tj * OJ - R, ke = tk * OK - R;  // E1 origin (global j, k)
   const int64_t P = a.plane, Pj = a.pitch;
@@ -97,7 +116,6 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
     return (uint32_t)(((int64_t)jc * Pj + kc) * 4);
   };
   using GB = __attribute__((address_space(1))) char;
-  using GF = __attribute__((address_space(1))) float;
   auto uni = [](const void *p) {
     const uint64_t x = (uint64_t)p;
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
@@ -108,17 +126,29 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
     const int ic = min(max(i, -R), a.xs + R - 1);
     return uni((const char *)f + ((int64_t)ic * P - R * Pj - R) * 4);
   };
-  auto ld = [](GB *p, uint32_t ob) { return *(const GF *)(p + ob); };
+  // Loads and stores go through a buffer resource per plane (a scalar base,
+  // 32-bit lane offsets): no 64-bit address arithmetic per access.  The
+  // load range is one plane (under 2 GiB), so lanes that have nothing to
+  // load pass kNone and get 0 without touching memory.
+  const uint32_t plane_bytes = (uint32_t)(P * 4);
+  constexpr uint32_t kNone = 0x80000000u;
+  auto ld = [&](GB *p, uint32_t ob) { return buf_load<float, 0>(buf_res_n(p, plane_bytes), ob); };
 
+  // inner[m]: E1 point inside the interior.  A point is an output when its
+  // row is (rowout, uniform) and its column is (kout, per lane).
   const int kg = ke + lane;
+  const bool kin = kg >= 0 && kg < a.zs;
+  const bool kout = kin && lane >= R && lane < R + OK;
   uint32_t off[MJ];
-  bool inner[MJ], outp[MJ];
+  bool inner[MJ];
+  int rowout[MJ];
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int er = wv * MJ + m, jg = je + er;
+    const bool jin = jg >= 0 && jg < a.ys;
     off[m] = boff(jg, kg);
-    inner[m] = jg >= 0 && jg < a.ys && kg >= 0 && kg < a.zs;
-    outp[m] = inner[m] && er >= R && er < R + OJ && lane >= R && lane < R + OK;
+    inner[m] = jin && kin;
+    rowout[m] = __builtin_amdgcn_readfirstlane((int)(jin && er >= R && er < R + OJ));
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
@@ -127,10 +157,10 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
     const int hh = wv + NW * h;  // halo row id 0..2R-1
     const int trow = hh < R ? hh - R : EJ + hh - R;
     hrow[h] = hh < 2 * R ? (hh < R ? hh : EJ + hh) : -1;
-    hoff[h] = boff(je + trow, kg);
+    hoff[h] = hrow[h] >= 0 ? boff(je + trow, kg) : kNone;
   }
-  const bool has_kh = tid < EJ * 2 * R;
-  uint32_t khoff = 0;
+  const bool has_kh = wv < EJ * 2 * R / 64;  // the first EJ*2R threads
+  uint32_t khoff = kNone;
   int khpos = 0;
   if (has_kh) {
     const int row = tid / (2 * R), c = tid % (2 * R);
@@ -139,8 +169,36 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
     khpos = (row + R) * LW + (c < R ? c : 64 + c);
   }
 
-  // u queue q[m][t] = u(p - R - 1 + t); s queue sq[m][t] = s(p - R - 1 + t).
-  float q[MJ][Q], sq[MJ][R + 2];
+  // LDS addressing.  Every LDS access of a step is a per-lane base plus a
+  // compile-time offset below 64 KiB.  The bases pass through an empty asm
+  // statement at each step, so the compiler cannot hoist one register per
+  // distinct address out of the loop.
+  //   lbL: LU[0] at row wv*MJ, column lane (offsets: buffer, row m+R+dr,
+  //        column R+dc, all >= 0);
+  //   lbW: W[0] at E1 row wv*MJ-R, column lane-R (offsets: buffer, row
+  //        m+R+dr, column R+dc);
+  //   lbS: s slot 0 at E1 row wv*MJ, column lane (offsets: slot, row m).
+  using LF = __attribute__((address_space(3))) float;
+  constexpr uint32_t LUB = F::LU * 4, LRB = F::LR * 4;
+  static_assert(2 * LUB <= 65536 && 2 * LRB + (2 * R + MJ) * 256 <= 65536 &&
+                    R * LRB + MJ * 256 <= 65536,
+                "offsets");
+  const uint32_t lu0 = (uint32_t)(uintptr_t)(LF *)&SM[0];
+  const uint32_t wbase = lu0 + 2 * LUB, sbase = wbase + 2 * LRB;
+  uint32_t lbL = lu0 + (uint32_t)((wv * MJ * LW + lane) * 4);
+  uint32_t lbW = wbase + (uint32_t)(((wv * MJ - R) * 64 + lane - R) * 4);
+  uint32_t lbS = sbase + (uint32_t)((wv * MJ * 64 + lane) * 4);
+  uint32_t hadr[HJ];
+#pragma unroll
+  for (int h = 0; h < HJ; h++) hadr[h] = lu0 + (uint32_t)((max(hrow[h], 0) * LW + lane + R) * 4);
+  const uint32_t kadr = lu0 + (uint32_t)(khpos * 4);
+  auto lds = [](uint32_t adr) -> LF & { return *(LF *)(uintptr_t)adr; };
+
+  // Register queues of the lane's points:
+  //   q[m][t]  = u_n(p - R - 1 + t),      t = 0 .. 2R+1;
+  //   wq[m][t] = u_{n+1}(p - 2R - 1 + t), t = 0 .. 2R (t = 2R+1 takes step
+  //              one's result at plane p before the shift).
+  float q[MJ][Q], wq[MJ][2 * R + 2];
   float hq[HJ][R + 1], kq[R + 1];
   const int p0 = ib - R, pe = ie + R;  // step one at p0 .. pe-1, step two at p0+R+1 .. pe
 #pragma unroll
@@ -153,27 +211,26 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   for (int m = 0; m < MJ; m++) {
     q[m][0] = 0.0f;
 #pragma unroll
-    for (int t = 0; t <= R; t++) sq[m][t] = 0.0f;
+    for (int t = 0; t < 2 * R + 2; t++) wq[m][t] = 0.0f;
   }
 #pragma unroll
   for (int t = 0; t < R; t++) {
     GB *Up = pbase(a.u, p0 + t);
 #pragma unroll
-    for (int h = 0; h < HJ; h++) hq[h][t] = hrow[h] >= 0 ? ld(Up, hoff[h]) : 0.0f;
-    kq[t] = has_kh ? ld(Up, khoff) : 0.0f;
+    for (int h = 0; h < HJ; h++) hq[h][t] = ld(Up, hoff[h]);
+    kq[t] = ld(Up, khoff);
   }
 
-  // Staged loads of iteration p: u(p+R) with its halo, v(p), s(p); two
-  // sets used in turn, filled two iterations ahead.
+  // Staged loads of step p: u(p+R) with its halo, v(p), s(p); two sets used
+  // in turn, filled two steps ahead.  The loads are unconditional (waves
+  // without a halo entry load at kNone), so no staged value has to be
+  // carried over a skipped load.
   struct Staged {
     float u[MJ], v[MJ], s[MJ], hj[HJ];
     float hk;
   };
   Staged SA, SB;
-  auto prefetch = [&](int p, Staged &n) __attribute__((always_inline)) {
-    GB *Up = pbase(a.u, p + R);
-    GB *Vp = pbase(a.v, p);
-    GB *Sp = pbase(a.s, p);
+  auto prefetch = [&](GB *Up, GB *Vp, GB *Sp, Staged &n) __attribute__((always_inline)) {
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       n.u[m] = ld(Up, off[m]);
@@ -181,88 +238,127 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
       n.s[m] = ld(Sp, off[m]);
     }
 #pragma unroll
-    for (int h = 0; h < HJ; h++) n.hj[h] = hrow[h] >= 0 ? ld(Up, hoff[h]) : 0.0f;
-    n.hk = has_kh ? ld(Up, khoff) : 0.0f;
+    for (int h = 0; h < HJ; h++) n.hj[h] = ld(Up, hoff[h]);
+    n.hk = ld(Up, khoff);
   };
-  prefetch(p0, SA);
-  prefetch(p0 + 1, SB);
+  prefetch(pbase(a.u, p0 + R), pbase(a.v, p0), pbase(a.s, p0), SA);
+  prefetch(pbase(a.u, p0 + 1 + R), pbase(a.v, p0 + 1), pbase(a.s, p0 + 1), SB);
 
-  auto slot = [](int x) { return (x + 4 * NS) % NS; };
+  // Plane pointers advanced by one plane per step, each held inside
+  // [-R, xs+R): the prefetch of step p reads u(p+2+R), v(p+2), s(p+2); step
+  // two stores plane p-R-1, from plane ib on.
+  const int64_t PB = P * 4;
+  const int last = a.xs + R - 1;
+  GB *Un = pbase(a.u, p0 + 2 + R);
+  GB *Vn = pbase(a.v, p0 + 2);
+  GB *Sn = pbase(a.s, p0 + 2);
+  GB *Wo = pbase(a.o1, ib);
+  GB *Zo = pbase(a.o2, ib);
 
-  auto step = [&](int p, Staged &n) __attribute__((always_inline)) {
-    float vv[MJ];
+  // Step of plane p = p0 + U*it + kc: the LDS buffers and the staging set
+  // alternate with kc and every queue position is a compile-time constant.
+  // The loop runs whole iterations of U steps; steps past pe compute values
+  // nobody stores (their loads stay on the last planes), so no register
+  // state depends on a skipped step.
+  auto step = [&](int p, Staged &n, auto kc) __attribute__((always_inline)) {
+    constexpr int k = decltype(kc)::value;
+    float vv[MJ], s1[MJ], s2c[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       q[m][Q - 1] = n.u[m];
-      sq[m][R + 1] = n.s[m];
+      s1[m] = n.s[m];
       vv[m] = n.v[m];
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][R] = n.hj[h];
     kq[R] = n.hk;
-    if (p + 2 <= pe) prefetch(p + 2, n);
+    prefetch(Un, Vn, Sn, n);
+    Un = p + 3 + R <= last ? Un + PB : Un;
+    Vn = p + 3 <= last ? Vn + PB : Vn;
+    Sn = p + 3 <= last ? Sn + PB : Sn;
 
-    // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns.
-    float *L = LU[p & 1];
+    asm volatile("" : "+v"(lbL), "+v"(lbW), "+v"(lbS));
+    // s(p-R-1) and s(p) share slot k mod (R+1): read the old value, then
+    // store the new one (the lane's own word, so no barrier is involved).
+    constexpr uint32_t SO = (k % (R + 1)) * LRB;
 #pragma unroll
-    for (int m = 0; m < MJ; m++) L[(wv * MJ + m + R) * LW + lane + R] = q[m][R + 1];
+    for (int m = 0; m < MJ; m++) {
+      s2c[m] = lds(lbS + SO + m * 256);
+      lds(lbS + SO + m * 256) = s1[m];
+    }
+    constexpr uint32_t LO = (k & 1) * LUB;  // this step's u buffer
+    constexpr uint32_t WO = (k & 1) * LRB;  // this step's u_{n+1} buffer
+    // LU element (row m+R+dr, column R+dc of this lane's rows)
+    auto Lat = [&](int m, int dr, int dc) -> LF & {
+      return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + R + dc) * 4));
+    };
+    // W element (E1 row of row m shifted by dr, column lane+dc)
+    auto Wat = [&](int m, int dr, int dc) -> LF & {
+      return lds(lbW + WO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
+    };
+
+    // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns;
+    // u_{n+1}(p-R-1) of the E1 rows.
+#pragma unroll
+    for (int m = 0; m < MJ; m++) {
+      Lat(m, 0, 0) = q[m][R + 1];
+      Wat(m, 0, 0) = wq[m][R];
+    }
 #pragma unroll
     for (int h = 0; h < HJ; h++)
-      if (hrow[h] >= 0) L[hrow[h] * LW + lane + R] = hq[h][0];
-    if (has_kh) L[khpos] = kq[0];
+      if (hrow[h] >= 0) lds(hadr[h] + LO) = hq[h][0];
+    if (has_kh) lds(kadr + LO) = kq[0];
     __syncthreads();
 
-    // Step one at plane p: u_{n+1} on E1.
-    if (p < pe) {
-      float *W = RING[slot(p)];
-      const bool pin = p >= 0 && p < a.xs;
+    // Step two at plane i = p-R-1: u_{n+2} on the output tile.  Rows
+    // without output points (a wave-uniform condition) are skipped.
+    const int i = p - R - 1;
+    const bool store = i >= ib && i < ie;
+    const BufRes Wr = buf_res(Wo), Zr = buf_res(Zo);
 #pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        const int lr = wv * MJ + m + R;
-        auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return q[m][R + 1 - r] + q[m][R + 1 + r];
-          if (ax == 2) return L[lr * LW + lane + R - r] + L[lr * LW + lane + R + r];
-          const float lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R + 1]
-                                        : L[(lr - r) * LW + lane + R];
-          const float hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R + 1]
-                                        : L[(lr + r) * LW + lane + R];
-          return lo + hi;
-        };
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+    for (int m = 0; m < MJ; m++) {
+      if (!rowout[m]) continue;
+      auto pair = [&](int ax, int r) -> float {
+        if (ax == 0) return wq[m][R - r] + wq[m][R + r];
+        if (ax == 2) return Wat(m, 0, -r) + Wat(m, 0, r);
+        const float lo = (m - r >= 0) ? wq[(m - r) >= 0 ? m - r : 0][R] : Wat(m, -r, 0);
+        const float hi = (m + r < MJ) ? wq[(m + r) < MJ ? m + r : 0][R] : Wat(m, r, 0);
+        return lo + hi;
+      };
+      float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
-        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float u0 = q[m][R + 1];
-        const float w1 =
-            __builtin_fmaf(a.c2a, sq[m][R + 1], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, u0, -vv[m])));
-        W[(wv * MJ + m) * 64 + lane] = (inner[m] && pin) ? w1 : vv[m];
+      for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+      const float wc = wq[m][R];
+      const float z =
+          __builtin_fmaf(a.c2b, s2c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
+      if (store && kout) {
+        buf_store<float, 2>(wc, Wr, off[m]);
+        buf_store<float, 2>(z, Zr, off[m]);
       }
     }
+    if (store) {
+      Wo += PB;
+      Zo += PB;
+    }
 
-    // Step two at plane i = p-R-1: u_{n+2} on the output tile; the ring
-    // holds u_{n+1} of planes i-R .. i+R.
-    const int i = p - R - 1;
-    if (i >= ib) {
-      const float *W0 = RING[slot(i)];
-      GB *Wp = pbase(a.o1, i);
-      GB *Zp = pbase(a.o2, i);
+    // Step one at plane p: u_{n+1} on E1, into the newest queue entry.
+    const bool pin = p >= 0 && p < a.xs;
 #pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        if (!outp[m]) continue;
-        const int idx = (wv * MJ + m) * 64 + lane;
-        auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return RING[slot(i - r)][idx] + RING[slot(i + r)][idx];
-          if (ax == 1) return W0[idx - r * 64] + W0[idx + r * 64];
-          return W0[idx - r] + W0[idx + r];
-        };
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+    for (int m = 0; m < MJ; m++) {
+      auto pair = [&](int ax, int r) -> float {
+        if (ax == 0) return q[m][R + 1 - r] + q[m][R + 1 + r];
+        if (ax == 2) return Lat(m, 0, -r) + Lat(m, 0, r);
+        const float lo = (m - r >= 0) ? q[(m - r) >= 0 ? m - r : 0][R + 1] : Lat(m, -r, 0);
+        const float hi = (m + r < MJ) ? q[(m + r) < MJ ? m + r : 0][R + 1] : Lat(m, r, 0);
+        return lo + hi;
+      };
+      float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
-        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float w0 = W0[idx];
-        const float z =
-            __builtin_fmaf(a.c2b, sq[m][0], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, w0, -q[m][0])));
-        __builtin_nontemporal_store(w0, (GF *)(Wp + off[m]));
-        __builtin_nontemporal_store(z, (GF *)(Zp + off[m]));
-      }
+      for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+      const float u0 = q[m][R + 1];
+      const float w1 =
+          __builtin_fmaf(a.c2a, s1[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, u0, -vv[m])));
+      wq[m][2 * R + 1] = (inner[m] && pin) ? w1 : vv[m];
     }
 
 #pragma unroll
@@ -270,7 +366,7 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
 #pragma unroll
       for (int t = 0; t < Q - 1; t++) q[m][t] = q[m][t + 1];
 #pragma unroll
-      for (int t = 0; t < R + 1; t++) sq[m][t] = sq[m][t + 1];
+      for (int t = 0; t < 2 * R + 1; t++) wq[m][t] = wq[m][t + 1];
     }
 #pragma unroll
     for (int t = 0; t < R; t++) {
@@ -279,9 +375,11 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
       kq[t] = kq[t + 1];
     }
   };
-  for (int p = p0; p <= pe; p += 2) {
-    step(p, SA);
-    if (p + 1 <= pe) step(p + 1, SB);
+  for (int p = p0; p <= pe; p += U) {
+    static_for<U>([&](auto kc) __attribute__((always_inline)) {
+      constexpr int k = decltype(kc)::value;
+      step(p + k, (k & 1) ? SB : SA, kc);
+    });
   }
 }
 

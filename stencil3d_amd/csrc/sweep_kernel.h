@@ -180,6 +180,13 @@ __device__ __forceinline__ BufRes buf_res(Ptr base) {
   return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)base, (short)0, (int)0xffffffff,
                                            0x00020000);
 }
+// A resource whose range is `bytes`: a load at an offset at or past it
+// returns 0 without a memory access.
+template <class Ptr>
+__device__ __forceinline__ BufRes buf_res_n(Ptr base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void *)(uintptr_t)base, (short)0, (int)bytes,
+                                           0x00020000);
+}
 template <class T, int AUX>
 __device__ __forceinline__ T buf_load(BufRes r, uint32_t off) {
   using I2 = int __attribute__((ext_vector_type(2)));

@@ -77,9 +77,17 @@ def test_layout_rejects_bad_geometry(sth):
     L = sth.lib()
     out = sth.LayoutInfo()
     for g in [sth.Geom(0, 4, 4, 7, 0), sth.Geom(4, 4, 4, 9, 0), sth.Geom(4, 4, 4, 7, 5),
-              sth.Geom(70000, 4, 4, 7, 0)]:
+              sth.Geom(70000, 4, 4, 7, 0), sth.Geom(4, 23200, 23200, 7, 0)]:
         assert L.sth_layout(C.byref(g), C.byref(out)) == sth.STH_ERR_ARG
     assert L.sth_layout(None, C.byref(out)) == sth.STH_ERR_ARG
+
+
+def test_layout_plane_limit(sth):
+    """A device plane must stay below 2 GiB (32-bit lane offsets)."""
+    L = sth.layout(4, 23000, 23100, 7, "fp32")  # 23002 rows x 23104 (pitch) x 4 B < 2^31
+    assert L.plane * 4 < 2 ** 31
+    with pytest.raises(sth.SthError):
+        sth.layout(4, 23200, 23200, 7, "fp32")
 
 
 def test_null_handles_are_errors(sth):
