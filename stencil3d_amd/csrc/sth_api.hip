@@ -380,7 +380,6 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (c->fmt != FMT_F32 || c->no_fuse) return false;
   if (t.steps_per_pass) return t.steps_per_pass == 2;
-  if (c->g.npts == 25) return false;
   const Plan2 p = make_plan2(c, t);
   return p.ntk * p.ntj >= 64;
 }
@@ -402,22 +401,38 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   p.ok = g.ok;
   p.ntk = ((int)c->g.zs + p.ok - 1) / p.ok;
   p.ntj = ((int)c->g.ys + p.oj - 1) / p.oj;
-  const int xs = (int)c->g.xs, tiles = p.ntk * p.ntj;
-  int ch = (int)t.chunks;
-  if (ch == 0) {
-    // one workgroup per CU fits (LDS): about twelve rounds of workgroups, so
-    // the last round is a small tail, and chunks of at least 8R planes (each
-    // chunk recomputes 2R planes of the first step, re-reads 4R planes of u).
-    ch = std::max(1, (12 * 256 + tiles - 1) / tiles);
-    ch = std::min(ch, std::max(1, xs / (8 * c->R)));
-  }
-  ch = std::max(1, std::min(ch, xs));
-  p.chunk_len = (xs + ch - 1) / ch;
+  const int xs = (int)c->g.xs, tiles = p.ntk * p.ntj, R = c->R;
   // A chunk of n planes takes n + 2R + 1 steps and the kernel runs them in
   // whole groups of 2R + 2: n = 1 mod 2R+2 leaves no idle step (except in
   // a shorter last chunk).
-  const int ns = 2 * c->R + 2;
-  p.chunk_len = std::min(xs, p.chunk_len + ((1 - p.chunk_len) % ns + ns) % ns);
+  const int ns = 2 * R + 2;
+  auto chunk_len = [&](int ch) {
+    const int n = (xs + ch - 1) / ch;
+    return std::min(xs, n + ((1 - n) % ns + ns) % ns);
+  };
+  int ch = (int)t.chunks;
+  if (ch == 0) {
+    // One workgroup per CU fits: the pass takes about ceil(workgroups /
+    // CUs) rounds of (chunk length + 2R + 1) steps.  Take the chunk count
+    // with the least such time among those giving at least six rounds (few
+    // long rounds balance badly) and chunks of at least 8R planes (each
+    // chunk recomputes 2R planes of the first step and re-reads 4R of u).
+    const int64_t slots = std::max(1, c->cus);
+    const int most = std::max(1, std::min(32, xs / (8 * R)));
+    const int least = (int)std::min<int64_t>(most, (6 * slots + tiles - 1) / tiles);
+    int64_t best = -1;
+    for (int k = std::max(1, least); k <= most; k++) {
+      const int n = chunk_len(k), chunks = (xs + n - 1) / n;
+      const int64_t rounds = ((int64_t)tiles * chunks + slots - 1) / slots;
+      const int64_t cost = rounds * (n + 2 * R + 1);
+      if (best < 0 || cost < best) {
+        best = cost;
+        ch = k;
+      }
+    }
+  }
+  ch = std::max(1, std::min(ch, xs));
+  p.chunk_len = chunk_len(ch);
   p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
   p.grid = (unsigned)(tiles * p.chunks);
   return p;

@@ -48,14 +48,14 @@ template <int NPTS, int MJ, int NW>
 struct Fused2Shape {
   static constexpr int R = Terms<NPTS>::R;
   static constexpr int EJ = NW * MJ, EK = 64;  // step-one tile
-  // Output tile.  For R <= 2 its rows are 48 points (192 B): every tile
-  // writes whole 64-byte segments that no other tile touches; rows of
-  // 64 - 2R points would share partly written segments with the next tile,
-  // which at 1040^3 cost 10-30 % and made the time depend on where the
-  // allocator placed the fields.  For R = 4 the narrower row would need 16 %
-  // more tiles of the (compute-bound) 25-point pass, so it keeps 56.
+  // Output tile.  Its rows are 48 points (192 B): every tile writes whole
+  // 64-byte segments that no other tile touches.  Rows of 64 - 2R points
+  // would share partly written segments with the next tile, which at 1040^3
+  // cost 10-30 % (R <= 2) and 15 % (R = 4, although 48-point rows need 16 %
+  // more tiles) and made the time depend on where the allocator placed the
+  // fields.
   static constexpr int OJ = EJ - 2 * R;
-  static constexpr int OK = R <= 2 ? ((EK - 2 * R) / 16) * 16 : EK - 2 * R;
+  static constexpr int OK = ((EK - 2 * R) / 16) * 16;
   static constexpr int LW = EK + 2 * R;  // u plane + R halo
   static constexpr int LU = (EJ + 2 * R) * LW;
   static constexpr int LR = EJ * EK;  // one plane of u_{n+1} (or s) on E1
