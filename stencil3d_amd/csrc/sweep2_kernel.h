@@ -88,6 +88,9 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   // R+1 planes of s, each lane's own values (step two needs s(p-R-1)).
   __shared__ float SM[2 * F::LU + (R + 3) * F::LR];
 
+  // Work item -> (chunk, tile): k fastest, then j, then the chunk.  (Tile
+  // orders that group the workgroups of an XCD into compact 2D patches, so
+  // their overlapping reads would meet in L2, measured 2-8 % slower.)
   int bid = blockIdx.x;
   if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
   const int tk = bid % a.ntk;
