@@ -263,7 +263,7 @@ def measured_traffic(n, dtype_code, tune):
         return None, None
     if tune.get("steps_per_pass") == 2:
         prefix = (f"void sth::sweep2_kernel<{NPTS}, {tune['rows_per_wave']}, "
-                  f"{tune['waves']}>")
+                  f"{tune['waves']}")
     else:
         # the last template argument (fold of the last k-tile) follows the grid
         prefix = (f"void sth::sweep_kernel<{NPTS}, 1, {dtype_code}, {tune['rows_per_wave']}, "
@@ -403,8 +403,13 @@ def main():
     tg = {"fp32_frac_of_peak": round(achieved / PEAK_GBS, 4), "fp32_frac_target": 0.8,
           "fp32_met": achieved / PEAK_GBS >= 0.8}
     if "fp16" in extra:
+        # against the FP32 headline (two steps per pass when that is the
+        # default); the ratio against the one-step FP32 sweep, the same
+        # schedule as the 16-bit sweep, is reported beside it
         sp = extra["fp16"]["speedup_vs_fp32"]
         tg.update({"fp16_speedup": sp, "fp16_speedup_target": 1.6, "fp16_speedup_met": sp >= 1.6})
+        if "speedup_vs_fp32_one_step" in extra["fp16"]:
+            tg["fp16_speedup_vs_fp32_one_step"] = extra["fp16"]["speedup_vs_fp32_one_step"]
         if "err_vs_fp64" in extra["fp16"]:
             mr = extra["fp16"]["err_vs_fp64"]["maxrel"]
             tg.update({"fp16_maxrel": mr, "fp16_maxrel_target": 1e-3,

@@ -90,9 +90,11 @@ typedef struct {
 /* Tile schedule knobs of the 2.5D sweep (the GPU counterpart of --yb/--zb,
  * iterators.cpp:42-89).  Zero fields select the built-in defaults. */
 typedef struct {
-  uint32_t rows_per_wave; /* j rows held in registers by one wave: 1, 2, 4, 8 */
-  uint32_t waves;         /* waves per workgroup: 2, 4, 8 */
-  uint32_t chunks;        /* i-range split: 1..64 chunks per (j,k) tile */
+  uint32_t rows_per_wave; /* j rows held in registers by one wave: 1, 2, 4, 8
+                             (two-step pass: 3 or 4) */
+  uint32_t waves;         /* waves per workgroup: 2, 4, 8 (two-step pass: 8, 12, 16) */
+  uint32_t chunks;        /* i-range split: 1..64 chunks per (j,k) tile (0: chosen from
+                             the grid and the device's CU count) */
   uint32_t xcd_remap;     /* 1: keep neighbouring tiles on one XCD (default), 0: off */
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
   uint32_t steps_per_pass; /* LEAPFROG, FP32: 2 = two steps per pass over memory
