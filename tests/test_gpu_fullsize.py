@@ -1,0 +1,78 @@
+"""GPU properties at the headline size (1040^3 interior, 25 points), where the
+oracle cannot follow: size-independent checks only.
+
+  * The two-steps-per-pass schedule (the FP32 LEAPFROG default) and the
+    one-step sweep give the same bits for u and v after six steps (hashes of
+    the downloaded fields, halos included), with a nonzero source term.
+  * The reference's own known-answer check (u = s = 1, v = 0; each REF sweep
+    adds the stencil's constant): every interior point within 0.1 of the
+    sweep count (iterators.cpp:304-325), the maximum equal to the exact FP32
+    recurrence, and a wrong expectation flags every point."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 1040
+
+
+def field_hash(ctx, field, halo, buf):
+    import xxhash
+
+    ctx.download(field, halo, out=buf)
+    return xxhash.xxh3_128_hexdigest(memoryview(buf).cast("B"))
+
+
+def test_two_step_pass_equals_one_step_at_headline_size(gpu):
+    npts, R = 25, 4
+    c1 = 0.16
+    c0 = float(np.float32(2.0 - 12.0 * c1))
+    c2 = np.linspace(-2e-3, 3e-3, 6).astype(np.float32)
+    buf = np.empty((N + 2 * R,) * 3, np.float32)
+    hashes = {}
+    for spp in (0, 1):  # 0: the default schedule
+        with gpu.Stencil(N, N, N, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 0)
+            ctx.init_gaussian("v", 1.0, 8.0, 1e-3, 1, 1)
+            ctx.init_uniform("s", 7, 2, -1.0, 1.0, 0.0)
+            ctx.timer_start()
+            ctx.leapfrog(6, c0, c1, c2)
+            _, launches = ctx.timer_read()
+            assert launches == (3 if spp == 0 else 6)  # the default runs three passes
+            hashes[spp] = (field_hash(ctx, "u", R, buf), field_hash(ctx, "v", R, buf))
+    assert hashes[0] == hashes[1]
+
+
+def kat_value(orc, npts, sweeps):
+    """v after `sweeps` REF sweeps with u = s = 1, v = 0: the scalar FP32
+    recurrence v <- fma(1, P, fma(-A, 1, v)) with P the stencil's constant,
+    each rounding done on exact rationals."""
+    def rnd(x):
+        return float(np.float32(float(x)))
+
+    coef0, A, w, _ = orc.coeffs(npts)
+    terms = orc.terms(npts)
+    a0, r0 = terms[0]
+    p = rnd(Fraction(float(coef0)) + Fraction(rnd(Fraction(float(w[a0, r0 - 1])) * 2)))
+    for ax, r in terms[1:]:
+        p = rnd(Fraction(float(w[ax, r - 1])) * 2 + Fraction(p))
+    v = 0.0
+    for _ in range(sweeps):
+        v = rnd(Fraction(p) + Fraction(rnd(Fraction(v) - Fraction(float(A)))))
+    return v
+
+
+def test_reference_kat_at_headline_size(gpu, orc):
+    npts = 25
+    with gpu.Stencil(N, N, N, npts=npts) as ctx:
+        ctx.init_const(0.0, 1.0, 1.0)
+        ctx.iterate(3)
+        assert ctx.check(3.0) == 0
+        assert ctx.check(4.0) == N ** 3
+        mx, l2 = ctx.norms("v")
+    want = kat_value(orc, npts, 3)
+    assert mx == want
+    assert l2 == pytest.approx(want * N ** 1.5, rel=1e-9)
