@@ -380,8 +380,15 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (c->fmt != FMT_F32 || c->no_fuse) return false;
   if (t.steps_per_pass) return t.steps_per_pass == 2;
+  // Measured on cubic grids (FP32 LEAPFROG, one-step time / two-step time):
+  // 7 points 0.98 at 4 tiles, 1.26-2.30 from 9; 13 points 0.71 at 9 tiles,
+  // 1.01-1.64 from 12; 25 points 0.41-0.80 up to 42 tiles, 0.98-1.18 from
+  // 56.  Each chunk also recomputes 2R+1 planes, so short grids stay
+  // one-step.
   const Plan2 p = make_plan2(c, t);
-  return p.ntk * p.ntj >= 64;
+  const int npts = (int)c->g.npts;
+  const int least = npts == 7 ? 8 : npts == 13 ? 12 : 50;
+  return p.ntk * p.ntj >= least && (int)c->g.xs >= 16 * c->R;
 }
 
 // Fused shapes (rows per wave, waves): a tuning that names a compiled shape

@@ -121,3 +121,29 @@ def test_two_steps_rejected_for_16bit(gpu):
     with gpu.Stencil(16, 16, 64, npts=25, dtype="fp16") as ctx:
         with pytest.raises(gpu.SthError):
             ctx.set_tuning(steps_per_pass=2)
+
+
+@pytest.mark.parametrize("npts,n,fused", [(7, 96, False), (7, 128, True), (13, 128, False),
+                                          (13, 160, True), (25, 256, False), (25, 320, True)])
+def test_default_schedule_by_grid_size(gpu, orc, npts, n, fused):
+    """The default FP32 LEAPFROG schedule takes two steps per pass where that
+    measured faster (enough output tiles for the stencil): four steps are two
+    passes or four sweeps, and either way bit-identical to the oracle."""
+    R = orc.radius(npts)
+    shape = (n, n, n)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 23, 0.125)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.full(4, 1e-3, np.float32)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.timer_start()
+        ctx.leapfrog(4, c0, c1, c2)
+        _, launches = ctx.timer_read()
+        gu = ctx.download("u", R)
+    assert launches == (2 if fused else 4)
+    if n <= 160:  # the oracle follows these sizes in seconds
+        want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 4)
+        assert same_bits(gu, want_u)
