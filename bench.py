@@ -127,9 +127,10 @@ def run_batch(S, steps=100, nshots=64, seed=2):
     """Batching form: nshots independent 13-point FP32 shots, sides drawn from
     seed 2 uniformly among {96,128,192,256,320,384} per axis, a Ricker source at
     a random interior point of each, c dt/dx = 0.4; `steps` steps in one call,
-    timed for the persistent launch (in a hipGraph), and for one plain launch
-    per shot per step (eager and in a hipGraph).  Tile and planes per item come
-    from the autotuner."""
+    timed for the persistent launch (in a hipGraph and eager), for one plain
+    launch per shot per step (eager and in a hipGraph), and for the persistent
+    two-steps-per-pass launch.  Tile and planes per item of the one-step
+    persistent launch come from the autotuner."""
     import numpy as np
     rng = np.random.default_rng(seed)
     dims = [tuple(int(x) for x in rng.choice([96, 128, 192, 256, 320, 384], 3))
@@ -146,7 +147,9 @@ def run_batch(S, steps=100, nshots=64, seed=2):
         for name, method, graph in (("persistent_graph", S.BATCH_PERSISTENT, True),
                                     ("persistent_eager", S.BATCH_PERSISTENT, False),
                                     ("sequential_eager", S.BATCH_SEQUENTIAL, False),
-                                    ("sequential_graph", S.BATCH_SEQUENTIAL, True)):
+                                    ("sequential_graph", S.BATCH_SEQUENTIAL, True),
+                                    ("two_step_graph", S.BATCH_TWO_STEP, True),
+                                    ("two_step_eager", S.BATCH_TWO_STEP, False)):
             for q, (d, sp) in enumerate(zip(dims, srcs)):
                 sh = bt.shot(q)
                 sh.init_point("u", 0, 0, 0, 0.0)
@@ -169,6 +172,7 @@ def run_batch(S, steps=100, nshots=64, seed=2):
                 f"{out[name]['gpts']} Gpts/s")
     p, q = out["persistent_graph"]["gpu_ms_per_step"], out["sequential_eager"]["gpu_ms_per_step"]
     out["persistent_speedup_vs_sequential"] = round(q / p, 3)
+    out["two_step_speedup_vs_persistent"] = round(p / out["two_step_graph"]["gpu_ms_per_step"], 3)
     return out
 
 

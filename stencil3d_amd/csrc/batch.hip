@@ -8,6 +8,8 @@
 //    queue (an atomic ticket counter), largest items first, and runs each with
 //    the same tile engine as the single-grid sweep (sweep_tile).
 //  * sequential: one plain sweep launch per shot per step.
+//  * two steps per pass: one persistent launch per two steps, items of the
+//    two-step sweep (sweep2_tile) of every shot from a queue.
 // Either way the time loop can be captured into one hipGraph per call.
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -17,6 +19,8 @@
 #include <vector>
 
 #include "sth_internal.h"
+#include "sweep2_dispatch.h"
+#include "sweep2_kernel.h"
 #include "sweep_dispatch.h"
 
 using namespace sth;
@@ -112,6 +116,12 @@ struct sth_batch {
   int total_items = 0;
   unsigned ticket_base = 0;
   bool tables_ok = false;
+  // device tables of the two-steps-per-pass kernel (rebuilt on every call:
+  // the shots' buffers trade places with their twins after each pass)
+  Batch2Shot *shot2 = nullptr;
+  Batch2Item *items2 = nullptr;
+  size_t shot2_cap = 0, items2_cap = 0;
+  int total_items2 = 0;
   hipGraphExec_t exec = nullptr;
   bool timing = false;
   double timed_ms = 0.0;
@@ -222,6 +232,136 @@ int build_tables(sth_batch *b) {
   return STH_OK;
 }
 
+// The two-step kernel's shape: the stencil's default (rows per wave, waves).
+void shape2(uint32_t npts, int &mj, int &nw) {
+  mj = npts == 25 ? 3 : 4;
+  nw = 16;
+}
+
+unsigned persistent_grid2(const sth_batch *b, int mj, int nw) {
+  int per_cu = 0;
+  const void *fn = batch2_fn((int)b->npts, mj, nw);
+  if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * nw, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return (unsigned)(per_cu * b->cus);
+}
+
+// Work list of a pass over all shots, from the shots' current buffers (the
+// twins must exist).  i-chunks: `planes` planes per item when set, else as
+// many as give about six rounds of the persistent grid (at least 8R planes
+// per chunk: each chunk recomputes 2R+1 planes).
+int build_tables2(sth_batch *b, int mj, int nw) {
+  const size_t ns = b->shots.size();
+  sth_tuning t = b->tune;
+  t.rows_per_wave = (uint32_t)mj;
+  t.waves = (uint32_t)nw;
+  t.steps_per_pass = 2;
+  int64_t tiles = 0;
+  for (sth_ctx *c : b->shots) {
+    t.chunks = 1;
+    const Plan2 p = make_plan2(c, t);
+    tiles += (int64_t)p.ntk * p.ntj;
+  }
+  const int64_t slots = persistent_grid2(b, mj, nw);
+  const int target = (int)std::max<int64_t>(1, (6 * slots + tiles - 1) / std::max<int64_t>(1, tiles));
+  std::vector<Batch2Shot> shots(ns);
+  std::vector<Batch2Item> items;
+  std::vector<int> len;
+  for (size_t q = 0; q < ns; q++) {
+    sth_ctx *c = b->shots[q];
+    const int xs = (int)c->g.xs;
+    const int most = std::max(1, xs / (8 * c->R));
+    t.chunks = (uint32_t)std::min(64, b->planes ? std::max(1, (xs + (int)b->planes - 1) /
+                                                                  (int)b->planes)
+                                                : std::min(target, most));
+    const Plan2 p = make_plan2(c, t);
+    const size_t o = c->L.origin * c->eb;
+    Batch2Shot &sh = shots[q];
+    sh.buf[0] = (const char *)c->base[0] + o;
+    sh.buf[1] = (const char *)c->base[1] + o;
+    sh.buf[2] = (const char *)c->twin[0] + o;
+    sh.buf[3] = (const char *)c->twin[1] + o;
+    sh.s = (const char *)c->base[2] + o;
+    sh.pitch = (int64_t)c->L.pitch;
+    sh.plane = (int64_t)c->L.plane;
+    sh.xs = xs;
+    sh.ys = (int)c->g.ys;
+    sh.zs = (int)c->g.zs;
+    sh.ntk = p.ntk;
+    sh.ntj = p.ntj;
+    sh.chunk_len = p.chunk_len;
+    for (int ch = 0; ch < p.chunks; ch++)
+      for (int tj = 0; tj < p.ntj; tj++)
+        for (int tk = 0; tk < p.ntk; tk++) {
+          items.push_back({(int)q, tk, tj, ch});
+          len.push_back(std::min(xs, (ch + 1) * p.chunk_len) - ch * p.chunk_len);
+        }
+  }
+  std::vector<int> order(items.size());
+  for (size_t q = 0; q < order.size(); q++) order[q] = (int)q;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return len[x] > len[y]; });
+  std::vector<Batch2Item> sorted(items.size());
+  for (size_t q = 0; q < order.size(); q++) sorted[q] = items[order[q]];
+  if (b->shot2_cap < ns) {
+    if (b->shot2) (void)hipFree(b->shot2);
+    b->shot2 = nullptr;
+    b->shot2_cap = 0;
+    STH_TRY(hipMalloc((void **)&b->shot2, ns * sizeof(Batch2Shot)));
+    b->shot2_cap = ns;
+  }
+  if (b->items2_cap < sorted.size()) {
+    if (b->items2) (void)hipFree(b->items2);
+    b->items2 = nullptr;
+    b->items2_cap = 0;
+    STH_TRY(hipMalloc((void **)&b->items2, sorted.size() * sizeof(Batch2Item)));
+    b->items2_cap = sorted.size();
+  }
+  STH_TRY(hipMemcpyAsync(b->shot2, shots.data(), ns * sizeof(Batch2Shot), hipMemcpyHostToDevice,
+                         b->stream));
+  STH_TRY(hipMemcpyAsync(b->items2, sorted.data(), sorted.size() * sizeof(Batch2Item),
+                         hipMemcpyHostToDevice, b->stream));
+  STH_TRY(hipStreamSynchronize(b->stream));
+  b->total_items2 = (int)sorted.size();
+  return STH_OK;
+}
+
+// Queues `passes` two-step passes (steps 2q, 2q+1 use c2[2q], c2[2q+1]).
+int enqueue_passes(sth_batch *b, uint32_t passes, float c0, float c1, const float *c2, int ub,
+                   uint64_t *launches) {
+  int mj, nw;
+  shape2(b->npts, mj, nw);
+  const unsigned G = persistent_grid2(b, mj, nw);
+  const unsigned per_launch = (unsigned)b->total_items2 + G;
+  if ((uint64_t)b->ticket_base + (uint64_t)per_launch * passes >= 0xF0000000ull) {
+    STH_TRY(launch_zero_u32(b->ticket, 1, b->stream));
+    b->ticket_base = 0;
+  }
+  SweepArgs wts;
+  memset(&wts, 0, sizeof(wts));
+  fill_weights(b->shots[0], true, wts);
+  Batch2Args a;
+  memset(&a, 0, sizeof(a));
+  a.shots = b->shot2;
+  a.items = b->items2;
+  a.total_items = b->total_items2;
+  a.ticket = b->ticket;
+  a.ub = ub;
+  a.c0 = c0;
+  a.c1 = c1;
+  memcpy(a.w, wts.w, sizeof(a.w));
+  for (uint32_t q = 0; q < passes; q++) {
+    a.ticket_base = b->ticket_base;
+    a.phase = (int)(q & 1);
+    a.c2a = c2 ? c2[2 * q] : 0.0f;
+    a.c2b = c2 ? c2[2 * q + 1] : 0.0f;
+    STH_TRY(launch_batch2((int)b->npts, mj, nw, G, b->stream, a));
+    b->ticket_base += per_launch;
+    (*launches)++;
+  }
+  return STH_OK;
+}
+
 // Orders the batch stream after all work queued on the shots' streams.
 int join_shots(sth_batch *b) {
   for (sth_ctx *c : b->shots) {
@@ -297,6 +437,18 @@ int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float 
   return STH_OK;
 }
 
+// The call's launches: `passes` two-step passes, then `single` steps of the
+// one-step method (persistent for STH_BATCH_TWO_STEP).
+int enqueue_all(sth_batch *b, uint32_t passes, uint32_t single, float c0, float c1,
+                const float *c2, int base, int parity0, uint64_t *launches) {
+  int st = STH_OK;
+  if (passes && (st = enqueue_passes(b, passes, c0, c1, c2, parity0, launches))) return st;
+  if (!single) return STH_OK;
+  const int method = base == STH_BATCH_TWO_STEP ? STH_BATCH_PERSISTENT : base;
+  return enqueue_steps(b, single, c0, c1, c2 ? c2 + 2 * passes : nullptr, method, parity0,
+                       launches);
+}
+
 }  // namespace
 
 extern "C" {
@@ -309,6 +461,8 @@ int sth_batch_destroy(sth_batch *b) {
     if (c) sth_destroy(c);
   if (b->shot_args) (void)hipFree(b->shot_args);
   if (b->items) (void)hipFree(b->items);
+  if (b->shot2) (void)hipFree(b->shot2);
+  if (b->items2) (void)hipFree(b->items2);
   if (b->ticket) (void)hipFree(b->ticket);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
@@ -399,11 +553,38 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
   if (!b) return STH_ERR_ARG;
   const int base = (int)(method & 0xff);
   const bool graph = (method & STH_BATCH_GRAPH) != 0;
-  if (base != STH_BATCH_PERSISTENT && base != STH_BATCH_SEQUENTIAL) return STH_ERR_ARG;
+  if (base != STH_BATCH_PERSISTENT && base != STH_BATCH_SEQUENTIAL && base != STH_BATCH_TWO_STEP)
+    return STH_ERR_ARG;
   if (steps == 0) return STH_OK;
   const int parity0 = b->shots[0]->ubuf;
-  int st = build_tables(b);
-  if (st) return st;
+  const uint32_t passes = base == STH_BATCH_TWO_STEP ? steps / 2 : 0;
+  int st = STH_OK;
+  if (passes) {
+    for (sth_ctx *c : b->shots)
+      if ((st = ensure_twins(c))) return st;
+    int mj, nw;
+    shape2(b->npts, mj, nw);
+    if ((st = build_tables2(b, mj, nw))) return st;
+    if (passes & 1) {
+      // after an odd number of passes each shot's u and v live in the
+      // buffers that are now its twins: swap the names now, so the one-step
+      // table below (for an odd last step) already points at them
+      for (sth_ctx *c : b->shots) {
+        std::swap(c->base[0], c->twin[0]);
+        std::swap(c->base[1], c->twin[1]);
+      }
+      b->tables_ok = false;
+    }
+  }
+  const uint32_t single = steps - 2 * passes;  // one-step steps of this call
+  if (single && (st = build_tables(b))) {
+    if (passes & 1)
+      for (sth_ctx *c : b->shots) {  // nothing was queued: restore the names
+        std::swap(c->base[0], c->twin[0]);
+        std::swap(c->base[1], c->twin[1]);
+      }
+    return st;
+  }
   if ((st = join_shots(b))) return st;
   if (b->exec) {  // the previous call's graph has been launched before: free it
     STH_TRY(hipStreamSynchronize(b->stream));
@@ -418,7 +599,7 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
     STH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_zero_u32(b->ticket, 1, b->stream);
     b->ticket_base = 0;
-    if (e == hipSuccess) st = enqueue_steps(b, steps, c0, c1, c2, base, parity0, &launches);
+    if (e == hipSuccess) st = enqueue_all(b, passes, single, c0, c1, c2, base, parity0, &launches);
     const hipError_t ec = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || st != STH_OK || ec != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
@@ -433,7 +614,7 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
     if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
   } else {
     if (b->timing) STH_TRY(hipEventRecord(b->ev0, b->stream));
-    st = enqueue_steps(b, steps, c0, c1, c2, base, parity0, &launches);
+    st = enqueue_all(b, passes, single, c0, c1, c2, base, parity0, &launches);
     if (st) return st;
     if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
   }

@@ -100,6 +100,9 @@ int hip_status(hipError_t e);
 Plan make_plan(const sth_ctx *c, const sth_tuning &t);
 bool fused_applies(const sth_ctx *c, const sth_tuning &t);
 Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t);
+// Allocates the twins of u and v on first use and brings their halos in
+// step with the fields (stream-ordered on the context's stream).
+int ensure_twins(sth_ctx *c);
 void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a);
 void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a);
 int push_slot(sth_ctx *c);

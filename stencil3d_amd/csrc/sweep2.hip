@@ -36,4 +36,34 @@ hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st
   return hipErrorInvalidValue;
 }
 
+#define STH_BATCH2_LIST(X) X(7, 4, 16) X(13, 4, 16) X(25, 3, 16)
+
+bool batch2_supported(int npts, int mj, int nw) {
+#define STH_B2_SUP(N, MJ, NW) \
+  if (npts == N && mj == MJ && nw == NW) return true;
+  STH_BATCH2_LIST(STH_B2_SUP)
+#undef STH_B2_SUP
+  return false;
+}
+
+const void *batch2_fn(int npts, int mj, int nw) {
+#define STH_B2_FN(N, MJ, NW) \
+  if (npts == N && mj == MJ && nw == NW) return (const void *)&batch2_kernel<N, MJ, NW>;
+  STH_BATCH2_LIST(STH_B2_FN)
+#undef STH_B2_FN
+  return nullptr;
+}
+
+hipError_t launch_batch2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+                         const Batch2Args &b) {
+#define STH_B2_LAUNCH(N, MJ, NW)                                                   \
+  if (npts == N && mj == MJ && nw == NW) {                                         \
+    hipLaunchKernelGGL((batch2_kernel<N, MJ, NW>), dim3(grid), dim3(64 * NW), 0, st, b); \
+    return hipGetLastError();                                                      \
+  }
+  STH_BATCH2_LIST(STH_B2_LAUNCH)
+#undef STH_B2_LAUNCH
+  return hipErrorInvalidValue;
+}
+
 }  // namespace sth

@@ -6,6 +6,7 @@
 namespace sth {
 
 struct Sweep2Args;
+struct Batch2Args;
 
 // Output tile of a compiled (stencil, rows per wave, waves) shape.
 struct Fused2Geom {
@@ -16,5 +17,11 @@ bool fused2_supported(int npts, int mj, int nw);
 Fused2Geom fused2_geom(int npts, int mj, int nw);
 hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
                          const Sweep2Args &a);
+// The batch form (persistent, items of many shots from a queue) exists for
+// each stencil's default shape.
+bool batch2_supported(int npts, int mj, int nw);
+const void *batch2_fn(int npts, int mj, int nw);
+hipError_t launch_batch2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+                         const Batch2Args &b);
 
 }  // namespace sth

@@ -72,8 +72,12 @@ __device__ __forceinline__ void static_for(Fn &&f) {
   static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
+// One work item of a pass: output tile (tk, tj), i-chunk ch.  Called by the
+// single-grid kernel (one item per workgroup) and the batch kernel (items
+// of many grids drawn from a queue; LDS is reused between items).
 template <int NPTS, int MJ, int NW>
-__global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
+__device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, const int tj,
+                                            const int ch) {
   using Tm = Terms<NPTS>;
   using F = Fused2Shape<NPTS, MJ, NW>;
   constexpr int R = Tm::R, T = Tm::T;
@@ -88,15 +92,6 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   // R+1 planes of s, each lane's own values (step two needs s(p-R-1)).
   __shared__ float SM[2 * F::LU + (R + 3) * F::LR];
 
-  // Work item -> (chunk, tile): k fastest, then j, then the chunk.  (Tile
-  // orders that group the workgroups of an XCD into compact 2D patches, so
-  // their overlapping reads would meet in L2, measured 2-8 % slower.)
-  int bid = blockIdx.x;
-  if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
-  const int tk = bid % a.ntk;
-  const int tt = bid / a.ntk;
-  const int tj = tt % a.ntj;
-  const int ch = tt / a.ntj;
   const int ib = ch * a.chunk_len;
   const int ie = min(a.xs, ib + a.chunk_len);
   if (ib >= ie) return;
@@ -383,6 +378,87 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
       constexpr int k = decltype(kc)::value;
       step(p + k, (k & 1) ? SB : SA, kc);
     });
+  }
+}
+
+template <int NPTS, int MJ, int NW>
+__global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
+  // Work item -> (chunk, tile): k fastest, then j, then the chunk.  (Tile
+  // orders that group the workgroups of an XCD into compact 2D patches, so
+  // their overlapping reads would meet in L2, measured 2-8 % slower.)
+  int bid = blockIdx.x;
+  if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
+  const int tk = bid % a.ntk;
+  const int tt = bid / a.ntk;
+  sweep2_tile<NPTS, MJ, NW>(a, tk, tt % a.ntj, tt / a.ntj);
+}
+
+// ---- many grids ("shots") in one launch per pass ----------------------------
+
+// A shot's u/v buffers: base a, base b and their twins (element (0,0,0) of
+// the interior), and s.  A pass reads the pair of phase ph (0: bases,
+// 1: twins) and writes the other pair.
+struct Batch2Shot {
+  const void *buf[4];  // a, b, twin of a, twin of b
+  const void *s;
+  int64_t pitch, plane;
+  int xs, ys, zs, ntk, ntj, chunk_len;
+};
+
+struct Batch2Item {
+  int shot, tk, tj, ch;
+};
+
+struct Batch2Args {
+  const Batch2Shot *shots;
+  const Batch2Item *items;  // sorted work list
+  int total_items;
+  unsigned *ticket;      // queue counter
+  unsigned ticket_base;  // tickets taken by earlier launches since the last reset
+  int ub;                // buffer of u: 0 = a, 1 = b
+  int phase;             // 0: read the bases, write the twins; 1: the reverse
+  float c0, c1, c2a, c2b;
+  float w[12];
+};
+
+// Every block takes tickets until it draws one past the end, so a launch
+// consumes exactly total_items + gridDim.x tickets.
+template <int NPTS, int MJ, int NW>
+__global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
+  __shared__ int sh_item;
+  for (;;) {
+    if (threadIdx.x == 0) sh_item = (int)(atomicAdd(b.ticket, 1u) - b.ticket_base);
+    __syncthreads();
+    const int item = sh_item;
+    __syncthreads();
+    if (item >= b.total_items) break;
+    const Batch2Item it = b.items[item];
+    const Batch2Shot &sh = b.shots[it.shot];
+    const int vb = 1 - b.ub, in = 2 * b.phase, out = 2 - in;
+    Sweep2Args a;
+    a.u = sh.buf[b.ub + in];
+    a.v = sh.buf[vb + in];
+    a.s = sh.s;
+    a.o1 = const_cast<void *>(sh.buf[vb + out]);  // u_{n+1}: the halo of v
+    a.o2 = const_cast<void *>(sh.buf[b.ub + out]);  // u_{n+2}: the halo of u
+    a.pitch = sh.pitch;
+    a.plane = sh.plane;
+    a.xs = sh.xs;
+    a.ys = sh.ys;
+    a.zs = sh.zs;
+    a.ntk = sh.ntk;
+    a.ntj = sh.ntj;
+    a.chunks = 0;
+    a.chunk_len = sh.chunk_len;
+    a.remap = 0;
+    a.c0 = b.c0;
+    a.c1 = b.c1;
+    a.c2a = b.c2a;
+    a.c2b = b.c2b;
+#pragma unroll
+    for (int t = 0; t < 12; t++) a.w[t] = b.w[t];
+    sweep2_tile<NPTS, MJ, NW>(a, it.tk, it.tj, it.ch);
+    __syncthreads();  // LDS is reused by the next item
   }
 }
 

@@ -140,3 +140,63 @@ def test_default_tuning_large_shot_matches_single_grid(gpu, orc):
             for q in range(len(dims)):
                 got = bt.shot(q).download("u", R)
                 assert np.array_equal(got.view(np.uint32), want[q].view(np.uint32)), (method, q)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("steps", [5, 6])
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_two_step_batch_matches_oracle(gpu, orc, graph, steps, npts):
+    """Two LEAPFROG steps per pass over all shots (an odd last step runs
+    one-step): every shot bit-identical to the oracle, halos included."""
+    R = orc.radius(npts)
+    fields = shot_fields(orc, DIMS, npts, 40)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-2, 1e-2, steps).astype(np.float32)
+    with gpu.Batch(DIMS, npts=npts) as bt:
+        load(bt, fields, R)
+        bt.timer_start()
+        bt.leapfrog(steps, c0, c1, c2, method=gpu.BATCH_TWO_STEP, graph=graph)
+        _, launches = bt.timer_read()
+        got = [(bt.shot(q).download("u", R), bt.shot(q).download("v", R))
+               for q in range(len(DIMS))]
+    assert launches == steps // 2 + steps % 2
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, steps)
+        wu, wv = (a, b) if newest == 0 else (b, a)
+        assert np.array_equal(got[q][0].view(np.uint32), wu.view(np.uint32)), q
+        assert np.array_equal(got[q][1].view(np.uint32), wv.view(np.uint32)), q
+
+
+def test_two_step_batch_mixed_with_other_methods(gpu, orc):
+    """Calls of every method in a row (the two-step calls swap each shot's
+    buffers with its twins), an upload in between: still the oracle's bits."""
+    npts = 13
+    R = orc.radius(npts)
+    fields = shot_fields(orc, DIMS, npts, 50)
+    c0, c1 = coeffs(orc, npts)
+    plan = [(3, gpu.BATCH_TWO_STEP, False), (2, gpu.BATCH_PERSISTENT, False),
+            (5, gpu.BATCH_TWO_STEP, True), (1, gpu.BATCH_SEQUENTIAL, False),
+            (4, gpu.BATCH_TWO_STEP, False)]
+    c2 = np.linspace(-5e-3, 5e-3, sum(p[0] for p in plan)).astype(np.float32)
+    new_v = orc.fill_uniform(fields[1][0], 77, 1, halo=0.5)
+    with gpu.Batch(DIMS, npts=npts) as bt:
+        load(bt, fields, R)
+        n = 0
+        for k, (steps, method, graph) in enumerate(plan):
+            bt.leapfrog(steps, c0, c1, c2[n:n + steps], method=method, graph=graph)
+            n += steps
+            if k == 1:
+                bt.sync()
+                bt.shot(1).upload("v", new_v, R)
+        got = [bt.shot(q).download("u", R) for q in range(len(DIMS))]
+    for q, (g, a, b, s) in enumerate(fields):
+        a, b = a.copy(), b.copy()
+        n = 0
+        for k, (steps, _, _) in enumerate(plan):
+            newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2[n:n + steps], steps)
+            if newest == 1:
+                a, b = b, a  # keep the newest field in a
+            n += steps
+            if k == 1 and q == 1:
+                b = new_v.copy()
+        assert np.array_equal(got[q].view(np.uint32), a.view(np.uint32)), q
