@@ -57,13 +57,15 @@ def test_batch_matches_oracle(gpu, orc, method, graph, npts):
 
 def test_persistent_equals_sequential_config_shapes(gpu, orc):
     """Eight shots with sides from the survey config's set, 12 steps: the
-    persistent, sequential and graphed runs agree bit for bit."""
+    persistent, sequential, two-steps-per-pass and graphed runs agree bit for
+    bit."""
     rng = np.random.default_rng(2)
     dims = [tuple(int(x) for x in rng.choice([96, 128, 192], 3)) for _ in range(8)]
     npts, R = 13, 2
     c0, c1 = coeffs(orc, npts)
     results = []
-    for method, graph in ((0, False), (1, False), (0, True)):
+    for method, graph in ((0, False), (1, False), (0, True), (gpu.BATCH_TWO_STEP, False),
+                          (gpu.BATCH_TWO_STEP, True)):
         with gpu.Batch(dims, npts=npts) as bt:
             for q in range(len(dims)):
                 bt.shot(q).init_gaussian("u", 1.0, 6.0, 1e-3, 1, q)

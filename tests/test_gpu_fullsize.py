@@ -7,7 +7,9 @@ oracle cannot follow: size-independent checks only.
   * The reference's own known-answer check (u = s = 1, v = 0; each REF sweep
     adds the stencil's constant): every interior point within 0.1 of the
     sweep count (iterators.cpp:304-325), the maximum equal to the exact FP32
-    recurrence, and a wrong expectation flags every point."""
+    recurrence, and a wrong expectation flags every point.
+  * 16-bit storage: the tracked maximum of each field equals the largest
+    stored value of its interior, inside the format's range."""
 from fractions import Fraction
 
 import numpy as np
@@ -76,3 +78,31 @@ def test_reference_kat_at_headline_size(gpu, orc):
     want = kat_value(orc, npts, 3)
     assert mx == want
     assert l2 == pytest.approx(want * N ** 1.5, rel=1e-9)
+
+
+def decode16(dtype, q):
+    if dtype == "fp16":
+        return q.view(np.float16).astype(np.float32)
+    return (q.astype(np.uint32) << 16).view(np.float32)
+
+
+@pytest.mark.parametrize("dtype", ["fp16", "bf16"])
+def test_16bit_scale_state_at_headline_size(gpu, dtype):
+    """16-bit storage at 1040^3, four steps: the maximum the device tracks
+    for each field equals the largest |stored value| of its interior (the
+    maximum is taken on the FP32 results and rounded once, which commutes
+    with the monotone rounding), and that value fits the format below 2^15."""
+    R = 4
+    c1 = 0.16
+    c0 = float(np.float32(2.0 - 12.0 * c1))
+    with gpu.Stencil(N, N, N, npts=25, dtype=dtype) as ctx:
+        ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 0)
+        ctx.init_gaussian("v", 1.0, 8.0, 1e-3, 1, 1)
+        ctx.init_point("s", 0, 0, 0, 0.0)
+        ctx.leapfrog(4, c0, c1)
+        for field in ("u", "v"):
+            q, e, mx = ctx.download_raw16(field, R)
+            top = float(np.abs(decode16(dtype, q[R:-R, R:-R, R:-R])).max())
+            del q
+            assert np.float32(mx) == np.float32(top), field
+            assert 0.0 < top < 2.0 ** 15
