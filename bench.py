@@ -195,7 +195,7 @@ def run_other_stencils(S, n, steps, warmup):
                 ctx.timer_start()
                 run(steps)
                 ms, nl = ctx.timer_read()
-                per = ms / steps  # per step (a two-step pass counts twice)
+                per = ms / steps  # per step (a pass counts as its steps)
                 g = float(n) ** 3 / (per * 1e-3) / 1e9
                 row[mode] = {"kernel_ms_per_step": round(per, 4), "gpts": round(g, 2),
                              "hbm_gbs_alg": round(16.0 * g, 1),
@@ -353,7 +353,7 @@ def main():
                 log(f"[bench] {dt}: max-rel {mr:.3e}, L2-rel {lr:.3e}")
 
     if not args.no_other:
-        extra["fp32_by_stencil"] = run_other_stencils(S, n, 20, 4)
+        extra["fp32_by_stencil"] = run_other_stencils(S, n, 24, 6)
     if not args.no_error:
         extra["physics_7pt_128"] = run_physics_7pt(S)
         log(f"[bench] 7-pt 128^3 physics: {extra['physics_7pt_128']}")

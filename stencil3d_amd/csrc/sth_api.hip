@@ -371,59 +371,62 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
   return STH_OK;
 }
 
-// Two-steps-per-pass sweep (FP32 LEAPFROG).  Default for 7 and 13 points
-// on grids of at least 64 output tiles (measured at 1040^3 on MI355X: +31 %
-// and +18 % over the one-step sweep); 25 points keep the one-step sweep, as
-// the 25-point pass (one 16-wave workgroup per CU, 1.37x recomputed first
-// step, 38 LDS reads per output point) stays below it.  Never for the shots
-// of a batch.
+// Several LEAPFROG steps per pass (FP32): the default on grids with enough
+// output tiles to fill the CUs (the thresholds below), three steps per pass
+// for 7 and 13 points, two for 25; otherwise, or when the tuning asks for
+// one step per pass, the one-step sweep.
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (c->fmt != FMT_F32 || c->no_fuse) return false;
-  if (t.steps_per_pass) return t.steps_per_pass == 2;
-  // Measured on cubic grids (FP32 LEAPFROG, one-step time / two-step time):
-  // 7 points 0.98 at 4 tiles, 1.26-2.30 from 9; 13 points 0.71 at 9 tiles,
-  // 1.01-1.64 from 12; 25 points 0.41-0.80 up to 42 tiles, 0.98-1.18 from
-  // 56.  Each chunk also recomputes 2R+1 planes, so short grids stay
-  // one-step.
+  if (t.steps_per_pass) return t.steps_per_pass >= 2;
+  // Measured on cubic grids (FP32 LEAPFROG, one-step time / pass time per
+  // step): 7 points, three-step pass 1.03 at 2 tiles, 1.09-3.25 from 4;
+  // 13 points, three-step pass 0.57-0.94 up to 9 tiles, 1.11-2.61 from 12;
+  // 25 points, two-step pass 0.41-0.80 up to 42 tiles, 0.98-1.18 from 56.
+  // The three-step pass was faster than the two-step pass at every size
+  // (1.11-1.60x).  Each chunk also recomputes (s-1)(2R+1) planes, so short
+  // grids stay one-step.
   const Plan2 p = make_plan2(c, t);
   const int npts = (int)c->g.npts;
-  const int least = npts == 7 ? 8 : npts == 13 ? 12 : 50;
+  const int least = npts == 7 ? 4 : npts == 13 ? 12 : 50;
   return p.ntk * p.ntj >= least && (int)c->g.xs >= 16 * c->R;
 }
 
-// Fused shapes (rows per wave, waves): a tuning that names a compiled shape
-// gets it, otherwise the default of the stencil.
+// Steps per pass (3 for 7 and 13 points unless the tuning asks for 2; 2 for
+// 25 points) and the fused shape (rows per wave, waves): a tuning that
+// names a compiled shape gets it, otherwise the default of the stencil.
 Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   const int npts = (int)c->g.npts;
   Plan2 p;
-  if (fused2_supported(npts, (int)t.rows_per_wave, (int)t.waves)) {
+  p.s = t.steps_per_pass != 2 && npts != 25 ? 3 : 2;
+  if (fused2_supported(npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
     p.mj = (int)t.rows_per_wave;
     p.nw = (int)t.waves;
   } else {
-    p.mj = npts == 25 ? 3 : 4;
+    p.mj = npts == 25 || p.s == 3 ? 3 : 4;
     p.nw = 16;
   }
-  const Fused2Geom g = fused2_geom(npts, p.mj, p.nw);
+  const Fused2Geom g = fused2_geom(npts, p.mj, p.nw, p.s);
   p.oj = g.oj;
   p.ok = g.ok;
   p.ntk = ((int)c->g.zs + p.ok - 1) / p.ok;
   p.ntj = ((int)c->g.ys + p.oj - 1) / p.oj;
   const int xs = (int)c->g.xs, tiles = p.ntk * p.ntj, R = c->R;
-  // A chunk of n planes takes n + 2R + 1 steps and the kernel runs them in
-  // whole groups of 2R + 2: n = 1 mod 2R+2 leaves no idle step (except in
-  // a shorter last chunk).
-  const int ns = 2 * R + 2;
+  // A chunk of n planes takes n + (s-1)(2R+1) steps and the kernel runs
+  // them in whole groups of 2R + 2: n = -(s-1)(2R+1) mod 2R+2 leaves no
+  // idle step (except in a shorter last chunk).
+  const int ns = 2 * R + 2, extra = (p.s - 1) * (2 * R + 1);
   auto chunk_len = [&](int ch) {
     const int n = (xs + ch - 1) / ch;
-    return std::min(xs, n + ((1 - n) % ns + ns) % ns);
+    return std::min(xs, n + ((-extra - n) % ns + ns) % ns);
   };
   int ch = (int)t.chunks;
   if (ch == 0) {
     // One workgroup per CU fits: the pass takes about ceil(workgroups /
-    // CUs) rounds of (chunk length + 2R + 1) steps.  Take the chunk count
-    // with the least such time among those giving at least six rounds (few
-    // long rounds balance badly) and chunks of at least 8R planes (each
-    // chunk recomputes 2R planes of the first step and re-reads 4R of u).
+    // CUs) rounds of (chunk length + (s-1)(2R+1)) steps.  Take the chunk
+    // count with the least such time among those giving at least six rounds
+    // (few long rounds balance badly) and chunks of at least 8R planes (each
+    // chunk recomputes (s-1)2R planes of the first step and re-reads
+    // (s+1)2R of u).
     const int64_t slots = std::max(1, c->cus);
     const int most = std::max(1, std::min(32, xs / (8 * R)));
     const int least = (int)std::min<int64_t>(most, (6 * slots + tiles - 1) / tiles);
@@ -431,7 +434,7 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
     for (int k = std::max(1, least); k <= most; k++) {
       const int n = chunk_len(k), chunks = (xs + n - 1) / n;
       const int64_t rounds = ((int64_t)tiles * chunks + slots - 1) / slots;
-      const int64_t cost = rounds * (n + 2 * R + 1);
+      const int64_t cost = rounds * (n + extra);
       if (best < 0 || cost < best) {
         best = cost;
         ch = k;
@@ -464,8 +467,9 @@ int ensure_twins(sth_ctx *c) {
   return STH_OK;
 }
 
-int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float c1, float c2a,
-              float c2b) {
+// One pass of p.s steps; c2n holds the source coefficients of its steps.
+int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float c1,
+              const float *c2n) {
   const size_t o = c->L.origin * c->eb;
   const int ub = c->ubuf, vb = 1 - c->ubuf;
   Sweep2Args a;
@@ -473,8 +477,16 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
   a.u = (const char *)c->base[ub] + o;
   a.v = (const char *)c->base[vb] + o;
   a.s = (const char *)c->base[2] + o;
-  a.o1 = (char *)c->twin[vb] + o;  // u_{n+1}: the halo of v
-  a.o2 = (char *)c->twin[ub] + o;  // u_{n+2}: the halo of u
+  // The one-step scheme writes u_{n+1} over u_{n-1} (the halo of v),
+  // u_{n+2} over u_n (the halo of u), u_{n+3} over u_{n+1}: the pass writes
+  // the two newest fields into the twins of the buffers they belong to.
+  if (p.s == 2) {
+    a.o1 = (char *)c->twin[vb] + o;
+    a.o2 = (char *)c->twin[ub] + o;
+  } else {
+    a.o1 = (char *)c->twin[ub] + o;
+    a.o2 = (char *)c->twin[vb] + o;
+  }
   a.pitch = (int64_t)c->L.pitch;
   a.plane = (int64_t)c->L.plane;
   a.xs = (int)c->g.xs;
@@ -487,15 +499,17 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
   a.remap = c->tune.xcd_remap ? 1 : 0;
   a.c0 = c0;
   a.c1 = c1;
-  a.c2a = c2a;
-  a.c2b = c2b;
+  a.c2a = c2n ? c2n[0] : 0.0f;
+  a.c2b = c2n ? c2n[1] : 0.0f;
+  a.c2c = c2n && p.s == 3 ? c2n[2] : 0.0f;
   memcpy(a.w, wts.w, sizeof(a.w));
-  const hipError_t e = launch_sweep2((int)c->g.npts, p.mj, p.nw, p.grid, c->stream, a);
+  const hipError_t e = launch_sweep2((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (c->timing) c->launches++;
   std::swap(c->base[0], c->twin[0]);
   std::swap(c->base[1], c->twin[1]);
-  c->slot = (c->slot + 2) % 3;
+  if (p.s == 3) c->ubuf = vb;
+  c->slot = (c->slot + p.s) % 3;
   return STH_OK;
 }
 
@@ -510,11 +524,19 @@ int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, cons
     st = ensure_twins(c);
     if (st == STH_OK) {
       const Plan2 p2 = make_plan2(c, c->tune);
-      for (; n + 2 <= steps; n += 2) {
-        st = run_pass2(c, p2, a, c0, c1, c2 ? c2[n] : 0.0f, c2 ? c2[n + 1] : 0.0f);
+      for (; n + p2.s <= steps; n += p2.s) {
+        st = run_pass2(c, p2, a, c0, c1, c2 ? c2 + n : nullptr);
         if (st) return st;
       }
-    } else if (c->tune.steps_per_pass == 2) {
+      if (n + 2 <= steps) {  // three steps per pass: two left
+        sth_tuning t2 = c->tune;
+        t2.steps_per_pass = 2;
+        t2.rows_per_wave = t2.waves = t2.chunks = 0;
+        st = run_pass2(c, make_plan2(c, t2), a, c0, c1, c2 ? c2 + n : nullptr);
+        if (st) return st;
+        n += 2;
+      }
+    } else if (c->tune.steps_per_pass >= 2) {
       return st;
     }
   }
@@ -1038,10 +1060,13 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 }
 
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
-  if (!c || !t || t->steps_per_pass > 2 || t->launch > 2 || t->chunks > 64) return STH_ERR_ARG;
-  if (t->steps_per_pass == 2 && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
-  const bool fused_shape = t->steps_per_pass == 2 &&
-                           fused2_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves);
+  if (!c || !t || t->steps_per_pass > 3 || t->launch > 2 || t->chunks > 64) return STH_ERR_ARG;
+  if (t->steps_per_pass >= 2 && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (t->steps_per_pass == 3 && c->g.npts == 25)
+    return STH_ERR_UNSUPPORTED;
+  const bool fused_shape =
+      t->steps_per_pass >= 2 && fused2_supported((int)c->g.npts, (int)t->rows_per_wave,
+                                                 (int)t->waves, (int)t->steps_per_pass);
   const TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, *t);
   if (!fused_shape && !tile_supported(ts.mj, ts.nw, ts.vec)) return STH_ERR_ARG;
   c->tune = *t;
@@ -1057,7 +1082,7 @@ int sth_get_tuning(const sth_ctx *c, sth_tuning *t) {
     t->waves = (uint32_t)p.nw;
     t->vec = 1;
     t->chunks = (uint32_t)p.chunks;
-    t->steps_per_pass = 2;
+    t->steps_per_pass = (uint32_t)p.s;
     return STH_OK;
   }
   const Plan p = make_plan(c, c->tune);
@@ -1090,9 +1115,10 @@ bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning 
                &xs, &ys, &zs, &mj, &nw, &vec, &ch, &spp, &m) != 13)
       continue;
     if (ver != 2 || npts != c->g.npts || dt != c->g.dtype || lf != (unsigned)leap ||
-        xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || spp < 1 || spp > 2)
+        xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || spp < 1 || spp > 3)
       continue;
-    const bool ok = spp == 2 ? (c->fmt == FMT_F32 && fused2_supported((int)npts, (int)mj, (int)nw))
+    const bool ok = spp >= 2 ? (c->fmt == FMT_F32 && !c->no_fuse &&
+                                fused2_supported((int)npts, (int)mj, (int)nw, (int)spp))
                              : tile_supported((int)mj, (int)nw, (int)vec);
     if (!ok) continue;
     t.rows_per_wave = mj;
@@ -1139,7 +1165,8 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
                                  {4, 8, 1}, {4, 2, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2},
                                  {2, 8, 2}, {4, 8, 2}};
   static const int chunk_opts[] = {0, 8, 16, 32};
-  static const int fused_shapes[][2] = {{3, 16}, {4, 16}, {4, 12}, {4, 8}, {3, 8}};
+  static const int fused_shapes[][3] = {{3, 16, 2}, {4, 16, 2}, {4, 12, 2}, {4, 8, 2},
+                                        {3, 8, 2},  {3, 16, 3}, {4, 12, 3}};
   static const int fused_chunks[] = {0, 2, 4, 8};
   const sth_tuning saved = c->tune;
   std::vector<sth_tuning> cand;
@@ -1157,14 +1184,14 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
     }
   if (leapfrog && c->fmt == FMT_F32 && !c->no_fuse)
     for (const auto &fs : fused_shapes) {
-      if (!fused2_supported((int)c->g.npts, fs[0], fs[1])) continue;
+      if (!fused2_supported((int)c->g.npts, fs[0], fs[1], fs[2])) continue;
       for (int ch : fused_chunks) {
         sth_tuning t = saved;
         t.rows_per_wave = (uint32_t)fs[0];
         t.waves = (uint32_t)fs[1];
         t.vec = 1;
         t.chunks = (uint32_t)ch;
-        t.steps_per_pass = 2;
+        t.steps_per_pass = (uint32_t)fs[2];
         const Plan2 p = make_plan2(c, t);
         if (ch != 0 && p.chunks != ch) continue;
         cand.push_back(t);
@@ -1174,12 +1201,13 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   c->timing = false;
   sth_tuning bt = saved;
   float bms = 1e30f;
-  // steps per timed run: even, so two-step passes run whole
-  const uint32_t nsteps = 2 * reps;
+  // steps per timed run: 2 reps rounded up to a multiple of 6, so two- and
+  // three-step passes run whole
+  const uint32_t nsteps = (2 * reps + 5) / 6 * 6;
   for (const sth_tuning &t : cand) {
     c->tune = t;
     const Plan p = make_plan(c, t);
-    int st = leapfrog ? run_leap(c, p, 2, 1.0f, 0.0f, nullptr) : run_ref(c, p, 1);
+    int st = leapfrog ? run_leap(c, p, 6, 1.0f, 0.0f, nullptr) : run_ref(c, p, 1);
     if (st) { c->tune = saved; c->timing = saved_timing; return st; }
     STH_TRY(hipEventRecord(c->ev0, c->stream));
     st = leapfrog ? run_leap(c, p, nsteps, 1.0f, 0.0f, nullptr) : run_ref(c, p, nsteps);
@@ -1192,7 +1220,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
     if (ms < bms) {
       bms = ms;
       bt = t;
-      bt.chunks = (uint32_t)(t.steps_per_pass == 2 ? make_plan2(c, t).chunks : p.chunks);
+      bt.chunks = (uint32_t)(t.steps_per_pass >= 2 ? make_plan2(c, t).chunks : p.chunks);
     }
   }
   c->tune = bt;

@@ -90,9 +90,9 @@ struct Plan {
   unsigned grid;
 };
 
-// Launch plan of a two-steps-per-pass sweep.
+// Launch plan of a pass of s (2 or 3) LEAPFROG steps.
 struct Plan2 {
-  int mj, nw, oj, ok, chunks, chunk_len, ntk, ntj;
+  int s, mj, nw, oj, ok, chunks, chunk_len, ntk, ntj;
   unsigned grid;
 };
 

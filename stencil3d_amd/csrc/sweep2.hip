@@ -4,32 +4,35 @@
 
 namespace sth {
 
-#define STH_FUSED2_LIST(X) \
-  X(7, 4, 16) X(7, 4, 8) X(13, 4, 16) X(13, 3, 16) X(13, 4, 8) X(25, 3, 16) X(25, 4, 12) X(25, 3, 8) X(25, 4, 8)
+// (stencil, rows per wave, waves, steps per pass)
+#define STH_FUSED2_LIST(X)                                                                \
+  X(7, 4, 16, 2) X(7, 4, 8, 2) X(13, 4, 16, 2) X(13, 3, 16, 2) X(13, 4, 8, 2) X(25, 3, 16, 2) \
+  X(25, 4, 12, 2) X(25, 3, 8, 2) X(25, 4, 8, 2) X(7, 3, 16, 3) X(7, 4, 12, 3) \
+  X(13, 3, 16, 3) X(13, 4, 12, 3)
 
-bool fused2_supported(int npts, int mj, int nw) {
-#define STH_F2_SUP(N, MJ, NW) \
-  if (npts == N && mj == MJ && nw == NW) return true;
+bool fused2_supported(int npts, int mj, int nw, int steps) {
+#define STH_F2_SUP(N, MJ, NW, S) \
+  if (npts == N && mj == MJ && nw == NW && steps == S) return true;
   STH_FUSED2_LIST(STH_F2_SUP)
 #undef STH_F2_SUP
   return false;
 }
 
-Fused2Geom fused2_geom(int npts, int mj, int nw) {
-#define STH_F2_GEOM(N, MJ, NW)                                         \
-  if (npts == N && mj == MJ && nw == NW)                               \
-    return {Fused2Shape<N, MJ, NW>::OJ, Fused2Shape<N, MJ, NW>::OK, 64 * NW};
+Fused2Geom fused2_geom(int npts, int mj, int nw, int steps) {
+#define STH_F2_GEOM(N, MJ, NW, S)                                     \
+  if (npts == N && mj == MJ && nw == NW && steps == S)                \
+    return {Fused2Shape<N, MJ, NW, S>::OJ, Fused2Shape<N, MJ, NW, S>::OK, 64 * NW};
   STH_FUSED2_LIST(STH_F2_GEOM)
 #undef STH_F2_GEOM
   return {0, 0, 0};
 }
 
-hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                          const Sweep2Args &a) {
-#define STH_F2_LAUNCH(N, MJ, NW)                                                  \
-  if (npts == N && mj == MJ && nw == NW) {                                        \
-    hipLaunchKernelGGL((sweep2_kernel<N, MJ, NW>), dim3(grid), dim3(64 * NW), 0, st, a); \
-    return hipGetLastError();                                                     \
+#define STH_F2_LAUNCH(N, MJ, NW, S)                                                   \
+  if (npts == N && mj == MJ && nw == NW && steps == S) {                              \
+    hipLaunchKernelGGL((sweep2_kernel<N, MJ, NW, S>), dim3(grid), dim3(64 * NW), 0, st, a); \
+    return hipGetLastError();                                                         \
   }
   STH_FUSED2_LIST(STH_F2_LAUNCH)
 #undef STH_F2_LAUNCH

@@ -8,14 +8,15 @@ namespace sth {
 struct Sweep2Args;
 struct Batch2Args;
 
-// Output tile of a compiled (stencil, rows per wave, waves) shape.
+// Output tile of a compiled (stencil, rows per wave, waves, steps per pass)
+// shape.
 struct Fused2Geom {
   int oj, ok, threads;
 };
 
-bool fused2_supported(int npts, int mj, int nw);
-Fused2Geom fused2_geom(int npts, int mj, int nw);
-hipError_t launch_sweep2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+bool fused2_supported(int npts, int mj, int nw, int steps);
+Fused2Geom fused2_geom(int npts, int mj, int nw, int steps);
+hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                          const Sweep2Args &a);
 // The batch form (persistent, items of many shots from a queue) exists for
 // each stencil's default shape.

@@ -40,13 +40,16 @@ struct Sweep2Args {
   int xs, ys, zs;
   int ntk, ntj, chunks, chunk_len, remap;
   float c0, c1;
-  float c2a, c2b;  // source coefficients of steps n and n+1
+  float c2a, c2b, c2c;  // source coefficients of steps n, n+1, n+2
   float w[12];     // LEAPFROG weights (2 * stencil weight), evaluation order
 };
 
-template <int NPTS, int MJ, int NW>
+// S steps per pass (2 or 3): step one runs on the output tile widened by
+// WD = (S-1)R on every side.
+template <int NPTS, int MJ, int NW, int S = 2>
 struct Fused2Shape {
   static constexpr int R = Terms<NPTS>::R;
+  static constexpr int WD = (S - 1) * R;
   static constexpr int EJ = NW * MJ, EK = 64;  // step-one tile
   // Output tile.  Its rows are 48 points (192 B): every tile writes whole
   // 64-byte segments that no other tile touches.  Rows of 64 - 2R points
@@ -54,12 +57,13 @@ struct Fused2Shape {
   // cost 10-30 % (R <= 2) and 15 % (R = 4, although 48-point rows need 16 %
   // more tiles) and made the time depend on where the allocator placed the
   // fields.
-  static constexpr int OJ = EJ - 2 * R;
-  static constexpr int OK = ((EK - 2 * R) / 16) * 16;
+  static constexpr int OJ = EJ - 2 * WD;
+  static constexpr int OK = ((EK - 2 * WD) / 16) * 16;
   static constexpr int LW = EK + 2 * R;  // u plane + R halo
   static constexpr int LU = (EJ + 2 * R) * LW;
-  static constexpr int LR = EJ * EK;  // one plane of u_{n+1} (or s) on E1
-  static constexpr int LDS_BYTES = (2 * LU + 2 * LR + (R + 1) * LR) * 4;
+  static constexpr int LR = EJ * EK;  // one plane of u_{n+1}, u_{n+2} or s on E1
+  static constexpr int SR = S == 2 ? R + 1 : 2 * R + 2;  // s ring slots
+  static constexpr int LDS_BYTES = (2 * LU + (2 * (S - 1) + SR) * LR) * 4;
 };
 
 // Calls f(integral_constant<int, 0>) .. f(integral_constant<int, N-1>).
@@ -75,22 +79,30 @@ __device__ __forceinline__ void static_for(Fn &&f) {
 // One work item of a pass: output tile (tk, tj), i-chunk ch.  Called by the
 // single-grid kernel (one item per workgroup) and the batch kernel (items
 // of many grids drawn from a queue; LDS is reused between items).
-template <int NPTS, int MJ, int NW>
+//
+// S = 3 adds a third step: step two runs on the tile widened by R (E2) at
+// plane i = p-R-1, its u_{n+2} values go to a third register queue and, for
+// the plane i2 = i-R-1 of step three, to LDS; step three writes u_{n+2} and
+// u_{n+3} of the output tile at plane i2.
+template <int NPTS, int MJ, int NW, int S = 2>
 __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, const int tj,
                                             const int ch) {
   using Tm = Terms<NPTS>;
-  using F = Fused2Shape<NPTS, MJ, NW>;
+  using F = Fused2Shape<NPTS, MJ, NW, S>;
   constexpr int R = Tm::R, T = Tm::T;
-  constexpr int EJ = F::EJ, LW = F::LW, OJ = F::OJ, OK = F::OK;
+  constexpr int EJ = F::EJ, LW = F::LW, OJ = F::OJ, OK = F::OK, WD = F::WD, SR = F::SR;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // u halo rows per wave
   constexpr int Q = 2 * R + 2;               // u queue: planes p-R-1 .. p+R
+  constexpr int WQ = S == 2 ? 2 * R + 2 : 2 * R + 3;  // u_{n+1} queue (+ the newest)
+  constexpr int WC = WQ - R - 2;             // u_{n+1} queue entry of plane i
   constexpr int U = Q;                       // steps per loop iteration
+  static_assert(S == 2 || S == 3, "two or three steps per pass");
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
-  static_assert(EJ * 2 * R % 64 == 0, "k-halo entries fill whole waves");
   static_assert(F::LDS_BYTES <= 160 * 1024, "LDS");
-  // One LDS array: the two u planes (LU), the two u_{n+1} planes (W), then
-  // R+1 planes of s, each lane's own values (step two needs s(p-R-1)).
-  __shared__ float SM[2 * F::LU + (R + 3) * F::LR];
+  // One LDS array: the two u planes (LU), the two u_{n+1} planes (W; S = 3:
+  // then the two u_{n+2} planes), then SR planes of s, each lane's own
+  // values (step two needs s(p-R-1), step three s(p-2R-2)).
+  __shared__ float SM[2 * F::LU + (2 * (S - 1) + SR) * F::LR];
 
   const int ib = ch * a.chunk_len;
   const int ie = min(a.xs, ib + a.chunk_len);
@@ -101,7 +113,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // compiles to a branch rather than to exec-mask updates.
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tid = threadIdx.x;
-  const int je = tj * OJ - R, ke = tk * OK - R;  // E1 origin (global j, k)
+  const int je = tj * OJ - WD, ke = tk * OK - WD;  // E1 origin (global j, k)
   const int64_t P = a.plane, Pj = a.pitch;
 
   // Byte offsets from the corner (i, -R, -R) of a plane.  Reads clamp to the
@@ -136,17 +148,18 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // row is (rowout, uniform) and its column is (kout, per lane).
   const int kg = ke + lane;
   const bool kin = kg >= 0 && kg < a.zs;
-  const bool kout = kin && lane >= R && lane < R + OK;
+  const bool kout = kin && lane >= WD && lane < WD + OK;
   uint32_t off[MJ];
   bool inner[MJ];
-  int rowout[MJ];
+  int rowout[MJ], row2[MJ];  // row2: S = 3, the row lies in E2
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int er = wv * MJ + m, jg = je + er;
     const bool jin = jg >= 0 && jg < a.ys;
     off[m] = boff(jg, kg);
     inner[m] = jin && kin;
-    rowout[m] = __builtin_amdgcn_readfirstlane((int)(jin && er >= R && er < R + OJ));
+    rowout[m] = __builtin_amdgcn_readfirstlane((int)(jin && er >= WD && er < WD + OJ));
+    row2[m] = __builtin_amdgcn_readfirstlane((int)(er >= R && er < EJ - R));
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
@@ -157,7 +170,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     hrow[h] = hh < 2 * R ? (hh < R ? hh : EJ + hh) : -1;
     hoff[h] = hrow[h] >= 0 ? boff(je + trow, kg) : kNone;
   }
-  const bool has_kh = wv < EJ * 2 * R / 64;  // the first EJ*2R threads
+  // the first EJ*2R threads (a wave-uniform condition when they fill whole
+  // waves)
+  const bool has_kh = EJ * 2 * R % 64 == 0 ? wv < EJ * 2 * R / 64 : tid < EJ * 2 * R;
   uint32_t khoff = kNone;
   int khpos = 0;
   if (has_kh) {
@@ -178,11 +193,11 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   //   lbS: s slot 0 at E1 row wv*MJ, column lane (offsets: slot, row m).
   using LF = __attribute__((address_space(3))) float;
   constexpr uint32_t LUB = F::LU * 4, LRB = F::LR * 4;
-  static_assert(2 * LUB <= 65536 && 2 * LRB + (2 * R + MJ) * 256 <= 65536 &&
-                    R * LRB + MJ * 256 <= 65536,
+  static_assert(2 * LUB <= 65536 && 2 * (S - 1) * LRB + (2 * R + MJ) * 256 <= 65536 &&
+                    (SR - 1) * LRB + MJ * 256 <= 65536,
                 "offsets");
   const uint32_t lu0 = (uint32_t)(uintptr_t)(LF *)&SM[0];
-  const uint32_t wbase = lu0 + 2 * LUB, sbase = wbase + 2 * LRB;
+  const uint32_t wbase = lu0 + 2 * LUB, sbase = wbase + 2 * (S - 1) * LRB;
   uint32_t lbL = lu0 + (uint32_t)((wv * MJ * LW + lane) * 4);
   uint32_t lbW = wbase + (uint32_t)(((wv * MJ - R) * 64 + lane - R) * 4);
   uint32_t lbS = sbase + (uint32_t)((wv * MJ * 64 + lane) * 4);
@@ -194,11 +209,15 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 
   // Register queues of the lane's points:
   //   q[m][t]  = u_n(p - R - 1 + t),      t = 0 .. 2R+1;
-  //   wq[m][t] = u_{n+1}(p - 2R - 1 + t), t = 0 .. 2R (t = 2R+1 takes step
-  //              one's result at plane p before the shift).
-  float q[MJ][Q], wq[MJ][2 * R + 2];
+  //   wq[m][t] = u_{n+1}(p - WQ + 1 + t), t = 0 .. WQ-2 (t = WQ-1 takes step
+  //              one's result at plane p before the shift);
+  //   zq[m][t] = u_{n+2}(i - 2R - 1 + t), t = 0 .. 2R (S = 3; t = 2R+1 takes
+  //              step two's result at plane i).
+  float q[MJ][Q], wq[MJ][WQ], zq[MJ][S == 3 ? 2 * R + 2 : 1];
   float hq[HJ][R + 1], kq[R + 1];
-  const int p0 = ib - R, pe = ie + R;  // step one at p0 .. pe-1, step two at p0+R+1 .. pe
+  // Step one from p0 (its planes past ie-1+WD are never used), the last
+  // step (output plane p - (S-1)(R+1)) up to plast.
+  const int p0 = ib - WD, plast = ie + (S - 1) * (R + 1) - 1;
 #pragma unroll
   for (int t = 1; t <= 2 * R; t++) {
     GB *Up = pbase(a.u, p0 - R - 1 + t);
@@ -209,7 +228,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   for (int m = 0; m < MJ; m++) {
     q[m][0] = 0.0f;
 #pragma unroll
-    for (int t = 0; t < 2 * R + 2; t++) wq[m][t] = 0.0f;
+    for (int t = 0; t < WQ; t++) wq[m][t] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < (S == 3 ? 2 * R + 2 : 1); t++) zq[m][t] = 0.0f;
   }
 #pragma unroll
   for (int t = 0; t < R; t++) {
@@ -243,8 +264,8 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   prefetch(pbase(a.u, p0 + 1 + R), pbase(a.v, p0 + 1), pbase(a.s, p0 + 1), SB);
 
   // Plane pointers advanced by one plane per step, each held inside
-  // [-R, xs+R): the prefetch of step p reads u(p+2+R), v(p+2), s(p+2); step
-  // two stores plane p-R-1, from plane ib on.
+  // [-R, xs+R): the prefetch of step p reads u(p+2+R), v(p+2), s(p+2); the
+  // last step stores plane p-(S-1)(R+1), from plane ib on.
   const int64_t PB = P * 4;
   const int last = a.xs + R - 1;
   GB *Un = pbase(a.u, p0 + 2 + R);
@@ -255,12 +276,12 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 
   // Step of plane p = p0 + U*it + kc: the LDS buffers and the staging set
   // alternate with kc and every queue position is a compile-time constant.
-  // The loop runs whole iterations of U steps; steps past pe compute values
-  // nobody stores (their loads stay on the last planes), so no register
-  // state depends on a skipped step.
+  // The loop runs whole iterations of U steps; steps past plast compute
+  // values nobody stores (their loads stay on the last planes), so no
+  // register state depends on a skipped step.
   auto step = [&](int p, Staged &n, auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
-    float vv[MJ], s1[MJ], s2c[MJ];
+    float vv[MJ], s1[MJ], s2c[MJ], s3c[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       q[m][Q - 1] = n.u[m];
@@ -276,16 +297,25 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     Sn = p + 3 <= last ? Sn + PB : Sn;
 
     asm volatile("" : "+v"(lbL), "+v"(lbW), "+v"(lbS));
-    // s(p-R-1) and s(p) share slot k mod (R+1): read the old value, then
-    // store the new one (the lane's own word, so no barrier is involved).
-    constexpr uint32_t SO = (k % (R + 1)) * LRB;
+    // The s ring (the lane's own words, so no barrier is involved): S = 2,
+    // s(p-R-1) and s(p) share slot k mod (R+1); S = 3, s(p-2R-2) and s(p)
+    // share slot k and s(p-R-1) is in slot k+R+1 (mod 2R+2).  Old values are
+    // read before the new one is stored.
+    constexpr uint32_t SO = (k % SR) * LRB;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      s2c[m] = lds(lbS + SO + m * 256);
+      if constexpr (S == 2) {
+        s2c[m] = lds(lbS + SO + m * 256);
+      } else {
+        constexpr uint32_t SO2 = ((k + R + 1) % SR) * LRB;
+        s3c[m] = lds(lbS + SO + m * 256);
+        s2c[m] = lds(lbS + SO2 + m * 256);
+      }
       lds(lbS + SO + m * 256) = s1[m];
     }
     constexpr uint32_t LO = (k & 1) * LUB;  // this step's u buffer
     constexpr uint32_t WO = (k & 1) * LRB;  // this step's u_{n+1} buffer
+    constexpr uint32_t ZO = 2 * LRB + (k & 1) * LRB;  // S = 3: its u_{n+2} buffer
     // LU element (row m+R+dr, column R+dc of this lane's rows)
     auto Lat = [&](int m, int dr, int dc) -> LF & {
       return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + R + dc) * 4));
@@ -294,13 +324,17 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     auto Wat = [&](int m, int dr, int dc) -> LF & {
       return lds(lbW + WO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
     };
+    auto Zat = [&](int m, int dr, int dc) -> LF & {
+      return lds(lbW + ZO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
+    };
 
     // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns;
-    // u_{n+1}(p-R-1) of the E1 rows.
+    // u_{n+1}(p-R-1) of the E1 rows (S = 3: and u_{n+2}(p-2R-2)).
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       Lat(m, 0, 0) = q[m][R + 1];
-      Wat(m, 0, 0) = wq[m][R];
+      Wat(m, 0, 0) = wq[m][WC];
+      if constexpr (S == 3) Zat(m, 0, 0) = zq[m][R];
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++)
@@ -308,30 +342,86 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     if (has_kh) lds(kadr + LO) = kq[0];
     __syncthreads();
 
-    // Step two at plane i = p-R-1: u_{n+2} on the output tile.  Rows
-    // without output points (a wave-uniform condition) are skipped.
+    // The last step writes plane io = p-(S-1)(R+1) of the output tile: the
+    // two newest fields there.  Rows without output points (a wave-uniform
+    // condition) are skipped.
     const int i = p - R - 1;
-    const bool store = i >= ib && i < ie;
+    const int io = p - (S - 1) * (R + 1);
+    const bool store = io >= ib && io < ie;
     const BufRes Wr = buf_res(Wo), Zr = buf_res(Zo);
+    if constexpr (S == 2) {
+      // Step two at plane i: u_{n+2}.
 #pragma unroll
-    for (int m = 0; m < MJ; m++) {
-      if (!rowout[m]) continue;
-      auto pair = [&](int ax, int r) -> float {
-        if (ax == 0) return wq[m][R - r] + wq[m][R + r];
-        if (ax == 2) return Wat(m, 0, -r) + Wat(m, 0, r);
-        const float lo = (m - r >= 0) ? wq[(m - r) >= 0 ? m - r : 0][R] : Wat(m, -r, 0);
-        const float hi = (m + r < MJ) ? wq[(m + r) < MJ ? m + r : 0][R] : Wat(m, r, 0);
-        return lo + hi;
-      };
-      float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+      for (int m = 0; m < MJ; m++) {
+        if (!rowout[m]) continue;
+        auto pair = [&](int ax, int r) -> float {
+          if (ax == 0) return wq[m][WC - r] + wq[m][WC + r];
+          if (ax == 2) return Wat(m, 0, -r) + Wat(m, 0, r);
+          const float lo = (m - r >= 0) ? wq[(m - r) >= 0 ? m - r : 0][WC] : Wat(m, -r, 0);
+          const float hi = (m + r < MJ) ? wq[(m + r) < MJ ? m + r : 0][WC] : Wat(m, r, 0);
+          return lo + hi;
+        };
+        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
-      for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-      const float wc = wq[m][R];
-      const float z =
-          __builtin_fmaf(a.c2b, s2c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
-      if (store && kout) {
-        buf_store<float, 2>(wc, Wr, off[m]);
-        buf_store<float, 2>(z, Zr, off[m]);
+        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        const float wc = wq[m][WC];
+        const float z = __builtin_fmaf(a.c2b, s2c[m],
+                                       __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
+        if (store && kout) {
+          buf_store<float, 2>(wc, Wr, off[m]);
+          buf_store<float, 2>(z, Zr, off[m]);
+        }
+      }
+    } else {
+      // Step three at plane io = i-R-1: u_{n+3} on the output tile from the
+      // u_{n+2} queue (i neighbours, in-wave j neighbours) and LDS; the
+      // -u_{n+1} term is the oldest u_{n+1} entry.
+#pragma unroll
+      for (int m = 0; m < MJ; m++) {
+        if (!rowout[m]) continue;
+        auto pair = [&](int ax, int r) -> float {
+          if (ax == 0) return zq[m][R - r] + zq[m][R + r];
+          if (ax == 2) return Zat(m, 0, -r) + Zat(m, 0, r);
+          const float lo = (m - r >= 0) ? zq[(m - r) >= 0 ? m - r : 0][R] : Zat(m, -r, 0);
+          const float hi = (m + r < MJ) ? zq[(m + r) < MJ ? m + r : 0][R] : Zat(m, r, 0);
+          return lo + hi;
+        };
+        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+#pragma unroll
+        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        const float zc = zq[m][R];
+        const float z3 = __builtin_fmaf(
+            a.c2c, s3c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, zc, -wq[m][0])));
+        if (store && kout) {
+          buf_store<float, 2>(zc, Wr, off[m]);
+          buf_store<float, 2>(z3, Zr, off[m]);
+        }
+      }
+      // Step two at plane i: u_{n+2} on E2, into the newest u_{n+2} entry.
+      // Outside the interior u_{n+2} takes u_n's value there (the one-step
+      // scheme writes u_{n+2} into u_n's buffer, whose halo no sweep
+      // writes).
+      const bool pin2 = i >= 0 && i < a.xs;
+#pragma unroll
+      for (int m = 0; m < MJ; m++) {
+        if (!row2[m]) {
+          zq[m][2 * R + 1] = q[m][0];
+          continue;
+        }
+        auto pair = [&](int ax, int r) -> float {
+          if (ax == 0) return wq[m][WC - r] + wq[m][WC + r];
+          if (ax == 2) return Wat(m, 0, -r) + Wat(m, 0, r);
+          const float lo = (m - r >= 0) ? wq[(m - r) >= 0 ? m - r : 0][WC] : Wat(m, -r, 0);
+          const float hi = (m + r < MJ) ? wq[(m + r) < MJ ? m + r : 0][WC] : Wat(m, r, 0);
+          return lo + hi;
+        };
+        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+#pragma unroll
+        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        const float wc = wq[m][WC];
+        const float z2 = __builtin_fmaf(a.c2b, s2c[m],
+                                        __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
+        zq[m][2 * R + 1] = (inner[m] && pin2) ? z2 : q[m][0];
       }
     }
     if (store) {
@@ -356,7 +446,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
       const float u0 = q[m][R + 1];
       const float w1 =
           __builtin_fmaf(a.c2a, s1[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, u0, -vv[m])));
-      wq[m][2 * R + 1] = (inner[m] && pin) ? w1 : vv[m];
+      wq[m][WQ - 1] = (inner[m] && pin) ? w1 : vv[m];
     }
 
 #pragma unroll
@@ -364,7 +454,11 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 #pragma unroll
       for (int t = 0; t < Q - 1; t++) q[m][t] = q[m][t + 1];
 #pragma unroll
-      for (int t = 0; t < 2 * R + 1; t++) wq[m][t] = wq[m][t + 1];
+      for (int t = 0; t < WQ - 1; t++) wq[m][t] = wq[m][t + 1];
+      if constexpr (S == 3) {
+#pragma unroll
+        for (int t = 0; t < 2 * R + 1; t++) zq[m][t] = zq[m][t + 1];
+      }
     }
 #pragma unroll
     for (int t = 0; t < R; t++) {
@@ -373,7 +467,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
       kq[t] = kq[t + 1];
     }
   };
-  for (int p = p0; p <= pe; p += U) {
+  for (int p = p0; p <= plast; p += U) {
     static_for<U>([&](auto kc) __attribute__((always_inline)) {
       constexpr int k = decltype(kc)::value;
       step(p + k, (k & 1) ? SB : SA, kc);
@@ -381,7 +475,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   }
 }
 
-template <int NPTS, int MJ, int NW>
+template <int NPTS, int MJ, int NW, int S>
 __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   // Work item -> (chunk, tile): k fastest, then j, then the chunk.  (Tile
   // orders that group the workgroups of an XCD into compact 2D patches, so
@@ -390,7 +484,7 @@ __global__ __launch_bounds__(64 * NW) void sweep2_kernel(const Sweep2Args a) {
   if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
   const int tk = bid % a.ntk;
   const int tt = bid / a.ntk;
-  sweep2_tile<NPTS, MJ, NW>(a, tk, tt % a.ntj, tt / a.ntj);
+  sweep2_tile<NPTS, MJ, NW, S>(a, tk, tt % a.ntj, tt / a.ntj);
 }
 
 // ---- many grids ("shots") in one launch per pass ----------------------------
@@ -455,9 +549,10 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.c1 = b.c1;
     a.c2a = b.c2a;
     a.c2b = b.c2b;
+    a.c2c = 0.0f;
 #pragma unroll
     for (int t = 0; t < 12; t++) a.w[t] = b.w[t];
-    sweep2_tile<NPTS, MJ, NW>(a, it.tk, it.tj, it.ch);
+    sweep2_tile<NPTS, MJ, NW, 2>(a, it.tk, it.tj, it.ch);
     __syncthreads();  // LDS is reused by the next item
   }
 }

@@ -38,18 +38,18 @@ def test_frontend_leapfrog_fp16(gpu):
 
 
 def test_frontend_leapfrog_steps_per_pass_and_launch(gpu):
-    """FP32 LEAPFROG from the CLI with one or two steps per pass and either
-    launch form: the printed norms agree to the last digit."""
+    """FP32 LEAPFROG from the CLI with one, two or three steps per pass and
+    either launch form: the printed norms agree to the last digit."""
     outs = []
     for extra in (["--steps-per-pass", "1", "--launch", "1"], ["--steps-per-pass", "2"],
-                  ["--steps-per-pass", "1", "--launch", "2"]):
+                  ["--steps-per-pass", "1", "--launch", "2"], ["--steps-per-pass", "3"]):
         r = run_fe(gpu, "--xs", "40", "--ys", "56", "--zs", "150", "--iter", "6", "--stencil",
                    "13", "--mode", "leapfrog", "--init", "gauss", "--c1", "0.2", *extra)
         assert r.returncode == 0, r.stderr
         m = re.search(r"norms: max \|u\| = ([0-9.e+-]+).*", r.stdout)
         assert m
         outs.append(m.group(0))
-    assert outs[0] == outs[1] == outs[2]
+    assert outs[0] == outs[1] == outs[2] == outs[3]
 
 
 def test_frontend_autotune(gpu):

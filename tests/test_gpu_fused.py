@@ -4,7 +4,8 @@ oracle, halos included: the fields carry nonzero halo values, which the
 one-step scheme never writes and which a pass must leave where the one-step
 scheme leaves them.  Covered: every compiled fused shape, i-chunks, odd step
 counts (the last step runs one-step), one-step calls and uploads between
-calls, and shapes that are not multiples of the output tile."""
+calls, and shapes that are not multiples of the output tile; the same for
+three steps per pass (7 and 13 points)."""
 import numpy as np
 import pytest
 
@@ -117,33 +118,132 @@ def test_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
     assert same_bits(out[0][1], out[1][1])
 
 
+FUSED3 = {7: [(3, 16), (4, 12)], 13: [(3, 16), (4, 12)]}
+
+
+@pytest.mark.parametrize("npts", [7, 13])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_three_step_passes_bit_exact(gpu, orc, npts, shape):
+    """Three steps per pass: 6 steps are two passes, 7 add a one-step sweep,
+    8 add a two-step pass; every compiled shape, with i-chunks."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 5, 0.375)
+    c0, c1 = coeffs(orc, npts)
+    c2 = (np.arange(1, 9, dtype=np.float32) * np.float32(-7e-4)).astype(np.float32)
+    want = {n: oracle_run(orc, npts, g, u, v, s, c0, c1, c2[:n], n) for n in (6, 7, 8)}
+    for mj, nw in FUSED3[npts]:
+        for chunks, steps in ((0, 6), (2, 7), (3, 8)):
+            with gpu.Stencil(*shape, npts=npts) as ctx:
+                ctx.set_tuning(mj, nw, chunks, steps_per_pass=3)
+                t = ctx.get_tuning()
+                assert t["steps_per_pass"] == 3 and (t["rows_per_wave"], t["waves"]) == (mj, nw)
+                ctx.upload("u", u, R)
+                ctx.upload("v", v, R)
+                ctx.upload("s", s, R)
+                ctx.timer_start()
+                ctx.leapfrog(steps, c0, c1, c2[:steps])
+                _, launches = ctx.timer_read()
+                assert launches == (2 if steps == 6 else 3)
+                gu = ctx.download("u", R)
+                gv = ctx.download("v", R)
+            assert same_bits(gu, want[steps][0]), (mj, nw, chunks, steps)
+            assert same_bits(gv, want[steps][1]), (mj, nw, chunks, steps)
+
+
+@pytest.mark.parametrize("npts", [7, 13])
+def test_three_step_passes_mixed_with_other_schedules(gpu, orc, npts):
+    """A three-step pass, a two-step pass, one one-step sweep, a new v, then
+    four steps of three-step tuning (a pass and a sweep): the buffer roles
+    after an odd pass and the twins' halos must follow."""
+    shape = (26, 31, 67)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 13, 0.25)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(1e-3, 4e-3, 4).astype(np.float32)
+    a, b = oracle_run(orc, npts, g, u, v, s, c0, c1, c2[:3], 3)
+    a, b = oracle_run(orc, npts, g, a, b, s, c0, c1, c2[:2], 2)
+    a, b = oracle_run(orc, npts, g, a, b, s, c0, c1, c2[:1], 1)
+    v2 = orc.fill_uniform(g, 14, 1, halo=0.625)
+    want_u, want_v = oracle_run(orc, npts, g, a, v2, s, c0, c1, c2, 4)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.set_tuning(steps_per_pass=3)
+        ctx.leapfrog(3, c0, c1, c2[:3])
+        ctx.set_tuning(steps_per_pass=2)
+        ctx.leapfrog(2, c0, c1, c2[:2])
+        ctx.set_tuning(steps_per_pass=1)
+        ctx.leapfrog(1, c0, c1, c2[:1])
+        ctx.set_tuning(steps_per_pass=3)
+        ctx.upload("v", v2, R)
+        ctx.leapfrog(4, c0, c1, c2)
+        gu = ctx.download("u", R)
+        gv = ctx.download("v", R)
+    assert same_bits(gu, want_u)
+    assert same_bits(gv, want_v)
+
+
+def test_three_step_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
+    """Several tiles and chunks in every direction: 9 steps as three-step
+    passes and as one-step sweeps give the same bits."""
+    shape = (96, 100, 200)
+    npts, R = 13, 2
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-3, 1e-3, 9).astype(np.float32)
+    out = []
+    for spp in (1, 3):
+        with gpu.Stencil(*shape, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            ctx.init_uniform("u", 6, 0, -1.0, 1.0, 0.5)
+            ctx.init_uniform("v", 6, 1, -1.0, 1.0, -0.5)
+            ctx.init_uniform("s", 6, 2, -1.0, 1.0, 0.25)
+            ctx.leapfrog(9, c0, c1, c2)
+            out.append((ctx.download("u", R), ctx.download("v", R)))
+    assert same_bits(out[0][0], out[1][0])
+    assert same_bits(out[0][1], out[1][1])
+
+
+def test_three_steps_rejected_for_25_points_and_16bit(gpu):
+    with gpu.Stencil(16, 16, 64, npts=25) as ctx:
+        with pytest.raises(gpu.SthError):
+            ctx.set_tuning(steps_per_pass=3)
+    with gpu.Stencil(16, 16, 64, npts=13, dtype="bf16") as ctx:
+        with pytest.raises(gpu.SthError):
+            ctx.set_tuning(steps_per_pass=3)
+
+
 def test_two_steps_rejected_for_16bit(gpu):
     with gpu.Stencil(16, 16, 64, npts=25, dtype="fp16") as ctx:
         with pytest.raises(gpu.SthError):
             ctx.set_tuning(steps_per_pass=2)
 
 
-@pytest.mark.parametrize("npts,n,fused", [(7, 96, False), (7, 128, True), (13, 128, False),
-                                          (13, 160, True), (25, 256, False), (25, 320, True)])
-def test_default_schedule_by_grid_size(gpu, orc, npts, n, fused):
-    """The default FP32 LEAPFROG schedule takes two steps per pass where that
-    measured faster (enough output tiles for the stencil): four steps are two
-    passes or four sweeps, and either way bit-identical to the oracle."""
+@pytest.mark.parametrize("npts,n,spp", [(7, 48, 1), (7, 64, 3), (13, 112, 1), (13, 128, 3),
+                                         (25, 256, 1), (25, 320, 2)])
+def test_default_schedule_by_grid_size(gpu, orc, npts, n, spp):
+    """The default FP32 LEAPFROG schedule takes several steps per pass where
+    that measured faster (enough output tiles for the stencil; three steps
+    for 7 and 13 points, two for 25): six steps are two or three passes or
+    six sweeps, and either way bit-identical to the oracle."""
     R = orc.radius(npts)
     shape = (n, n, n)
     g = orc.Grid(*shape, R, R, R)
     u, v, s = seeded(orc, g, 23, 0.125)
     c0, c1 = coeffs(orc, npts)
-    c2 = np.full(4, 1e-3, np.float32)
+    c2 = np.full(6, 1e-3, np.float32)
     with gpu.Stencil(*shape, npts=npts) as ctx:
+        assert ctx.get_tuning()["steps_per_pass"] == spp
         ctx.upload("u", u, R)
         ctx.upload("v", v, R)
         ctx.upload("s", s, R)
         ctx.timer_start()
-        ctx.leapfrog(4, c0, c1, c2)
+        ctx.leapfrog(6, c0, c1, c2)
         _, launches = ctx.timer_read()
         gu = ctx.download("u", R)
-    assert launches == (2 if fused else 4)
+    assert launches == 6 // spp
     if n <= 160:  # the oracle follows these sizes in seconds
-        want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 4)
+        want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 6)
         assert same_bits(gu, want_u)

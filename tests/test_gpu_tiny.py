@@ -1,6 +1,7 @@
 """Degenerate and tiny grids through every path: one-point rows, one-row
 planes, one plane, widths below one wave, for REF and LEAPFROG in FP32, the
-16-bit forms, the two-steps-per-pass sweep and both launch forms.  Every
+16-bit forms, the two- and three-steps-per-pass sweeps and both launch
+forms.  Every
 result must equal the oracle bit for bit."""
 import numpy as np
 import pytest
@@ -32,7 +33,7 @@ def test_tiny_fp32(gpu, orc, npts, shape):
     want_u = a if newest == 0 else b
     I = g.interior
     for tuning in ({"launch": 1, "steps_per_pass": 1}, {"launch": 2, "steps_per_pass": 1},
-                   {"steps_per_pass": 2}):
+                   {"steps_per_pass": 2}) + (({"steps_per_pass": 3},) if npts != 25 else ()):
         with gpu.Stencil(*shape, npts=npts) as ctx:
             ctx.set_tuning(**tuning)
             for f, x in (("u", u), ("v", v), ("s", s)):
