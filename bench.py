@@ -129,8 +129,8 @@ def run_batch(S, steps=100, nshots=64, seed=2):
     a random interior point of each, c dt/dx = 0.4; `steps` steps in one call,
     timed for the persistent launch (in a hipGraph and eager), for one plain
     launch per shot per step (eager and in a hipGraph), and for the persistent
-    two-steps-per-pass launch.  Tile and planes per item of the one-step
-    persistent launch come from the autotuner."""
+    two- and three-steps-per-pass launches.  Tile and planes per item of the
+    one-step persistent launch come from the autotuner."""
     import numpy as np
     rng = np.random.default_rng(seed)
     dims = [tuple(int(x) for x in rng.choice([96, 128, 192, 256, 320, 384], 3))
@@ -149,13 +149,15 @@ def run_batch(S, steps=100, nshots=64, seed=2):
                                     ("sequential_eager", S.BATCH_SEQUENTIAL, False),
                                     ("sequential_graph", S.BATCH_SEQUENTIAL, True),
                                     ("two_step_graph", S.BATCH_TWO_STEP, True),
-                                    ("two_step_eager", S.BATCH_TWO_STEP, False)):
+                                    ("two_step_eager", S.BATCH_TWO_STEP, False),
+                                    ("three_step_graph", S.BATCH_THREE_STEP, True),
+                                    ("three_step_eager", S.BATCH_THREE_STEP, False)):
             for q, (d, sp) in enumerate(zip(dims, srcs)):
                 sh = bt.shot(q)
                 sh.init_point("u", 0, 0, 0, 0.0)
                 sh.init_point("v", 0, 0, 0, 0.0)
                 sh.init_point("s", *sp, 1.0)
-            bt.leapfrog(2, c0, c1, None, method=method, graph=graph)  # warm-up
+            bt.leapfrog(6, c0, c1, None, method=method, graph=graph)  # warm-up
             bt.sync()
             bt.timer_start()
             t0 = time.perf_counter()
@@ -173,6 +175,8 @@ def run_batch(S, steps=100, nshots=64, seed=2):
     p, q = out["persistent_graph"]["gpu_ms_per_step"], out["sequential_eager"]["gpu_ms_per_step"]
     out["persistent_speedup_vs_sequential"] = round(q / p, 3)
     out["two_step_speedup_vs_persistent"] = round(p / out["two_step_graph"]["gpu_ms_per_step"], 3)
+    out["three_step_speedup_vs_persistent"] = round(
+        p / out["three_step_graph"]["gpu_ms_per_step"], 3)
     return out
 
 
@@ -265,9 +269,9 @@ def measured_traffic(n, dtype_code, tune):
         return None, None
     if t.get("grid") != [n, n, n]:
         return None, None
-    if tune.get("steps_per_pass") == 2:
+    if tune.get("steps_per_pass", 0) >= 2:
         prefix = (f"void sth::sweep2_kernel<{NPTS}, {tune['rows_per_wave']}, "
-                  f"{tune['waves']}")
+                  f"{tune['waves']}, {tune['steps_per_pass']}")
     else:
         # the last template argument (fold of the last k-tile) follows the grid
         prefix = (f"void sth::sweep_kernel<{NPTS}, 1, {dtype_code}, {tune['rows_per_wave']}, "
@@ -307,7 +311,7 @@ def main():
                                                              copy_bytes=copy_bytes)
     copy_gbs = 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9
     gpts = pts * K / wall / 1e9
-    # A launch advances `units` steps (2 for the two-steps-per-pass sweep):
+    # A launch advances `units` steps (2 or 3 for a fused pass):
     # algorithmic bytes per launch = 16 B x points x units.
     units = K / launches
     per_launch_s = kms / launches * 1e-3
@@ -317,7 +321,7 @@ def main():
         f"({units:g} steps), {achieved:.0f} GB/s")
     traffic, traffic_src = measured_traffic(n, 0, tune)
     one_step = None
-    if tune.get("steps_per_pass") == 2:
+    if tune.get("steps_per_pass", 0) >= 2:
         # the one-step sweep on the same fields: the baseline of the FP16 ratio
         w1, k1, l1, t1, _, _ = run_throughput(S, n, "fp32", K, W, tuning={"steps_per_pass": 1})
         g1 = pts * K / w1 / 1e9

@@ -111,7 +111,7 @@ SIGNATURES = {
     "sth_batch_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
 }
 
-BATCH_PERSISTENT, BATCH_SEQUENTIAL, BATCH_TWO_STEP, BATCH_GRAPH = 0, 1, 2, 0x100
+BATCH_PERSISTENT, BATCH_SEQUENTIAL, BATCH_TWO_STEP, BATCH_THREE_STEP, BATCH_GRAPH = 0, 1, 2, 3, 0x100
 
 _lib = None
 

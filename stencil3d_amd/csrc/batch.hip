@@ -8,8 +8,8 @@
 //    queue (an atomic ticket counter), largest items first, and runs each with
 //    the same tile engine as the single-grid sweep (sweep_tile).
 //  * sequential: one plain sweep launch per shot per step.
-//  * two steps per pass: one persistent launch per two steps, items of the
-//    two-step sweep (sweep2_tile) of every shot from a queue.
+//  * two or three steps per pass: one persistent launch per pass, items of
+//    the fused sweep (sweep2_tile) of every shot from a queue.
 // Either way the time loop can be captured into one hipGraph per call.
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -232,38 +232,39 @@ int build_tables(sth_batch *b) {
   return STH_OK;
 }
 
-// The two-step kernel's shape: the stencil's default (rows per wave, waves).
-void shape2(uint32_t npts, int &mj, int &nw) {
-  mj = npts == 25 ? 3 : 4;
+// The fused kernel's shape for S steps per pass: the stencil's default
+// (rows per wave, waves).
+void shape2(uint32_t npts, int S, int &mj, int &nw) {
+  mj = npts == 25 || S == 3 ? 3 : 4;
   nw = 16;
 }
 
-unsigned persistent_grid2(const sth_batch *b, int mj, int nw) {
+unsigned persistent_grid2(const sth_batch *b, int mj, int nw, int S) {
   int per_cu = 0;
-  const void *fn = batch2_fn((int)b->npts, mj, nw);
+  const void *fn = batch2_fn((int)b->npts, mj, nw, S);
   if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * nw, 0) != hipSuccess ||
       per_cu < 1)
     per_cu = 1;
   return (unsigned)(per_cu * b->cus);
 }
 
-// Work list of a pass over all shots, from the shots' current buffers (the
-// twins must exist).  i-chunks: `planes` planes per item when set, else as
-// many as give about six rounds of the persistent grid (at least 8R planes
-// per chunk: each chunk recomputes 2R+1 planes).
-int build_tables2(sth_batch *b, int mj, int nw) {
+// Work list of a pass of S steps over all shots, from the shots' current
+// buffers (the twins must exist).  i-chunks: `planes` planes per item when
+// set, else as many as give about six rounds of the persistent grid (at
+// least 8R planes per chunk: each chunk recomputes (S-1)(2R+1) planes).
+int build_tables2(sth_batch *b, int mj, int nw, int S) {
   const size_t ns = b->shots.size();
   sth_tuning t = b->tune;
   t.rows_per_wave = (uint32_t)mj;
   t.waves = (uint32_t)nw;
-  t.steps_per_pass = 2;
+  t.steps_per_pass = (uint32_t)S;
   int64_t tiles = 0;
   for (sth_ctx *c : b->shots) {
     t.chunks = 1;
     const Plan2 p = make_plan2(c, t);
     tiles += (int64_t)p.ntk * p.ntj;
   }
-  const int64_t slots = persistent_grid2(b, mj, nw);
+  const int64_t slots = persistent_grid2(b, mj, nw, S);
   const int target = (int)std::max<int64_t>(1, (6 * slots + tiles - 1) / std::max<int64_t>(1, tiles));
   std::vector<Batch2Shot> shots(ns);
   std::vector<Batch2Item> items;
@@ -326,12 +327,13 @@ int build_tables2(sth_batch *b, int mj, int nw) {
   return STH_OK;
 }
 
-// Queues `passes` two-step passes (steps 2q, 2q+1 use c2[2q], c2[2q+1]).
-int enqueue_passes(sth_batch *b, uint32_t passes, float c0, float c1, const float *c2, int ub,
-                   uint64_t *launches) {
+// Queues `passes` passes of S steps (step S*q+r uses c2[S*q+r]); u starts
+// in buffer ub0.
+int enqueue_passes(sth_batch *b, int S, uint32_t passes, float c0, float c1, const float *c2,
+                   int ub0, uint64_t *launches) {
   int mj, nw;
-  shape2(b->npts, mj, nw);
-  const unsigned G = persistent_grid2(b, mj, nw);
+  shape2(b->npts, S, mj, nw);
+  const unsigned G = persistent_grid2(b, mj, nw, S);
   const unsigned per_launch = (unsigned)b->total_items2 + G;
   if ((uint64_t)b->ticket_base + (uint64_t)per_launch * passes >= 0xF0000000ull) {
     STH_TRY(launch_zero_u32(b->ticket, 1, b->stream));
@@ -346,16 +348,18 @@ int enqueue_passes(sth_batch *b, uint32_t passes, float c0, float c1, const floa
   a.items = b->items2;
   a.total_items = b->total_items2;
   a.ticket = b->ticket;
-  a.ub = ub;
   a.c0 = c0;
   a.c1 = c1;
   memcpy(a.w, wts.w, sizeof(a.w));
   for (uint32_t q = 0; q < passes; q++) {
     a.ticket_base = b->ticket_base;
+    a.ub = (ub0 + S * (int)q) & 1;
     a.phase = (int)(q & 1);
-    a.c2a = c2 ? c2[2 * q] : 0.0f;
-    a.c2b = c2 ? c2[2 * q + 1] : 0.0f;
-    STH_TRY(launch_batch2((int)b->npts, mj, nw, G, b->stream, a));
+    const float *cq = c2 ? c2 + (size_t)S * q : nullptr;
+    a.c2a = cq ? cq[0] : 0.0f;
+    a.c2b = cq ? cq[1] : 0.0f;
+    a.c2c = cq && S == 3 ? cq[2] : 0.0f;
+    STH_TRY(launch_batch2((int)b->npts, mj, nw, S, G, b->stream, a));
     b->ticket_base += per_launch;
     (*launches)++;
   }
@@ -437,16 +441,17 @@ int enqueue_steps(sth_batch *b, uint32_t steps, float c0, float c1, const float 
   return STH_OK;
 }
 
-// The call's launches: `passes` two-step passes, then `single` steps of the
-// one-step method (persistent for STH_BATCH_TWO_STEP).
-int enqueue_all(sth_batch *b, uint32_t passes, uint32_t single, float c0, float c1,
+// The call's launches: `passes` passes of S steps, then `single` steps of
+// the one-step method (persistent after fused passes).
+int enqueue_all(sth_batch *b, int S, uint32_t passes, uint32_t single, float c0, float c1,
                 const float *c2, int base, int parity0, uint64_t *launches) {
   int st = STH_OK;
-  if (passes && (st = enqueue_passes(b, passes, c0, c1, c2, parity0, launches))) return st;
+  if (passes && (st = enqueue_passes(b, S, passes, c0, c1, c2, parity0, launches))) return st;
   if (!single) return STH_OK;
-  const int method = base == STH_BATCH_TWO_STEP ? STH_BATCH_PERSISTENT : base;
-  return enqueue_steps(b, single, c0, c1, c2 ? c2 + 2 * passes : nullptr, method, parity0,
-                       launches);
+  const int method = S ? STH_BATCH_PERSISTENT : base;
+  const uint32_t done = (uint32_t)S * passes;
+  return enqueue_steps(b, single, c0, c1, c2 ? c2 + done : nullptr, method,
+                       (parity0 + (int)done) & 1, launches);
 }
 
 }  // namespace
@@ -553,18 +558,21 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
   if (!b) return STH_ERR_ARG;
   const int base = (int)(method & 0xff);
   const bool graph = (method & STH_BATCH_GRAPH) != 0;
-  if (base != STH_BATCH_PERSISTENT && base != STH_BATCH_SEQUENTIAL && base != STH_BATCH_TWO_STEP)
+  if (base != STH_BATCH_PERSISTENT && base != STH_BATCH_SEQUENTIAL &&
+      base != STH_BATCH_TWO_STEP && base != STH_BATCH_THREE_STEP)
     return STH_ERR_ARG;
+  if (base == STH_BATCH_THREE_STEP && b->npts == 25) return STH_ERR_UNSUPPORTED;
   if (steps == 0) return STH_OK;
   const int parity0 = b->shots[0]->ubuf;
-  const uint32_t passes = base == STH_BATCH_TWO_STEP ? steps / 2 : 0;
+  const int S = base == STH_BATCH_TWO_STEP ? 2 : base == STH_BATCH_THREE_STEP ? 3 : 0;
+  const uint32_t passes = S ? steps / (uint32_t)S : 0;
   int st = STH_OK;
   if (passes) {
     for (sth_ctx *c : b->shots)
       if ((st = ensure_twins(c))) return st;
     int mj, nw;
-    shape2(b->npts, mj, nw);
-    if ((st = build_tables2(b, mj, nw))) return st;
+    shape2(b->npts, S, mj, nw);
+    if ((st = build_tables2(b, mj, nw, S))) return st;
     if (passes & 1) {
       // after an odd number of passes each shot's u and v live in the
       // buffers that are now its twins: swap the names now, so the one-step
@@ -576,7 +584,7 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
       b->tables_ok = false;
     }
   }
-  const uint32_t single = steps - 2 * passes;  // one-step steps of this call
+  const uint32_t single = steps - (uint32_t)S * passes;  // one-step steps of this call
   if (single && (st = build_tables(b))) {
     if (passes & 1)
       for (sth_ctx *c : b->shots) {  // nothing was queued: restore the names
@@ -599,7 +607,8 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
     STH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = launch_zero_u32(b->ticket, 1, b->stream);
     b->ticket_base = 0;
-    if (e == hipSuccess) st = enqueue_all(b, passes, single, c0, c1, c2, base, parity0, &launches);
+    if (e == hipSuccess)
+      st = enqueue_all(b, S, passes, single, c0, c1, c2, base, parity0, &launches);
     const hipError_t ec = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess || st != STH_OK || ec != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
@@ -614,7 +623,7 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
     if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
   } else {
     if (b->timing) STH_TRY(hipEventRecord(b->ev0, b->stream));
-    st = enqueue_all(b, passes, single, c0, c1, c2, base, parity0, &launches);
+    st = enqueue_all(b, S, passes, single, c0, c1, c2, base, parity0, &launches);
     if (st) return st;
     if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
   }

@@ -39,30 +39,33 @@ hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hip
   return hipErrorInvalidValue;
 }
 
-#define STH_BATCH2_LIST(X) X(7, 4, 16) X(13, 4, 16) X(25, 3, 16)
+// (stencil, rows per wave, waves, steps per pass)
+#define STH_BATCH2_LIST(X) \
+  X(7, 4, 16, 2) X(13, 4, 16, 2) X(25, 3, 16, 2) X(7, 3, 16, 3) X(13, 3, 16, 3)
 
-bool batch2_supported(int npts, int mj, int nw) {
-#define STH_B2_SUP(N, MJ, NW) \
-  if (npts == N && mj == MJ && nw == NW) return true;
+bool batch2_supported(int npts, int mj, int nw, int steps) {
+#define STH_B2_SUP(N, MJ, NW, S) \
+  if (npts == N && mj == MJ && nw == NW && steps == S) return true;
   STH_BATCH2_LIST(STH_B2_SUP)
 #undef STH_B2_SUP
   return false;
 }
 
-const void *batch2_fn(int npts, int mj, int nw) {
-#define STH_B2_FN(N, MJ, NW) \
-  if (npts == N && mj == MJ && nw == NW) return (const void *)&batch2_kernel<N, MJ, NW>;
+const void *batch2_fn(int npts, int mj, int nw, int steps) {
+#define STH_B2_FN(N, MJ, NW, S)                                  \
+  if (npts == N && mj == MJ && nw == NW && steps == S)           \
+    return (const void *)&batch2_kernel<N, MJ, NW, S>;
   STH_BATCH2_LIST(STH_B2_FN)
 #undef STH_B2_FN
   return nullptr;
 }
 
-hipError_t launch_batch2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+hipError_t launch_batch2(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                          const Batch2Args &b) {
-#define STH_B2_LAUNCH(N, MJ, NW)                                                   \
-  if (npts == N && mj == MJ && nw == NW) {                                         \
-    hipLaunchKernelGGL((batch2_kernel<N, MJ, NW>), dim3(grid), dim3(64 * NW), 0, st, b); \
-    return hipGetLastError();                                                      \
+#define STH_B2_LAUNCH(N, MJ, NW, S)                                                   \
+  if (npts == N && mj == MJ && nw == NW && steps == S) {                              \
+    hipLaunchKernelGGL((batch2_kernel<N, MJ, NW, S>), dim3(grid), dim3(64 * NW), 0, st, b); \
+    return hipGetLastError();                                                         \
   }
   STH_BATCH2_LIST(STH_B2_LAUNCH)
 #undef STH_B2_LAUNCH

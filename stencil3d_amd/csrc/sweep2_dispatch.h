@@ -19,10 +19,10 @@ Fused2Geom fused2_geom(int npts, int mj, int nw, int steps);
 hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                          const Sweep2Args &a);
 // The batch form (persistent, items of many shots from a queue) exists for
-// each stencil's default shape.
-bool batch2_supported(int npts, int mj, int nw);
-const void *batch2_fn(int npts, int mj, int nw);
-hipError_t launch_batch2(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+// each stencil's default shape of each pass length.
+bool batch2_supported(int npts, int mj, int nw, int steps);
+const void *batch2_fn(int npts, int mj, int nw, int steps);
+hipError_t launch_batch2(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                          const Batch2Args &b);
 
 }  // namespace sth

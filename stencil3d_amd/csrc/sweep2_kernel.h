@@ -511,13 +511,13 @@ struct Batch2Args {
   unsigned ticket_base;  // tickets taken by earlier launches since the last reset
   int ub;                // buffer of u: 0 = a, 1 = b
   int phase;             // 0: read the bases, write the twins; 1: the reverse
-  float c0, c1, c2a, c2b;
+  float c0, c1, c2a, c2b, c2c;
   float w[12];
 };
 
 // Every block takes tickets until it draws one past the end, so a launch
-// consumes exactly total_items + gridDim.x tickets.
-template <int NPTS, int MJ, int NW>
+// consumes exactly total_items + gridDim.x tickets.  S steps per pass.
+template <int NPTS, int MJ, int NW, int S>
 __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
   __shared__ int sh_item;
   for (;;) {
@@ -533,8 +533,11 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.u = sh.buf[b.ub + in];
     a.v = sh.buf[vb + in];
     a.s = sh.s;
-    a.o1 = const_cast<void *>(sh.buf[vb + out]);  // u_{n+1}: the halo of v
-    a.o2 = const_cast<void *>(sh.buf[b.ub + out]);  // u_{n+2}: the halo of u
+    // the two newest fields go to the twins of the buffers the one-step
+    // scheme writes them to: S = 2, u_{n+1} over v and u_{n+2} over u;
+    // S = 3, u_{n+2} over u and u_{n+3} over v
+    a.o1 = const_cast<void *>(sh.buf[(S == 2 ? vb : b.ub) + out]);
+    a.o2 = const_cast<void *>(sh.buf[(S == 2 ? b.ub : vb) + out]);
     a.pitch = sh.pitch;
     a.plane = sh.plane;
     a.xs = sh.xs;
@@ -549,10 +552,10 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.c1 = b.c1;
     a.c2a = b.c2a;
     a.c2b = b.c2b;
-    a.c2c = 0.0f;
+    a.c2c = b.c2c;
 #pragma unroll
     for (int t = 0; t < 12; t++) a.w[t] = b.w[t];
-    sweep2_tile<NPTS, MJ, NW, 2>(a, it.tk, it.tj, it.ch);
+    sweep2_tile<NPTS, MJ, NW, S>(a, it.tk, it.tj, it.ch);
     __syncthreads();  // LDS is reused by the next item
   }
 }

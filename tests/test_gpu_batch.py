@@ -57,15 +57,16 @@ def test_batch_matches_oracle(gpu, orc, method, graph, npts):
 
 def test_persistent_equals_sequential_config_shapes(gpu, orc):
     """Eight shots with sides from the survey config's set, 12 steps: the
-    persistent, sequential, two-steps-per-pass and graphed runs agree bit for
-    bit."""
+    persistent, sequential, two- and three-steps-per-pass and graphed runs
+    agree bit for bit."""
     rng = np.random.default_rng(2)
     dims = [tuple(int(x) for x in rng.choice([96, 128, 192], 3)) for _ in range(8)]
     npts, R = 13, 2
     c0, c1 = coeffs(orc, npts)
     results = []
     for method, graph in ((0, False), (1, False), (0, True), (gpu.BATCH_TWO_STEP, False),
-                          (gpu.BATCH_TWO_STEP, True)):
+                          (gpu.BATCH_TWO_STEP, True), (gpu.BATCH_THREE_STEP, False),
+                          (gpu.BATCH_THREE_STEP, True)):
         with gpu.Batch(dims, npts=npts) as bt:
             for q in range(len(dims)):
                 bt.shot(q).init_gaussian("u", 1.0, 6.0, 1e-3, 1, q)
@@ -169,6 +170,38 @@ def test_two_step_batch_matches_oracle(gpu, orc, graph, steps, npts):
         assert np.array_equal(got[q][1].view(np.uint32), wv.view(np.uint32)), q
 
 
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("steps", [6, 7, 8])
+@pytest.mark.parametrize("npts", [7, 13])
+def test_three_step_batch_matches_oracle(gpu, orc, graph, steps, npts):
+    """Three LEAPFROG steps per pass over all shots (the one or two steps
+    left over run one-step): every shot bit-identical to the oracle, halos
+    included."""
+    R = orc.radius(npts)
+    fields = shot_fields(orc, DIMS, npts, 60)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-2, 1e-2, steps).astype(np.float32)
+    with gpu.Batch(DIMS, npts=npts) as bt:
+        load(bt, fields, R)
+        bt.timer_start()
+        bt.leapfrog(steps, c0, c1, c2, method=gpu.BATCH_THREE_STEP, graph=graph)
+        _, launches = bt.timer_read()
+        got = [(bt.shot(q).download("u", R), bt.shot(q).download("v", R))
+               for q in range(len(DIMS))]
+    assert launches == steps // 3 + steps % 3
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, steps)
+        wu, wv = (a, b) if newest == 0 else (b, a)
+        assert np.array_equal(got[q][0].view(np.uint32), wu.view(np.uint32)), q
+        assert np.array_equal(got[q][1].view(np.uint32), wv.view(np.uint32)), q
+
+
+def test_three_step_batch_rejected_for_25_points(gpu):
+    with gpu.Batch(DIMS[:2], npts=25) as bt:
+        with pytest.raises(gpu.SthError):
+            bt.leapfrog(3, 0.08, 0.16, method=gpu.BATCH_THREE_STEP)
+
+
 def test_two_step_batch_mixed_with_other_methods(gpu, orc):
     """Calls of every method in a row (the two-step calls swap each shot's
     buffers with its twins), an upload in between: still the oracle's bits."""
@@ -178,7 +211,8 @@ def test_two_step_batch_mixed_with_other_methods(gpu, orc):
     c0, c1 = coeffs(orc, npts)
     plan = [(3, gpu.BATCH_TWO_STEP, False), (2, gpu.BATCH_PERSISTENT, False),
             (5, gpu.BATCH_TWO_STEP, True), (1, gpu.BATCH_SEQUENTIAL, False),
-            (4, gpu.BATCH_TWO_STEP, False)]
+            (4, gpu.BATCH_TWO_STEP, False), (7, gpu.BATCH_THREE_STEP, False),
+            (3, gpu.BATCH_THREE_STEP, True), (5, gpu.BATCH_THREE_STEP, False)]
     c2 = np.linspace(-5e-3, 5e-3, sum(p[0] for p in plan)).astype(np.float32)
     new_v = orc.fill_uniform(fields[1][0], 77, 1, halo=0.5)
     with gpu.Batch(DIMS, npts=npts) as bt:
