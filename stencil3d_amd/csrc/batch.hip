@@ -116,7 +116,7 @@ struct sth_batch {
   int total_items = 0;
   unsigned ticket_base = 0;
   bool tables_ok = false;
-  // device tables of the two-steps-per-pass kernel (rebuilt on every call:
+  // device tables of the fused-pass kernel (rebuilt on every call:
   // the shots' buffers trade places with their twins after each pass)
   Batch2Shot *shot2 = nullptr;
   Batch2Item *items2 = nullptr;

@@ -1,4 +1,4 @@
-// sweep2_dispatch.h -- host-side entry to the two-steps-per-pass sweep.
+// sweep2_dispatch.h -- host-side entry to the fused sweep (two or three steps per pass).
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -1,9 +1,11 @@
-// sweep2_kernel.h -- two LEAPFROG steps per pass over memory (FP32).
+// sweep2_kernel.h -- two or three LEAPFROG steps per pass over memory (FP32).
 //
 // The one-step sweep moves 16 B per point-update (read u, v, s; write v)
 // and on MI355X sits at the read/write mix bandwidth of HBM.  A pass of this
-// kernel reads u_n (with its halo), u_{n-1} and s once and writes u_{n+1}
-// and u_{n+2}: 20 B per two point-updates.
+// kernel reads u_n (with its halo), u_{n-1} and s once and writes the two
+// newest fields: 20 B per two (or three) point-updates.  The two-step form
+// is described here; the three-step form (S = 3) adds one stage, described
+// at sweep2_tile.
 //
 //  * A workgroup owns an output tile of OJ x OK (j, k) points.  Step one
 //    (u_{n+1}) runs on the tile widened by R on every side (E1: EJ rows x 64

@@ -126,3 +126,25 @@ def test_autotune_cache_round_trip(gpu, tmp_path, monkeypatch):
         assert ctx.get_tuning()["rows_per_wave"] == best["rows_per_wave"]
     assert best2 == best and abs(ms2 - ms) < 1e-4
     assert len(cache.read_text().splitlines()) == 1
+
+
+def test_tune_cache_three_step_lines(gpu, tmp_path, monkeypatch):
+    """A stored three-steps-per-pass result is applied without timing where
+    the stencil has that pass (13 points) and ignored where it has not (25
+    points: the tuner times its candidates and appends its own line)."""
+    cache = tmp_path / "tune.txt"
+    cache.write_text("sth-tune 2 13 0 1 40 44 100 4 12 1 2 3 0.01000\n"
+                     "sth-tune 2 25 0 1 30 34 90 3 16 1 2 3 0.01000\n")
+    monkeypatch.setenv("STH_TUNE_CACHE", str(cache))
+    with gpu.Stencil(40, 44, 100, npts=13) as ctx:
+        best, ms = ctx.autotune(leapfrog=True, reps=1)
+        t = ctx.get_tuning()
+    assert (best["rows_per_wave"], best["waves"], best["steps_per_pass"]) == (4, 12, 3)
+    assert ms == pytest.approx(0.01)
+    assert (t["rows_per_wave"], t["waves"], t["chunks"], t["steps_per_pass"]) == (4, 12, 2, 3)
+    with gpu.Stencil(30, 34, 90, npts=25) as ctx:
+        ctx.init_const(0.0, 1.0, 1.0)
+        best, ms = ctx.autotune(leapfrog=True, reps=1)
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 3 and lines[2].startswith("sth-tune 2 25 0 1 30 34 90")
+    assert best["steps_per_pass"] in (1, 2) and ms != pytest.approx(0.01)

@@ -1,9 +1,10 @@
 """GPU properties at the headline size (1040^3 interior, 25 points), where the
 oracle cannot follow: size-independent checks only.
 
-  * The two-steps-per-pass schedule (the FP32 LEAPFROG default) and the
-    one-step sweep give the same bits for u and v after six steps (hashes of
-    the downloaded fields, halos included), with a nonzero source term.
+  * The two-steps-per-pass schedule (the 25-point FP32 LEAPFROG default) and
+    the one-step sweep give the same bits for u and v after six steps (hashes
+    of the downloaded fields, halos included), with a nonzero source term;
+    the same for the three-steps-per-pass default of 7 and 13 points.
   * The reference's own known-answer check (u = s = 1, v = 0; each REF sweep
     adds the stencil's constant): every interior point within 0.1 of the
     sweep count (iterators.cpp:304-325), the maximum equal to the exact FP32
@@ -44,6 +45,28 @@ def test_two_step_pass_equals_one_step_at_headline_size(gpu):
             ctx.leapfrog(6, c0, c1, c2)
             _, launches = ctx.timer_read()
             assert launches == (3 if spp == 0 else 6)  # the default runs three passes
+            hashes[spp] = (field_hash(ctx, "u", R, buf), field_hash(ctx, "v", R, buf))
+    assert hashes[0] == hashes[1]
+
+
+@pytest.mark.parametrize("npts", [7, 13])
+def test_three_step_pass_equals_one_step_at_headline_size(gpu, npts):
+    R = npts // 6  # 1, 2
+    c1 = 0.25
+    c0 = float(np.float32(2.0 - 6.0 * c1))
+    c2 = np.linspace(-2e-3, 3e-3, 7).astype(np.float32)
+    buf = np.empty((N + 2 * R,) * 3, np.float32)
+    hashes = {}
+    for spp in (0, 1):  # 0: the default schedule
+        with gpu.Stencil(N, N, N, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 2)
+            ctx.init_gaussian("v", 1.0, 8.0, 1e-3, 1, 3)
+            ctx.init_uniform("s", 9, 2, -1.0, 1.0, 0.0)
+            ctx.timer_start()
+            ctx.leapfrog(7, c0, c1, c2)
+            _, launches = ctx.timer_read()
+            assert launches == (3 if spp == 0 else 7)  # two three-step passes and a sweep
             hashes[spp] = (field_hash(ctx, "u", R, buf), field_hash(ctx, "v", R, buf))
     assert hashes[0] == hashes[1]
 
