@@ -42,11 +42,11 @@ struct Sweep2Args {
   int xs, ys, zs;
   int ntk, ntj, chunks, chunk_len, remap;
   float c0, c1;
-  float c2a, c2b, c2c;  // source coefficients of steps n, n+1, n+2
+  float c2a, c2b, c2c, c2d;  // source coefficients of steps n .. n+3
   float w[12];     // LEAPFROG weights (2 * stencil weight), evaluation order
 };
 
-// S steps per pass (2 or 3): step one runs on the output tile widened by
+// S steps per pass (2, 3 or 4): step one runs on the output tile widened by
 // WD = (S-1)R on every side.
 template <int NPTS, int MJ, int NW, int S = 2>
 struct Fused2Shape {
@@ -64,7 +64,7 @@ struct Fused2Shape {
   static constexpr int LW = EK + 2 * R;  // u plane + R halo
   static constexpr int LU = (EJ + 2 * R) * LW;
   static constexpr int LR = EJ * EK;  // one plane of u_{n+1}, u_{n+2} or s on E1
-  static constexpr int SR = S == 2 ? R + 1 : 2 * R + 2;  // s ring slots
+  static constexpr int SR = (S - 1) * (R + 1);  // s ring slots
   static constexpr int LDS_BYTES = (2 * LU + (2 * (S - 1) + SR) * LR) * 4;
 };
 
@@ -85,7 +85,10 @@ __device__ __forceinline__ void static_for(Fn &&f) {
 // S = 3 adds a third step: step two runs on the tile widened by R (E2) at
 // plane i = p-R-1, its u_{n+2} values go to a third register queue and, for
 // the plane i2 = i-R-1 of step three, to LDS; step three writes u_{n+2} and
-// u_{n+3} of the output tile at plane i2.
+// u_{n+3} of the output tile at plane i2.  S = 4 adds a fourth in the same
+// way: step two on the tile widened by 2R, step three on the tile widened
+// by R at plane i-R-1 (a fourth register queue and LDS plane), step four
+// on the output tile at plane i-2R-2, writing u_{n+3} and u_{n+4}.
 template <int NPTS, int MJ, int NW, int S = 2>
 __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, const int tj,
                                             const int ch) {
@@ -97,13 +100,20 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   constexpr int Q = 2 * R + 2;               // u queue: planes p-R-1 .. p+R
   constexpr int WQ = S == 2 ? 2 * R + 2 : 2 * R + 3;  // u_{n+1} queue (+ the newest)
   constexpr int WC = WQ - R - 2;             // u_{n+1} queue entry of plane i
-  constexpr int U = Q;                       // steps per loop iteration
-  static_assert(S == 2 || S == 3, "two or three steps per pass");
+  constexpr int ZQ = S == 4 ? 2 * R + 3 : 2 * R + 2;  // u_{n+2} queue (S >= 3)
+  constexpr int ZC = ZQ - R - 2;             // u_{n+2} queue entry of step three's plane
+  constexpr int YQ = 2 * R + 2;              // u_{n+3} queue (S = 4)
+  constexpr int YC = YQ - R - 2;             // u_{n+3} queue entry of step four's plane
+  // Steps per loop iteration: a multiple of the u queue length and of the
+  // s ring (S = 4: 3R+3 slots).
+  constexpr int U = S == 4 ? 3 * Q : Q;
+  static_assert(S >= 2 && S <= 4, "two to four steps per pass");
+  static_assert(U % SR == 0 && U % 2 == 0, "ring slots and buffers repeat with the loop");
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
   static_assert(F::LDS_BYTES <= 160 * 1024, "LDS");
-  // One LDS array: the two u planes (LU), the two u_{n+1} planes (W; S = 3:
-  // then the two u_{n+2} planes), then SR planes of s, each lane's own
-  // values (step two needs s(p-R-1), step three s(p-2R-2)).
+  // One LDS array: the two u planes (LU), the two u_{n+1} planes (W; S >= 3:
+  // then the two u_{n+2} planes; S = 4: the two u_{n+3} planes), then SR
+  // planes of s, each lane's own values (step m needs s(p-(m-1)(R+1))).
   __shared__ float SM[2 * F::LU + (2 * (S - 1) + SR) * F::LR];
 
   const int ib = ch * a.chunk_len;
@@ -153,7 +163,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   const bool kout = kin && lane >= WD && lane < WD + OK;
   uint32_t off[MJ];
   bool inner[MJ];
-  int rowout[MJ], row2[MJ];  // row2: S = 3, the row lies in E2
+  // row2 / row3: the row is one of step two's (the tile widened by (S-2)R:
+  // E1 less R on each side) / step three's (S = 4: E1 less 2R)
+  int rowout[MJ], row2[MJ], row3[MJ];
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int er = wv * MJ + m, jg = je + er;
@@ -162,6 +174,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     inner[m] = jin && kin;
     rowout[m] = __builtin_amdgcn_readfirstlane((int)(jin && er >= WD && er < WD + OJ));
     row2[m] = __builtin_amdgcn_readfirstlane((int)(er >= R && er < EJ - R));
+    row3[m] = __builtin_amdgcn_readfirstlane((int)(er >= 2 * R && er < EJ - 2 * R));
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
@@ -213,9 +226,11 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   //   q[m][t]  = u_n(p - R - 1 + t),      t = 0 .. 2R+1;
   //   wq[m][t] = u_{n+1}(p - WQ + 1 + t), t = 0 .. WQ-2 (t = WQ-1 takes step
   //              one's result at plane p before the shift);
-  //   zq[m][t] = u_{n+2}(i - 2R - 1 + t), t = 0 .. 2R (S = 3; t = 2R+1 takes
-  //              step two's result at plane i).
-  float q[MJ][Q], wq[MJ][WQ], zq[MJ][S == 3 ? 2 * R + 2 : 1];
+  //   zq[m][t] = u_{n+2}(i - ZQ + 1 + t), t = 0 .. ZQ-2 (S >= 3; t = ZQ-1
+  //              takes step two's result at plane i);
+  //   yq[m][t] = u_{n+3}(i3 - YQ + 1 + t), t = 0 .. YQ-2 (S = 4; t = YQ-1
+  //              takes step three's result at its plane i3 = i-R-1).
+  float q[MJ][Q], wq[MJ][WQ], zq[MJ][S >= 3 ? ZQ : 1], yq[MJ][S == 4 ? YQ : 1];
   float hq[HJ][R + 1], kq[R + 1];
   // Step one from p0 (its planes past ie-1+WD are never used), the last
   // step (output plane p - (S-1)(R+1)) up to plast.
@@ -232,7 +247,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 #pragma unroll
     for (int t = 0; t < WQ; t++) wq[m][t] = 0.0f;
 #pragma unroll
-    for (int t = 0; t < (S == 3 ? 2 * R + 2 : 1); t++) zq[m][t] = 0.0f;
+    for (int t = 0; t < (S >= 3 ? ZQ : 1); t++) zq[m][t] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < (S == 4 ? YQ : 1); t++) yq[m][t] = 0.0f;
   }
 #pragma unroll
   for (int t = 0; t < R; t++) {
@@ -283,7 +300,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // register state depends on a skipped step.
   auto step = [&](int p, Staged &n, auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
-    float vv[MJ], s1[MJ], s2c[MJ], s3c[MJ];
+    float vv[MJ], s1[MJ], s2c[MJ], s3c[MJ], s4c[MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       q[m][Q - 1] = n.u[m];
@@ -299,25 +316,31 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     Sn = p + 3 <= last ? Sn + PB : Sn;
 
     asm volatile("" : "+v"(lbL), "+v"(lbW), "+v"(lbS));
-    // The s ring (the lane's own words, so no barrier is involved): S = 2,
-    // s(p-R-1) and s(p) share slot k mod (R+1); S = 3, s(p-2R-2) and s(p)
-    // share slot k and s(p-R-1) is in slot k+R+1 (mod 2R+2).  Old values are
+    // The s ring of (S-1)(R+1) slots (the lane's own words, so no barrier is
+    // involved): step m of plane p-(m-1)(R+1) finds its s in slot
+    // k + (S-m)(R+1); the last step's s and s(p) share slot k, the old value
     // read before the new one is stored.
     constexpr uint32_t SO = (k % SR) * LRB;
+    constexpr uint32_t SO2 = ((k + (S - 2) * (R + 1)) % SR) * LRB;
+    constexpr uint32_t SO3 = ((k + (S - 3) * (R + 1)) % SR) * LRB;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       if constexpr (S == 2) {
         s2c[m] = lds(lbS + SO + m * 256);
-      } else {
-        constexpr uint32_t SO2 = ((k + R + 1) % SR) * LRB;
+      } else if constexpr (S == 3) {
         s3c[m] = lds(lbS + SO + m * 256);
+        s2c[m] = lds(lbS + SO2 + m * 256);
+      } else {
+        s4c[m] = lds(lbS + SO + m * 256);
+        s3c[m] = lds(lbS + SO3 + m * 256);
         s2c[m] = lds(lbS + SO2 + m * 256);
       }
       lds(lbS + SO + m * 256) = s1[m];
     }
     constexpr uint32_t LO = (k & 1) * LUB;  // this step's u buffer
     constexpr uint32_t WO = (k & 1) * LRB;  // this step's u_{n+1} buffer
-    constexpr uint32_t ZO = 2 * LRB + (k & 1) * LRB;  // S = 3: its u_{n+2} buffer
+    constexpr uint32_t ZO = 2 * LRB + (k & 1) * LRB;  // S >= 3: its u_{n+2} buffer
+    constexpr uint32_t YO = 4 * LRB + (k & 1) * LRB;  // S = 4: its u_{n+3} buffer
     // LU element (row m+R+dr, column R+dc of this lane's rows)
     auto Lat = [&](int m, int dr, int dc) -> LF & {
       return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + R + dc) * 4));
@@ -329,14 +352,19 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     auto Zat = [&](int m, int dr, int dc) -> LF & {
       return lds(lbW + ZO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
     };
+    auto Yat = [&](int m, int dr, int dc) -> LF & {
+      return lds(lbW + YO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
+    };
 
     // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns;
-    // u_{n+1}(p-R-1) of the E1 rows (S = 3: and u_{n+2}(p-2R-2)).
+    // u_{n+1}(p-R-1) of the E1 rows (S >= 3: and u_{n+2}(p-2R-2); S = 4:
+    // and u_{n+3}(p-3R-3)).
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       Lat(m, 0, 0) = q[m][R + 1];
       Wat(m, 0, 0) = wq[m][WC];
-      if constexpr (S == 3) Zat(m, 0, 0) = zq[m][R];
+      if constexpr (S >= 3) Zat(m, 0, 0) = zq[m][ZC];
+      if constexpr (S == 4) Yat(m, 0, 0) = yq[m][YC];
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++)
@@ -374,7 +402,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
           buf_store<float, 2>(z, Zr, off[m]);
         }
       }
-    } else {
+    } else if constexpr (S == 3) {
       // Step three at plane io = i-R-1: u_{n+3} on the output tile from the
       // u_{n+2} queue (i neighbours, in-wave j neighbours) and LDS; the
       // -u_{n+1} term is the oldest u_{n+1} entry.
@@ -382,16 +410,16 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
       for (int m = 0; m < MJ; m++) {
         if (!rowout[m]) continue;
         auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return zq[m][R - r] + zq[m][R + r];
+          if (ax == 0) return zq[m][ZC - r] + zq[m][ZC + r];
           if (ax == 2) return Zat(m, 0, -r) + Zat(m, 0, r);
-          const float lo = (m - r >= 0) ? zq[(m - r) >= 0 ? m - r : 0][R] : Zat(m, -r, 0);
-          const float hi = (m + r < MJ) ? zq[(m + r) < MJ ? m + r : 0][R] : Zat(m, r, 0);
+          const float lo = (m - r >= 0) ? zq[(m - r) >= 0 ? m - r : 0][ZC] : Zat(m, -r, 0);
+          const float hi = (m + r < MJ) ? zq[(m + r) < MJ ? m + r : 0][ZC] : Zat(m, r, 0);
           return lo + hi;
         };
         float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
         for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float zc = zq[m][R];
+        const float zc = zq[m][ZC];
         const float z3 = __builtin_fmaf(
             a.c2c, s3c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, zc, -wq[m][0])));
         if (store && kout) {
@@ -399,15 +427,67 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
           buf_store<float, 2>(z3, Zr, off[m]);
         }
       }
-      // Step two at plane i: u_{n+2} on E2, into the newest u_{n+2} entry.
-      // Outside the interior u_{n+2} takes u_n's value there (the one-step
-      // scheme writes u_{n+2} into u_n's buffer, whose halo no sweep
-      // writes).
+    } else {
+      // Step four at plane io = i-2R-2: u_{n+4} on the output tile from the
+      // u_{n+3} queue and LDS; the -u_{n+2} term is the oldest u_{n+2} entry.
+#pragma unroll
+      for (int m = 0; m < MJ; m++) {
+        if (!rowout[m]) continue;
+        auto pair = [&](int ax, int r) -> float {
+          if (ax == 0) return yq[m][YC - r] + yq[m][YC + r];
+          if (ax == 2) return Yat(m, 0, -r) + Yat(m, 0, r);
+          const float lo = (m - r >= 0) ? yq[(m - r) >= 0 ? m - r : 0][YC] : Yat(m, -r, 0);
+          const float hi = (m + r < MJ) ? yq[(m + r) < MJ ? m + r : 0][YC] : Yat(m, r, 0);
+          return lo + hi;
+        };
+        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+#pragma unroll
+        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        const float yc = yq[m][YC];
+        const float y4 = __builtin_fmaf(
+            a.c2d, s4c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, yc, -zq[m][0])));
+        if (store && kout) {
+          buf_store<float, 2>(yc, Wr, off[m]);
+          buf_store<float, 2>(y4, Zr, off[m]);
+        }
+      }
+      // Step three at plane i3 = i-R-1: u_{n+3} on the tile widened by R,
+      // into the newest u_{n+3} entry; the -u_{n+1} term is the oldest u_{n+1}
+      // entry.  Outside the interior u_{n+3} takes u_{n+1}'s value there (the
+      // one-step scheme writes u_{n+3} into u_{n+1}'s buffer).
+      const bool pin3 = i - R - 1 >= 0 && i - R - 1 < a.xs;
+#pragma unroll
+      for (int m = 0; m < MJ; m++) {
+        if (!row3[m]) {
+          yq[m][YQ - 1] = wq[m][0];
+          continue;
+        }
+        auto pair = [&](int ax, int r) -> float {
+          if (ax == 0) return zq[m][ZC - r] + zq[m][ZC + r];
+          if (ax == 2) return Zat(m, 0, -r) + Zat(m, 0, r);
+          const float lo = (m - r >= 0) ? zq[(m - r) >= 0 ? m - r : 0][ZC] : Zat(m, -r, 0);
+          const float hi = (m + r < MJ) ? zq[(m + r) < MJ ? m + r : 0][ZC] : Zat(m, r, 0);
+          return lo + hi;
+        };
+        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+#pragma unroll
+        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+        const float zc = zq[m][ZC];
+        const float y3 = __builtin_fmaf(
+            a.c2c, s3c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, zc, -wq[m][0])));
+        yq[m][YQ - 1] = (inner[m] && pin3) ? y3 : wq[m][0];
+      }
+    }
+    if constexpr (S >= 3) {
+      // Step two at plane i: u_{n+2} on the tile widened by (S-2)R, into the
+      // newest u_{n+2} entry.  Outside the interior u_{n+2} takes u_n's value
+      // there (the one-step scheme writes u_{n+2} into u_n's buffer, whose
+      // halo no sweep writes).
       const bool pin2 = i >= 0 && i < a.xs;
 #pragma unroll
       for (int m = 0; m < MJ; m++) {
         if (!row2[m]) {
-          zq[m][2 * R + 1] = q[m][0];
+          zq[m][ZQ - 1] = q[m][0];
           continue;
         }
         auto pair = [&](int ax, int r) -> float {
@@ -423,7 +503,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
         const float wc = wq[m][WC];
         const float z2 = __builtin_fmaf(a.c2b, s2c[m],
                                         __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
-        zq[m][2 * R + 1] = (inner[m] && pin2) ? z2 : q[m][0];
+        zq[m][ZQ - 1] = (inner[m] && pin2) ? z2 : q[m][0];
       }
     }
     if (store) {
@@ -457,9 +537,13 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
       for (int t = 0; t < Q - 1; t++) q[m][t] = q[m][t + 1];
 #pragma unroll
       for (int t = 0; t < WQ - 1; t++) wq[m][t] = wq[m][t + 1];
-      if constexpr (S == 3) {
+      if constexpr (S >= 3) {
 #pragma unroll
-        for (int t = 0; t < 2 * R + 1; t++) zq[m][t] = zq[m][t + 1];
+        for (int t = 0; t < ZQ - 1; t++) zq[m][t] = zq[m][t + 1];
+      }
+      if constexpr (S == 4) {
+#pragma unroll
+        for (int t = 0; t < YQ - 1; t++) yq[m][t] = yq[m][t + 1];
       }
     }
 #pragma unroll
@@ -555,6 +639,7 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.c2a = b.c2a;
     a.c2b = b.c2b;
     a.c2c = b.c2c;
+    a.c2d = 0.0f;
 #pragma unroll
     for (int t = 0; t < 12; t++) a.w[t] = b.w[t];
     sweep2_tile<NPTS, MJ, NW, S>(a, it.tk, it.tj, it.ch);

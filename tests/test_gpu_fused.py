@@ -5,7 +5,7 @@ one-step scheme never writes and which a pass must leave where the one-step
 scheme leaves them.  Covered: every compiled fused shape, i-chunks, odd step
 counts (the last step runs one-step), one-step calls and uploads between
 calls, and shapes that are not multiples of the output tile; the same for
-three steps per pass (7 and 13 points)."""
+three steps per pass (7 and 13 points) and four (7 points)."""
 import numpy as np
 import pytest
 
@@ -206,6 +206,104 @@ def test_three_step_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
     assert same_bits(out[0][1], out[1][1])
 
 
+FUSED4 = [(4, 10), (5, 8), (2, 16)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_four_step_passes_bit_exact(gpu, orc, shape):
+    """Four steps per pass (7 points): 8 steps are two passes; 9, 10 and 11
+    add a one-step sweep, a two-step pass or a three-step pass; every
+    compiled shape, with i-chunks."""
+    npts = 7
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 9, 0.375)
+    c0, c1 = coeffs(orc, npts)
+    c2 = (np.arange(1, 12, dtype=np.float32) * np.float32(6e-4)).astype(np.float32)
+    want = {n: oracle_run(orc, npts, g, u, v, s, c0, c1, c2[:n], n) for n in (8, 9, 10, 11)}
+    for mj, nw in FUSED4:
+        for chunks, steps in ((0, 8), (2, 9), (3, 10), (0, 11)):
+            with gpu.Stencil(*shape, npts=npts) as ctx:
+                ctx.set_tuning(mj, nw, chunks, steps_per_pass=4)
+                t = ctx.get_tuning()
+                assert t["steps_per_pass"] == 4 and (t["rows_per_wave"], t["waves"]) == (mj, nw)
+                ctx.upload("u", u, R)
+                ctx.upload("v", v, R)
+                ctx.upload("s", s, R)
+                ctx.timer_start()
+                ctx.leapfrog(steps, c0, c1, c2[:steps])
+                _, launches = ctx.timer_read()
+                assert launches == (2 if steps == 8 else 3)
+                gu = ctx.download("u", R)
+                gv = ctx.download("v", R)
+            assert same_bits(gu, want[steps][0]), (mj, nw, chunks, steps)
+            assert same_bits(gv, want[steps][1]), (mj, nw, chunks, steps)
+
+
+def test_four_step_passes_mixed_with_other_schedules(gpu, orc):
+    """A four-step pass, a three-step pass, one one-step sweep, a new v, then
+    five steps of four-step tuning (a pass and a sweep)."""
+    npts = 7
+    shape = (26, 31, 67)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 15, 0.25)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(1e-3, 5e-3, 5).astype(np.float32)
+    a, b = oracle_run(orc, npts, g, u, v, s, c0, c1, c2[:4], 4)
+    a, b = oracle_run(orc, npts, g, a, b, s, c0, c1, c2[:3], 3)
+    a, b = oracle_run(orc, npts, g, a, b, s, c0, c1, c2[:1], 1)
+    v2 = orc.fill_uniform(g, 16, 1, halo=0.625)
+    want_u, want_v = oracle_run(orc, npts, g, a, v2, s, c0, c1, c2, 5)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.set_tuning(steps_per_pass=4)
+        ctx.leapfrog(4, c0, c1, c2[:4])
+        ctx.set_tuning(steps_per_pass=3)
+        ctx.leapfrog(3, c0, c1, c2[:3])
+        ctx.set_tuning(steps_per_pass=1)
+        ctx.leapfrog(1, c0, c1, c2[:1])
+        ctx.set_tuning(steps_per_pass=4)
+        ctx.upload("v", v2, R)
+        ctx.leapfrog(5, c0, c1, c2)
+        gu = ctx.download("u", R)
+        gv = ctx.download("v", R)
+    assert same_bits(gu, want_u)
+    assert same_bits(gv, want_v)
+
+
+def test_four_step_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
+    """Several tiles and chunks in every direction: 12 steps as four-step
+    passes and as one-step sweeps give the same bits."""
+    shape = (96, 100, 200)
+    npts, R = 7, 1
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-3, 1e-3, 12).astype(np.float32)
+    out = []
+    for spp in (1, 4):
+        with gpu.Stencil(*shape, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            ctx.init_uniform("u", 7, 0, -1.0, 1.0, 0.5)
+            ctx.init_uniform("v", 7, 1, -1.0, 1.0, -0.5)
+            ctx.init_uniform("s", 7, 2, -1.0, 1.0, 0.25)
+            ctx.leapfrog(12, c0, c1, c2)
+            out.append((ctx.download("u", R), ctx.download("v", R)))
+    assert same_bits(out[0][0], out[1][0])
+    assert same_bits(out[0][1], out[1][1])
+
+
+def test_four_steps_only_for_7_points(gpu):
+    for npts in (13, 25):
+        with gpu.Stencil(16, 16, 64, npts=npts) as ctx:
+            with pytest.raises(gpu.SthError):
+                ctx.set_tuning(steps_per_pass=4)
+    with gpu.Stencil(16, 16, 64, npts=7, dtype="fp16") as ctx:
+        with pytest.raises(gpu.SthError):
+            ctx.set_tuning(steps_per_pass=4)
+
+
 def test_three_steps_rejected_for_25_points_and_16bit(gpu):
     with gpu.Stencil(16, 16, 64, npts=25) as ctx:
         with pytest.raises(gpu.SthError):
@@ -221,29 +319,30 @@ def test_two_steps_rejected_for_16bit(gpu):
             ctx.set_tuning(steps_per_pass=2)
 
 
-@pytest.mark.parametrize("npts,n,spp", [(7, 48, 1), (7, 64, 3), (13, 112, 1), (13, 128, 3),
+@pytest.mark.parametrize("npts,n,spp", [(7, 12, 1), (7, 48, 4), (13, 112, 1), (13, 128, 3),
                                          (25, 256, 1), (25, 320, 2)])
 def test_default_schedule_by_grid_size(gpu, orc, npts, n, spp):
     """The default FP32 LEAPFROG schedule takes several steps per pass where
-    that measured faster (enough output tiles for the stencil; three steps
-    for 7 and 13 points, two for 25): six steps are two or three passes or
-    six sweeps, and either way bit-identical to the oracle."""
+    that measured faster (enough output tiles and planes for the stencil;
+    four steps for 7 points, three for 13, two for 25): twelve steps are
+    three to six passes or twelve sweeps, and either way bit-identical to
+    the oracle."""
     R = orc.radius(npts)
     shape = (n, n, n)
     g = orc.Grid(*shape, R, R, R)
     u, v, s = seeded(orc, g, 23, 0.125)
     c0, c1 = coeffs(orc, npts)
-    c2 = np.full(6, 1e-3, np.float32)
+    c2 = np.full(12, 1e-3, np.float32)
     with gpu.Stencil(*shape, npts=npts) as ctx:
         assert ctx.get_tuning()["steps_per_pass"] == spp
         ctx.upload("u", u, R)
         ctx.upload("v", v, R)
         ctx.upload("s", s, R)
         ctx.timer_start()
-        ctx.leapfrog(6, c0, c1, c2)
+        ctx.leapfrog(12, c0, c1, c2)
         _, launches = ctx.timer_read()
         gu = ctx.download("u", R)
-    assert launches == 6 // spp
+    assert launches == 12 // spp
     if n <= 160:  # the oracle follows these sizes in seconds
-        want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 6)
+        want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 12)
         assert same_bits(gu, want_u)
