@@ -91,17 +91,19 @@ typedef struct {
  * iterators.cpp:42-89).  Zero fields select the built-in defaults. */
 typedef struct {
   uint32_t rows_per_wave; /* j rows held in registers by one wave: 1, 2, 4, 8
-                             (two- or three-step pass: 3 or 4; four-step: 2, 4, 5) */
+                             (two- or three-step pass: 3 or 4; four-step: 2, 4, 5;
+                             five-step: 2) */
   uint32_t waves;         /* waves per workgroup: 2, 4, 8 (two-step pass: 8, 12, 16;
-                             three-step pass: 12, 16; four-step: 8, 10, 16) */
+                             three-step pass: 12, 16; four-step: 8, 10, 16;
+                             five-step: 16) */
   uint32_t chunks;        /* i-range split: 1..64 chunks per (j,k) tile (0: chosen from
                              the grid and the device's CU count) */
   uint32_t xcd_remap;     /* 1: keep neighbouring tiles on one XCD (default), 0: off */
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
   uint32_t steps_per_pass; /* LEAPFROG, FP32: 2 = two steps per pass over memory
                               (temporal blocking), 3 = three (7 and 13 points),
-                              4 = four (7 points); steps left over run as one
-                              shorter pass each of three and two steps, then
+                              4 or 5 = four or five (7 points); steps left over
+                              run as the longest shorter pass that fits, then
                               one-step; 1 = one; 0: default */
   uint32_t launch;        /* one-step sweep: 1 = one workgroup per tile, 2 = persistent
                              workgroups drawing tiles from a work queue; 0: default (2) */
@@ -187,8 +189,8 @@ int sth_norms(sth_ctx *ctx, sth_field f, double *maxabs, double *l2);
 int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
 int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
 /* Times the candidate schedules on this context's geometry (REF or LEAPFROG
- * sweeps, 2*reps steps rounded up to a multiple of 12 each, fields are
- * modified; FP32 LEAPFROG also tries two, three and four steps per pass), keeps
+ * sweeps, 2*reps steps rounded up to a multiple of 60 each, fields are
+ * modified; FP32 LEAPFROG also tries two to five steps per pass), keeps
  * the fastest and returns it with its mean time
  * per step in ms.  If the environment variable
  * STH_TUNE_CACHE names a file, a result stored there for the same (stencil,

@@ -153,7 +153,7 @@ int main(int argc, char **argv) {
     args += get_int_arg(argv, iarg, argc, "--waves", waves, 1, 16, 1);
     args += get_int_arg(argv, iarg, argc, "--chunks", chunks, 1, 64, 1);
     args += get_int_arg(argv, iarg, argc, "--vec", vec, 1, 2, 1);
-    args += get_int_arg(argv, iarg, argc, "--steps-per-pass", spp, 1, 4, 1);
+    args += get_int_arg(argv, iarg, argc, "--steps-per-pass", spp, 1, 5, 1);
     args += get_int_arg(argv, iarg, argc, "--launch", launch, 1, 2, 1);
     if (args > 0) {
       iarg += args;

@@ -372,7 +372,7 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
 }
 
 // Several LEAPFROG steps per pass (FP32): the default on grids with enough
-// output tiles to fill the CUs (the thresholds below), four steps per pass
+// output tiles to fill the CUs (the thresholds below), five steps per pass
 // for 7 points, three for 13, two for 25; otherwise, or when the tuning asks
 // for one step per pass, the one-step sweep.
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
@@ -380,12 +380,13 @@ bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (t.steps_per_pass) return t.steps_per_pass >= 2;
   // Measured on cubic grids (FP32 LEAPFROG, one-step time / pass time per
   // step): 7 points, four-step pass 1.14-1.22 from 24^3 (1 tile) to 48^3,
-  // 1.21-3.36 from 64^3; 13 points, three-step pass 0.57-0.94 up to 9
-  // tiles, 1.11-2.61 from 12; 25 points, two-step pass 0.41-0.80 up to 42
-  // tiles, 0.98-1.18 from 56.  Longer passes were faster than shorter ones
-  // at every size (7 points: four steps 1.01-1.36x three; three steps
-  // 1.11-1.60x two).  Each chunk also recomputes (s-1)(2R+1) planes, so
-  // short grids stay one-step.
+  // 1.21-3.36 from 64^3, five-step pass 1.21-2.58 from 64^3 to 1040^3;
+  // 13 points, three-step pass 0.57-0.94 up to 9 tiles, 1.11-2.61 from 12;
+  // 25 points, two-step pass 0.41-0.80 up to 42 tiles, 0.98-1.18 from 56.
+  // Longer passes were faster than shorter ones nearly everywhere (7 points:
+  // five steps 1.03-1.16x four except 0.96 at 256^3, four 1.01-1.36x three;
+  // three steps 1.11-1.60x two).  Each chunk also recomputes (s-1)(2R+1)
+  // planes, so short grids stay one-step.
   const Plan2 p = make_plan2(c, t);
   const int npts = (int)c->g.npts;
   const int least = npts == 7 ? 1 : npts == 13 ? 12 : 50;
@@ -393,7 +394,7 @@ bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
 }
 
 // Steps per pass (the tuning's, where the stencil has that pass; by default
-// 4 for 7 points, 3 for 13, 2 for 25) and the fused shape (rows per wave,
+// 5 for 7 points, 3 for 13, 2 for 25) and the fused shape (rows per wave,
 // waves): a tuning that names a compiled shape gets it, otherwise the
 // default of the stencil and pass length.
 Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
@@ -401,14 +402,16 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   Plan2 p;
   if (npts == 25 || t.steps_per_pass == 2)
     p.s = 2;
-  else if (npts == 7 && (t.steps_per_pass == 4 || t.steps_per_pass == 0))
+  else if (npts == 7 && t.steps_per_pass == 4)
     p.s = 4;
+  else if (npts == 7 && (t.steps_per_pass == 5 || t.steps_per_pass == 0))
+    p.s = 5;
   else
     p.s = 3;
   if (fused2_supported(npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
     p.mj = (int)t.rows_per_wave;
     p.nw = (int)t.waves;
-  } else if (p.s == 4) {
+  } else if (p.s >= 4) {
     p.mj = 2;
     p.nw = 16;
   } else {
@@ -422,9 +425,10 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   p.ntj = ((int)c->g.ys + p.oj - 1) / p.oj;
   const int xs = (int)c->g.xs, tiles = p.ntk * p.ntj, R = c->R;
   // A chunk of n planes takes n + (s-1)(2R+1) steps and the kernel runs
-  // them in whole groups of ns = 2R+2 (s = 4: 6R+6): n = -(s-1)(2R+1) mod ns
-  // leaves no idle step (except in a shorter last chunk).
-  const int ns = (p.s == 4 ? 3 : 1) * (2 * R + 2), extra = (p.s - 1) * (2 * R + 1);
+  // them in whole groups of ns = lcm(2, s-1)(R+1) steps: n = -(s-1)(2R+1)
+  // mod ns leaves no idle step (except in a shorter last chunk).
+  const int ns = ((p.s - 1) % 2 ? 2 : 1) * (p.s - 1) * (R + 1),
+            extra = (p.s - 1) * (2 * R + 1);
   auto chunk_len = [&](int ch) {
     const int n = (xs + ch - 1) / ch;
     return std::min(xs, n + ((-extra - n) % ns + ns) % ns);
@@ -513,7 +517,8 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
   a.c2a = c2n ? c2n[0] : 0.0f;
   a.c2b = c2n ? c2n[1] : 0.0f;
   a.c2c = c2n && p.s >= 3 ? c2n[2] : 0.0f;
-  a.c2d = c2n && p.s == 4 ? c2n[3] : 0.0f;
+  a.c2d = c2n && p.s >= 4 ? c2n[3] : 0.0f;
+  a.c2e = c2n && p.s >= 5 ? c2n[4] : 0.0f;
   memcpy(a.w, wts.w, sizeof(a.w));
   const hipError_t e = launch_sweep2((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
@@ -540,8 +545,8 @@ int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, cons
         st = run_pass2(c, p2, a, c0, c1, c2 ? c2 + n : nullptr);
         if (st) return st;
       }
-      // steps left over: one shorter pass each of 3 (after four-step
-      // passes) and 2 steps, then one-step sweeps
+      // steps left over: the longest shorter pass that fits (down to two
+      // steps), then one-step sweeps
       for (int rs = p2.s - 1; rs >= 2; rs--) {
         if (n + (uint32_t)rs > steps) continue;
         sth_tuning t2 = c->tune;
@@ -1075,9 +1080,9 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 }
 
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
-  if (!c || !t || t->steps_per_pass > 4 || t->launch > 2 || t->chunks > 64) return STH_ERR_ARG;
+  if (!c || !t || t->steps_per_pass > 5 || t->launch > 2 || t->chunks > 64) return STH_ERR_ARG;
   if (t->steps_per_pass >= 2 && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
-  if ((t->steps_per_pass == 3 && c->g.npts == 25) || (t->steps_per_pass == 4 && c->g.npts != 7))
+  if ((t->steps_per_pass == 3 && c->g.npts == 25) || (t->steps_per_pass >= 4 && c->g.npts != 7))
     return STH_ERR_UNSUPPORTED;
   const bool fused_shape =
       t->steps_per_pass >= 2 && fused2_supported((int)c->g.npts, (int)t->rows_per_wave,
@@ -1130,7 +1135,7 @@ bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning 
                &xs, &ys, &zs, &mj, &nw, &vec, &ch, &spp, &m) != 13)
       continue;
     if (ver != 2 || npts != c->g.npts || dt != c->g.dtype || lf != (unsigned)leap ||
-        xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || spp < 1 || spp > 4)
+        xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || spp < 1 || spp > 5)
       continue;
     const bool ok = spp >= 2 ? (c->fmt == FMT_F32 && !c->no_fuse &&
                                 fused2_supported((int)npts, (int)mj, (int)nw, (int)spp))
@@ -1182,7 +1187,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   static const int chunk_opts[] = {0, 8, 16, 32};
   static const int fused_shapes[][3] = {{3, 16, 2}, {4, 16, 2}, {4, 12, 2}, {4, 8, 2},
                                         {3, 8, 2},  {3, 16, 3}, {4, 12, 3}, {4, 10, 4},
-                                        {5, 8, 4},  {2, 16, 4}};
+                                        {5, 8, 4},  {2, 16, 4}, {2, 16, 5}};
   static const int fused_chunks[] = {0, 2, 4, 8};
   const sth_tuning saved = c->tune;
   std::vector<sth_tuning> cand;
@@ -1217,9 +1222,9 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   c->timing = false;
   sth_tuning bt = saved;
   float bms = 1e30f;
-  // steps per timed run: 2 reps rounded up to a multiple of 12, so passes
-  // of two, three and four steps run whole
-  const uint32_t nsteps = (2 * reps + 11) / 12 * 12;
+  // steps per timed run: 2 reps rounded up to a multiple of 60, so passes
+  // of two to five steps run whole
+  const uint32_t nsteps = (2 * reps + 59) / 60 * 60;
   for (const sth_tuning &t : cand) {
     c->tune = t;
     const Plan p = make_plan(c, t);

@@ -8,7 +8,8 @@ namespace sth {
 #define STH_FUSED2_LIST(X)                                                                \
   X(7, 4, 16, 2) X(7, 4, 8, 2) X(13, 4, 16, 2) X(13, 3, 16, 2) X(13, 4, 8, 2) X(25, 3, 16, 2) \
   X(25, 4, 12, 2) X(25, 3, 8, 2) X(25, 4, 8, 2) X(7, 3, 16, 3) X(7, 4, 12, 3) \
-  X(13, 3, 16, 3) X(13, 4, 12, 3) X(7, 4, 10, 4) X(7, 5, 8, 4) X(7, 2, 16, 4)
+  X(13, 3, 16, 3) X(13, 4, 12, 3) X(7, 4, 10, 4) X(7, 5, 8, 4) X(7, 2, 16, 4) \
+  X(7, 2, 16, 5)
 
 bool fused2_supported(int npts, int mj, int nw, int steps) {
 #define STH_F2_SUP(N, MJ, NW, S) \

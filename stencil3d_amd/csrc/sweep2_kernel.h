@@ -42,7 +42,7 @@ struct Sweep2Args {
   int xs, ys, zs;
   int ntk, ntj, chunks, chunk_len, remap;
   float c0, c1;
-  float c2a, c2b, c2c, c2d;  // source coefficients of steps n .. n+3
+  float c2a, c2b, c2c, c2d, c2e;  // source coefficients of steps n .. n+4
   float w[12];     // LEAPFROG weights (2 * stencil weight), evaluation order
 };
 
@@ -78,6 +78,17 @@ __device__ __forceinline__ void static_for(Fn &&f) {
   static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
+// Length of the u_{n+j} register queue of an S-step pass, and its entry
+// at the plane of step j+1.
+template <int S, int R>
+__host__ __device__ constexpr int glen(int j) {
+  return j == S - 1 ? 2 * R + 2 : 2 * R + 3;
+}
+template <int S, int R>
+__host__ __device__ constexpr int gcen(int j) {
+  return glen<S, R>(j) - R - 2;
+}
+
 // One work item of a pass: output tile (tk, tj), i-chunk ch.  Called by the
 // single-grid kernel (one item per workgroup) and the batch kernel (items
 // of many grids drawn from a queue; LDS is reused between items).
@@ -98,22 +109,20 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   constexpr int EJ = F::EJ, LW = F::LW, OJ = F::OJ, OK = F::OK, WD = F::WD, SR = F::SR;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // u halo rows per wave
   constexpr int Q = 2 * R + 2;               // u queue: planes p-R-1 .. p+R
-  constexpr int WQ = S == 2 ? 2 * R + 2 : 2 * R + 3;  // u_{n+1} queue (+ the newest)
-  constexpr int WC = WQ - R - 2;             // u_{n+1} queue entry of plane i
-  constexpr int ZQ = S == 4 ? 2 * R + 3 : 2 * R + 2;  // u_{n+2} queue (S >= 3)
-  constexpr int ZC = ZQ - R - 2;             // u_{n+2} queue entry of step three's plane
-  constexpr int YQ = 2 * R + 2;              // u_{n+3} queue (S = 4)
-  constexpr int YC = YQ - R - 2;             // u_{n+3} queue entry of step four's plane
+  // Queue j (1 <= j < S) holds the lane's u_{n+j} values: 2R+2 entries for
+  // the last one, one more for the others (step j+2 takes its -u_{n+j} term
+  // from the oldest entry).  GL: the array length.
+  constexpr int GL = 2 * R + 3;
   // Steps per loop iteration: a multiple of the u queue length and of the
-  // s ring (S = 4: 3R+3 slots).
-  constexpr int U = S == 4 ? 3 * Q : Q;
-  static_assert(S >= 2 && S <= 4, "two to four steps per pass");
+  // s ring ((S-1)(R+1) slots).
+  constexpr int U = (S - 1) % 2 == 0 ? (S - 1) * (R + 1) : 2 * (S - 1) * (R + 1);
+  static_assert(S >= 2 && S <= 5, "two to five steps per pass");
   static_assert(U % SR == 0 && U % 2 == 0, "ring slots and buffers repeat with the loop");
   static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
   static_assert(F::LDS_BYTES <= 160 * 1024, "LDS");
-  // One LDS array: the two u planes (LU), the two u_{n+1} planes (W; S >= 3:
-  // then the two u_{n+2} planes; S = 4: the two u_{n+3} planes), then SR
-  // planes of s, each lane's own values (step m needs s(p-(m-1)(R+1))).
+  // One LDS array: the two u planes (LU), two planes each of u_{n+1} ..
+  // u_{n+S-1} (step j+1's j and k neighbours), then SR planes of s, each
+  // lane's own values (step m needs s(p-(m-1)(R+1))).
   __shared__ float SM[2 * F::LU + (2 * (S - 1) + SR) * F::LR];
 
   const int ib = ch * a.chunk_len;
@@ -163,9 +172,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   const bool kout = kin && lane >= WD && lane < WD + OK;
   uint32_t off[MJ];
   bool inner[MJ];
-  // row2 / row3: the row is one of step two's (the tile widened by (S-2)R:
-  // E1 less R on each side) / step three's (S = 4: E1 less 2R)
-  int rowout[MJ], row2[MJ], row3[MJ];
+  // rowst[M][m]: the row is one of step M's (2 <= M < S: the tile widened by
+  // (S-M)R, E1 less (M-1)R on each side)
+  int rowout[MJ], rowst[S][MJ];
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int er = wv * MJ + m, jg = je + er;
@@ -173,8 +182,10 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     off[m] = boff(jg, kg);
     inner[m] = jin && kin;
     rowout[m] = __builtin_amdgcn_readfirstlane((int)(jin && er >= WD && er < WD + OJ));
-    row2[m] = __builtin_amdgcn_readfirstlane((int)(er >= R && er < EJ - R));
-    row3[m] = __builtin_amdgcn_readfirstlane((int)(er >= 2 * R && er < EJ - 2 * R));
+#pragma unroll
+    for (int M = 0; M < S; M++)
+      rowst[M][m] =
+          __builtin_amdgcn_readfirstlane((int)(er >= (M - 1) * R && er < EJ - (M - 1) * R));
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
@@ -208,7 +219,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   //   lbS: s slot 0 at E1 row wv*MJ, column lane (offsets: slot, row m).
   using LF = __attribute__((address_space(3))) float;
   constexpr uint32_t LUB = F::LU * 4, LRB = F::LR * 4;
-  static_assert(2 * LUB <= 65536 && 2 * (S - 1) * LRB + (2 * R + MJ) * 256 <= 65536 &&
+  static_assert(2 * LUB <= 65536 && (2 * S - 3) * LRB + (2 * R + MJ) * 256 <= 65536 &&
                     (SR - 1) * LRB + MJ * 256 <= 65536,
                 "offsets");
   const uint32_t lu0 = (uint32_t)(uintptr_t)(LF *)&SM[0];
@@ -224,13 +235,12 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 
   // Register queues of the lane's points:
   //   q[m][t]  = u_n(p - R - 1 + t),      t = 0 .. 2R+1;
-  //   wq[m][t] = u_{n+1}(p - WQ + 1 + t), t = 0 .. WQ-2 (t = WQ-1 takes step
-  //              one's result at plane p before the shift);
-  //   zq[m][t] = u_{n+2}(i - ZQ + 1 + t), t = 0 .. ZQ-2 (S >= 3; t = ZQ-1
-  //              takes step two's result at plane i);
-  //   yq[m][t] = u_{n+3}(i3 - YQ + 1 + t), t = 0 .. YQ-2 (S = 4; t = YQ-1
-  //              takes step three's result at its plane i3 = i-R-1).
-  float q[MJ][Q], wq[MJ][WQ], zq[MJ][S >= 3 ? ZQ : 1], yq[MJ][S == 4 ? YQ : 1];
+  //   G[j][m][t] = u_{n+j}(p_j - L_j + 1 + t), t = 0 .. L_j-2, with p_j =
+  //              p-(j-1)(R+1) the plane of step j and L_j = glen(j) (t = L_j-1
+  //              takes step j's result at p_j before the shift).  Step j+1
+  //              reads entries gcen(j) -+ R (its plane p_{j+1} and i
+  //              neighbours), step j+2 entry 0 (u_{n+j} at p_{j+2}).
+  float q[MJ][Q], G[S][MJ][GL];
   float hq[HJ][R + 1], kq[R + 1];
   // Step one from p0 (its planes past ie-1+WD are never used), the last
   // step (output plane p - (S-1)(R+1)) up to plast.
@@ -245,11 +255,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   for (int m = 0; m < MJ; m++) {
     q[m][0] = 0.0f;
 #pragma unroll
-    for (int t = 0; t < WQ; t++) wq[m][t] = 0.0f;
+    for (int j = 1; j < S; j++)
 #pragma unroll
-    for (int t = 0; t < (S >= 3 ? ZQ : 1); t++) zq[m][t] = 0.0f;
-#pragma unroll
-    for (int t = 0; t < (S == 4 ? YQ : 1); t++) yq[m][t] = 0.0f;
+      for (int t = 0; t < glen<S, R>(j); t++) G[j][m][t] = 0.0f;
   }
 #pragma unroll
   for (int t = 0; t < R; t++) {
@@ -300,7 +308,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // register state depends on a skipped step.
   auto step = [&](int p, Staged &n, auto kc) __attribute__((always_inline)) {
     constexpr int k = decltype(kc)::value;
-    float vv[MJ], s1[MJ], s2c[MJ], s3c[MJ], s4c[MJ];
+    float vv[MJ], s1[MJ], sc[S + 1][MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       q[m][Q - 1] = n.u[m];
@@ -311,60 +319,46 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     for (int h = 0; h < HJ; h++) hq[h][R] = n.hj[h];
     kq[R] = n.hk;
     prefetch(Un, Vn, Sn, n);
-    Un = p + 3 + R <= last ? Un + PB : Un;
-    Vn = p + 3 <= last ? Vn + PB : Vn;
-    Sn = p + 3 <= last ? Sn + PB : Sn;
+    // Each pointer stays at its next plane clamped to [-R, last]: it moves
+    // on only while that plane is inside.  The first step of a chunk is WD
+    // planes below it, so the lower bound only binds for WD > R+2 (v, s)
+    // and WD > 2R+2 (u); elsewhere the test folds away.
+    constexpr bool low_v = WD > R + 2, low_u = WD > 2 * R + 2;
+    Un = ((!low_u || p + 2 + R >= -R) && p + 3 + R <= last) ? Un + PB : Un;
+    Vn = ((!low_v || p + 2 >= -R) && p + 3 <= last) ? Vn + PB : Vn;
+    Sn = ((!low_v || p + 2 >= -R) && p + 3 <= last) ? Sn + PB : Sn;
 
     asm volatile("" : "+v"(lbL), "+v"(lbW), "+v"(lbS));
     // The s ring of (S-1)(R+1) slots (the lane's own words, so no barrier is
-    // involved): step m of plane p-(m-1)(R+1) finds its s in slot
-    // k + (S-m)(R+1); the last step's s and s(p) share slot k, the old value
+    // involved): step M of plane p-(M-1)(R+1) finds its s in slot
+    // k + (S-M)(R+1); the last step's s and s(p) share slot k, the old value
     // read before the new one is stored.
-    constexpr uint32_t SO = (k % SR) * LRB;
-    constexpr uint32_t SO2 = ((k + (S - 2) * (R + 1)) % SR) * LRB;
-    constexpr uint32_t SO3 = ((k + (S - 3) * (R + 1)) % SR) * LRB;
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      if constexpr (S == 2) {
-        s2c[m] = lds(lbS + SO + m * 256);
-      } else if constexpr (S == 3) {
-        s3c[m] = lds(lbS + SO + m * 256);
-        s2c[m] = lds(lbS + SO2 + m * 256);
-      } else {
-        s4c[m] = lds(lbS + SO + m * 256);
-        s3c[m] = lds(lbS + SO3 + m * 256);
-        s2c[m] = lds(lbS + SO2 + m * 256);
-      }
-      lds(lbS + SO + m * 256) = s1[m];
+#pragma unroll
+      for (int M = S; M >= 2; M--)
+        sc[M][m] = lds(lbS + (uint32_t)(((k + (S - M) * (R + 1)) % SR) * LRB) + m * 256);
+      lds(lbS + (uint32_t)((k % SR) * LRB) + m * 256) = s1[m];
     }
     constexpr uint32_t LO = (k & 1) * LUB;  // this step's u buffer
-    constexpr uint32_t WO = (k & 1) * LRB;  // this step's u_{n+1} buffer
-    constexpr uint32_t ZO = 2 * LRB + (k & 1) * LRB;  // S >= 3: its u_{n+2} buffer
-    constexpr uint32_t YO = 4 * LRB + (k & 1) * LRB;  // S = 4: its u_{n+3} buffer
     // LU element (row m+R+dr, column R+dc of this lane's rows)
     auto Lat = [&](int m, int dr, int dc) -> LF & {
       return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + R + dc) * 4));
     };
-    // W element (E1 row of row m shifted by dr, column lane+dc)
-    auto Wat = [&](int m, int dr, int dc) -> LF & {
-      return lds(lbW + WO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
-    };
-    auto Zat = [&](int m, int dr, int dc) -> LF & {
-      return lds(lbW + ZO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
-    };
-    auto Yat = [&](int m, int dr, int dc) -> LF & {
-      return lds(lbW + YO + (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
+    // This step's buffer of u_{n+j} (E1 row of row m shifted by dr, column
+    // lane+dc)
+    auto Gat = [&](int j, int m, int dr, int dc) -> LF & {
+      return lds(lbW + (uint32_t)((2 * (j - 1) + (k & 1)) * LRB) +
+                 (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
     };
 
-    // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns;
-    // u_{n+1}(p-R-1) of the E1 rows (S >= 3: and u_{n+2}(p-2R-2); S = 4:
-    // and u_{n+3}(p-3R-3)).
+    // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns; u_{n+j}
+    // at the plane of step j+1 (p-j(R+1)), j = 1 .. S-1, of the E1 rows.
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       Lat(m, 0, 0) = q[m][R + 1];
-      Wat(m, 0, 0) = wq[m][WC];
-      if constexpr (S >= 3) Zat(m, 0, 0) = zq[m][ZC];
-      if constexpr (S == 4) Yat(m, 0, 0) = yq[m][YC];
+#pragma unroll
+      for (int j = 1; j < S; j++) Gat(j, m, 0, 0) = G[j][m][gcen<S, R>(j)];
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++)
@@ -372,139 +366,89 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     if (has_kh) lds(kadr + LO) = kq[0];
     __syncthreads();
 
+    // u_{n+M-2} at step M's plane: u_n from the u queue, else the oldest
+    // entry of queue M-2.
+    auto older = [&](auto jc, int m) __attribute__((always_inline)) -> float {
+      constexpr int J = decltype(jc)::value;
+      if constexpr (J == 0)
+        return q[m][0];
+      else
+        return G[J][m][0];
+    };
+    auto c2of = [&](int M) -> float {
+      return M == 2 ? a.c2b : M == 3 ? a.c2c : M == 4 ? a.c2d : a.c2e;
+    };
+
     // The last step writes plane io = p-(S-1)(R+1) of the output tile: the
     // two newest fields there.  Rows without output points (a wave-uniform
     // condition) are skipped.
-    const int i = p - R - 1;
     const int io = p - (S - 1) * (R + 1);
     const bool store = io >= ib && io < ie;
     const BufRes Wr = buf_res(Wo), Zr = buf_res(Zo);
-    if constexpr (S == 2) {
-      // Step two at plane i: u_{n+2}.
 #pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        if (!rowout[m]) continue;
-        auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return wq[m][WC - r] + wq[m][WC + r];
-          if (ax == 2) return Wat(m, 0, -r) + Wat(m, 0, r);
-          const float lo = (m - r >= 0) ? wq[(m - r) >= 0 ? m - r : 0][WC] : Wat(m, -r, 0);
-          const float hi = (m + r < MJ) ? wq[(m + r) < MJ ? m + r : 0][WC] : Wat(m, r, 0);
-          return lo + hi;
-        };
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+    for (int m = 0; m < MJ; m++) {
+      if (!rowout[m]) continue;
+      constexpr int J = S - 1, C = gcen<S, R>(J);
+      auto pair = [&](int ax, int r) -> float {
+        if (ax == 0) return G[J][m][C - r] + G[J][m][C + r];
+        if (ax == 2) return Gat(J, m, 0, -r) + Gat(J, m, 0, r);
+        const float lo = (m - r >= 0) ? G[J][(m - r) >= 0 ? m - r : 0][C] : Gat(J, m, -r, 0);
+        const float hi = (m + r < MJ) ? G[J][(m + r) < MJ ? m + r : 0][C] : Gat(J, m, r, 0);
+        return lo + hi;
+      };
+      float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
-        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float wc = wq[m][WC];
-        const float z = __builtin_fmaf(a.c2b, s2c[m],
-                                       __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
-        if (store && kout) {
-          buf_store<float, 2>(wc, Wr, off[m]);
-          buf_store<float, 2>(z, Zr, off[m]);
-        }
-      }
-    } else if constexpr (S == 3) {
-      // Step three at plane io = i-R-1: u_{n+3} on the output tile from the
-      // u_{n+2} queue (i neighbours, in-wave j neighbours) and LDS; the
-      // -u_{n+1} term is the oldest u_{n+1} entry.
-#pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        if (!rowout[m]) continue;
-        auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return zq[m][ZC - r] + zq[m][ZC + r];
-          if (ax == 2) return Zat(m, 0, -r) + Zat(m, 0, r);
-          const float lo = (m - r >= 0) ? zq[(m - r) >= 0 ? m - r : 0][ZC] : Zat(m, -r, 0);
-          const float hi = (m + r < MJ) ? zq[(m + r) < MJ ? m + r : 0][ZC] : Zat(m, r, 0);
-          return lo + hi;
-        };
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
-#pragma unroll
-        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float zc = zq[m][ZC];
-        const float z3 = __builtin_fmaf(
-            a.c2c, s3c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, zc, -wq[m][0])));
-        if (store && kout) {
-          buf_store<float, 2>(zc, Wr, off[m]);
-          buf_store<float, 2>(z3, Zr, off[m]);
-        }
-      }
-    } else {
-      // Step four at plane io = i-2R-2: u_{n+4} on the output tile from the
-      // u_{n+3} queue and LDS; the -u_{n+2} term is the oldest u_{n+2} entry.
-#pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        if (!rowout[m]) continue;
-        auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return yq[m][YC - r] + yq[m][YC + r];
-          if (ax == 2) return Yat(m, 0, -r) + Yat(m, 0, r);
-          const float lo = (m - r >= 0) ? yq[(m - r) >= 0 ? m - r : 0][YC] : Yat(m, -r, 0);
-          const float hi = (m + r < MJ) ? yq[(m + r) < MJ ? m + r : 0][YC] : Yat(m, r, 0);
-          return lo + hi;
-        };
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
-#pragma unroll
-        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float yc = yq[m][YC];
-        const float y4 = __builtin_fmaf(
-            a.c2d, s4c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, yc, -zq[m][0])));
-        if (store && kout) {
-          buf_store<float, 2>(yc, Wr, off[m]);
-          buf_store<float, 2>(y4, Zr, off[m]);
-        }
-      }
-      // Step three at plane i3 = i-R-1: u_{n+3} on the tile widened by R,
-      // into the newest u_{n+3} entry; the -u_{n+1} term is the oldest u_{n+1}
-      // entry.  Outside the interior u_{n+3} takes u_{n+1}'s value there (the
-      // one-step scheme writes u_{n+3} into u_{n+1}'s buffer).
-      const bool pin3 = i - R - 1 >= 0 && i - R - 1 < a.xs;
-#pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        if (!row3[m]) {
-          yq[m][YQ - 1] = wq[m][0];
-          continue;
-        }
-        auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return zq[m][ZC - r] + zq[m][ZC + r];
-          if (ax == 2) return Zat(m, 0, -r) + Zat(m, 0, r);
-          const float lo = (m - r >= 0) ? zq[(m - r) >= 0 ? m - r : 0][ZC] : Zat(m, -r, 0);
-          const float hi = (m + r < MJ) ? zq[(m + r) < MJ ? m + r : 0][ZC] : Zat(m, r, 0);
-          return lo + hi;
-        };
-        float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
-#pragma unroll
-        for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float zc = zq[m][ZC];
-        const float y3 = __builtin_fmaf(
-            a.c2c, s3c[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, zc, -wq[m][0])));
-        yq[m][YQ - 1] = (inner[m] && pin3) ? y3 : wq[m][0];
+      for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+      const float cen = G[J][m][C];
+      const float z = __builtin_fmaf(
+          c2of(S), sc[S][m],
+          __builtin_fmaf(a.c1, Lp,
+                         __builtin_fmaf(a.c0, cen, -older(std::integral_constant<int, J - 1>{}, m))));
+      if (store && kout) {
+        buf_store<float, 2>(cen, Wr, off[m]);
+        buf_store<float, 2>(z, Zr, off[m]);
       }
     }
+    // Steps S-1 .. 2, each into the newest entry of its queue.  Outside the
+    // interior u_{n+M} takes u_{n+M-2}'s value there (the one-step scheme
+    // writes u_{n+M} into u_{n+M-2}'s buffer, whose halo no sweep writes).
+    // (Only instantiated for S >= 3: an unused closure over the queues
+    // changed the register allocation of the two-step pass.)
     if constexpr (S >= 3) {
-      // Step two at plane i: u_{n+2} on the tile widened by (S-2)R, into the
-      // newest u_{n+2} entry.  Outside the interior u_{n+2} takes u_n's value
-      // there (the one-step scheme writes u_{n+2} into u_n's buffer, whose
-      // halo no sweep writes).
-      const bool pin2 = i >= 0 && i < a.xs;
-#pragma unroll
-      for (int m = 0; m < MJ; m++) {
-        if (!row2[m]) {
-          zq[m][ZQ - 1] = q[m][0];
-          continue;
-        }
+      // Step M (M >= 2) at plane p-(M-1)(R+1) of row m: the stencil of u_{n+M-1}
+      // (queue M-1: i neighbours and in-wave j neighbours; LDS: the others).
+      auto lap = [&](auto jc, int m) __attribute__((always_inline)) {
+        constexpr int J = decltype(jc)::value, C = gcen<S, R>(J);
         auto pair = [&](int ax, int r) -> float {
-          if (ax == 0) return wq[m][WC - r] + wq[m][WC + r];
-          if (ax == 2) return Wat(m, 0, -r) + Wat(m, 0, r);
-          const float lo = (m - r >= 0) ? wq[(m - r) >= 0 ? m - r : 0][WC] : Wat(m, -r, 0);
-          const float hi = (m + r < MJ) ? wq[(m + r) < MJ ? m + r : 0][WC] : Wat(m, r, 0);
+          if (ax == 0) return G[J][m][C - r] + G[J][m][C + r];
+          if (ax == 2) return Gat(J, m, 0, -r) + Gat(J, m, 0, r);
+          const float lo = (m - r >= 0) ? G[J][(m - r) >= 0 ? m - r : 0][C] : Gat(J, m, -r, 0);
+          const float hi = (m + r < MJ) ? G[J][(m + r) < MJ ? m + r : 0][C] : Gat(J, m, r, 0);
           return lo + hi;
         };
         float Lp = a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
         for (int t = 1; t < T; t++) Lp = __builtin_fmaf(a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
-        const float wc = wq[m][WC];
-        const float z2 = __builtin_fmaf(a.c2b, s2c[m],
-                                        __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, wc, -q[m][0])));
-        zq[m][ZQ - 1] = (inner[m] && pin2) ? z2 : q[m][0];
-      }
+        return Lp;
+      };
+      static_for<S - 2>([&](auto ic) __attribute__((always_inline)) {
+        constexpr int M = S - 1 - decltype(ic)::value;  // S-1 .. 2
+        const int pm = p - (M - 1) * (R + 1);
+        const bool pinm = pm >= 0 && pm < a.xs;
+#pragma unroll
+        for (int m = 0; m < MJ; m++) {
+          const float old = older(std::integral_constant<int, M - 2>{}, m);
+          if (!rowst[M][m]) {
+            G[M][m][glen<S, R>(M) - 1] = old;
+            continue;
+          }
+          const float Lp = lap(std::integral_constant<int, M - 1>{}, m);
+          const float cen = G[M - 1][m][gcen<S, R>(M - 1)];
+          const float z = __builtin_fmaf(c2of(M), sc[M][m],
+                                         __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, cen, -old)));
+          G[M][m][glen<S, R>(M) - 1] = (inner[m] && pinm) ? z : old;
+        }
+      });
     }
     if (store) {
       Wo += PB;
@@ -528,7 +472,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
       const float u0 = q[m][R + 1];
       const float w1 =
           __builtin_fmaf(a.c2a, s1[m], __builtin_fmaf(a.c1, Lp, __builtin_fmaf(a.c0, u0, -vv[m])));
-      wq[m][WQ - 1] = (inner[m] && pin) ? w1 : vv[m];
+      G[1][m][glen<S, R>(1) - 1] = (inner[m] && pin) ? w1 : vv[m];
     }
 
 #pragma unroll
@@ -536,15 +480,9 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 #pragma unroll
       for (int t = 0; t < Q - 1; t++) q[m][t] = q[m][t + 1];
 #pragma unroll
-      for (int t = 0; t < WQ - 1; t++) wq[m][t] = wq[m][t + 1];
-      if constexpr (S >= 3) {
+      for (int j = 1; j < S; j++)
 #pragma unroll
-        for (int t = 0; t < ZQ - 1; t++) zq[m][t] = zq[m][t + 1];
-      }
-      if constexpr (S == 4) {
-#pragma unroll
-        for (int t = 0; t < YQ - 1; t++) yq[m][t] = yq[m][t + 1];
-      }
+        for (int t = 0; t < glen<S, R>(j) - 1; t++) G[j][m][t] = G[j][m][t + 1];
     }
 #pragma unroll
     for (int t = 0; t < R; t++) {
@@ -640,6 +578,7 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.c2b = b.c2b;
     a.c2c = b.c2c;
     a.c2d = 0.0f;
+    a.c2e = 0.0f;
 #pragma unroll
     for (int t = 0; t < 12; t++) a.w[t] = b.w[t];
     sweep2_tile<NPTS, MJ, NW, S>(a, it.tk, it.tj, it.ch);

@@ -4,7 +4,7 @@ oracle cannot follow: size-independent checks only.
   * The two-steps-per-pass schedule (the 25-point FP32 LEAPFROG default) and
     the one-step sweep give the same bits for u and v after six steps (hashes
     of the downloaded fields, halos included), with a nonzero source term;
-    the same for the defaults of 7 and 13 points (four and three steps per
+    the same for the defaults of 7 and 13 points (five and three steps per
     pass).
   * The reference's own known-answer check (u = s = 1, v = 0; each REF sweep
     adds the stencil's constant): every interior point within 0.1 of the
@@ -67,7 +67,7 @@ def test_longer_passes_equal_one_step_at_headline_size(gpu, npts):
             ctx.timer_start()
             ctx.leapfrog(7, c0, c1, c2)
             _, launches = ctx.timer_read()
-            # default: 7 points a four- and a three-step pass; 13 points two
+            # default: 7 points a five- and a two-step pass; 13 points two
             # three-step passes and a sweep
             assert launches == ({7: 2, 13: 3}[npts] if spp == 0 else 7)
             hashes[spp] = (field_hash(ctx, "u", R, buf), field_hash(ctx, "v", R, buf))

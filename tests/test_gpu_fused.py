@@ -5,7 +5,7 @@ one-step scheme never writes and which a pass must leave where the one-step
 scheme leaves them.  Covered: every compiled fused shape, i-chunks, odd step
 counts (the last step runs one-step), one-step calls and uploads between
 calls, and shapes that are not multiples of the output tile; the same for
-three steps per pass (7 and 13 points) and four (7 points)."""
+three steps per pass (7 and 13 points) and four and five (7 points)."""
 import numpy as np
 import pytest
 
@@ -240,6 +240,35 @@ def test_four_step_passes_bit_exact(gpu, orc, shape):
             assert same_bits(gv, want[steps][1]), (mj, nw, chunks, steps)
 
 
+@pytest.mark.parametrize("shape", SHAPES)
+def test_five_step_passes_bit_exact(gpu, orc, shape):
+    """Five steps per pass (7 points): 10 steps are two passes; 11 to 14 add
+    a one-step sweep or a two-, three- or four-step pass; with i-chunks."""
+    npts = 7
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 19, 0.375)
+    c0, c1 = coeffs(orc, npts)
+    c2 = (np.arange(1, 15, dtype=np.float32) * np.float32(-4e-4)).astype(np.float32)
+    want = {n: oracle_run(orc, npts, g, u, v, s, c0, c1, c2[:n], n) for n in range(10, 15)}
+    for chunks, steps in ((0, 10), (2, 11), (3, 12), (0, 13), (2, 14)):
+        with gpu.Stencil(*shape, npts=npts) as ctx:
+            ctx.set_tuning(2, 16, chunks, steps_per_pass=5)
+            t = ctx.get_tuning()
+            assert t["steps_per_pass"] == 5 and (t["rows_per_wave"], t["waves"]) == (2, 16)
+            ctx.upload("u", u, R)
+            ctx.upload("v", v, R)
+            ctx.upload("s", s, R)
+            ctx.timer_start()
+            ctx.leapfrog(steps, c0, c1, c2[:steps])
+            _, launches = ctx.timer_read()
+            assert launches == (2 if steps == 10 else 3)
+            gu = ctx.download("u", R)
+            gv = ctx.download("v", R)
+        assert same_bits(gu, want[steps][0]), (chunks, steps)
+        assert same_bits(gv, want[steps][1]), (chunks, steps)
+
+
 def test_four_step_passes_mixed_with_other_schedules(gpu, orc):
     """A four-step pass, a three-step pass, one one-step sweep, a new v, then
     five steps of four-step tuning (a pass and a sweep)."""
@@ -294,14 +323,18 @@ def test_four_step_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
     assert same_bits(out[0][1], out[1][1])
 
 
-def test_four_steps_only_for_7_points(gpu):
+@pytest.mark.parametrize("spp", [4, 5])
+def test_four_and_five_steps_only_for_7_points(gpu, spp):
     for npts in (13, 25):
         with gpu.Stencil(16, 16, 64, npts=npts) as ctx:
             with pytest.raises(gpu.SthError):
-                ctx.set_tuning(steps_per_pass=4)
+                ctx.set_tuning(steps_per_pass=spp)
     with gpu.Stencil(16, 16, 64, npts=7, dtype="fp16") as ctx:
         with pytest.raises(gpu.SthError):
-            ctx.set_tuning(steps_per_pass=4)
+            ctx.set_tuning(steps_per_pass=spp)
+    with gpu.Stencil(16, 16, 64, npts=7) as ctx:
+        with pytest.raises(gpu.SthError):
+            ctx.set_tuning(steps_per_pass=6)
 
 
 def test_three_steps_rejected_for_25_points_and_16bit(gpu):
@@ -319,14 +352,14 @@ def test_two_steps_rejected_for_16bit(gpu):
             ctx.set_tuning(steps_per_pass=2)
 
 
-@pytest.mark.parametrize("npts,n,spp", [(7, 12, 1), (7, 48, 4), (13, 112, 1), (13, 128, 3),
+@pytest.mark.parametrize("npts,n,spp", [(7, 12, 1), (7, 48, 5), (13, 112, 1), (13, 128, 3),
                                          (25, 256, 1), (25, 320, 2)])
 def test_default_schedule_by_grid_size(gpu, orc, npts, n, spp):
     """The default FP32 LEAPFROG schedule takes several steps per pass where
     that measured faster (enough output tiles and planes for the stencil;
-    four steps for 7 points, three for 13, two for 25): twelve steps are
-    three to six passes or twelve sweeps, and either way bit-identical to
-    the oracle."""
+    five steps for 7 points, three for 13, two for 25): twelve steps are
+    passes (the remainder one shorter pass) or twelve sweeps, and either way
+    bit-identical to the oracle."""
     R = orc.radius(npts)
     shape = (n, n, n)
     g = orc.Grid(*shape, R, R, R)
@@ -342,7 +375,7 @@ def test_default_schedule_by_grid_size(gpu, orc, npts, n, spp):
         ctx.leapfrog(12, c0, c1, c2)
         _, launches = ctx.timer_read()
         gu = ctx.download("u", R)
-    assert launches == 12 // spp
+    assert launches == 12 // spp + (1 if 12 % spp else 0)
     if n <= 160:  # the oracle follows these sizes in seconds
         want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 12)
         assert same_bits(gu, want_u)

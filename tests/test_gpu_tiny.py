@@ -34,7 +34,7 @@ def test_tiny_fp32(gpu, orc, npts, shape):
     I = g.interior
     for tuning in ({"launch": 1, "steps_per_pass": 1}, {"launch": 2, "steps_per_pass": 1},
                    {"steps_per_pass": 2}) + (({"steps_per_pass": 3},) if npts != 25 else ()) \
-            + (({"steps_per_pass": 4},) if npts == 7 else ()):
+            + (({"steps_per_pass": 4}, {"steps_per_pass": 5}) if npts == 7 else ()):
         with gpu.Stencil(*shape, npts=npts) as ctx:
             ctx.set_tuning(**tuning)
             for f, x in (("u", u), ("v", v), ("s", s)):
