@@ -311,7 +311,7 @@ def main():
                                                              copy_bytes=copy_bytes)
     copy_gbs = 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9
     gpts = pts * K / wall / 1e9
-    # A launch advances `units` steps (2 or 3 for a fused pass):
+    # A launch advances `units` steps (2 to 5 for a fused pass):
     # algorithmic bytes per launch = 16 B x points x units.
     units = K / launches
     per_launch_s = kms / launches * 1e-3

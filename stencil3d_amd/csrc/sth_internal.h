@@ -90,7 +90,7 @@ struct Plan {
   unsigned grid;
 };
 
-// Launch plan of a pass of s (2 or 3) LEAPFROG steps.
+// Launch plan of a pass of s (2 to 5) LEAPFROG steps.
 struct Plan2 {
   int s, mj, nw, oj, ok, chunks, chunk_len, ntk, ntj;
   unsigned grid;

@@ -1,4 +1,4 @@
-// sweep2.hip -- launches of the fused LEAPFROG sweep (FP32, two or three steps per pass).
+// sweep2.hip -- launches of the fused LEAPFROG sweep (FP32, two to five steps per pass).
 #include "sweep2_kernel.h"
 #include "sweep2_dispatch.h"
 

@@ -1,4 +1,4 @@
-// sweep2_dispatch.h -- host-side entry to the fused sweep (two or three steps per pass).
+// sweep2_dispatch.h -- host-side entry to the fused sweep (two to five steps per pass).
 #pragma once
 
 #include <hip/hip_runtime.h>

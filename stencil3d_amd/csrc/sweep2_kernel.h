@@ -1,11 +1,11 @@
-// sweep2_kernel.h -- two or three LEAPFROG steps per pass over memory (FP32).
+// sweep2_kernel.h -- two to five LEAPFROG steps per pass over memory (FP32).
 //
 // The one-step sweep moves 16 B per point-update (read u, v, s; write v)
 // and on MI355X sits at the read/write mix bandwidth of HBM.  A pass of this
 // kernel reads u_n (with its halo), u_{n-1} and s once and writes the two
-// newest fields: 20 B per two (or three) point-updates.  The two-step form
-// is described here; the three-step form (S = 3) adds one stage, described
-// at sweep2_tile.
+// newest fields: 20 B per S point-updates (S = 2 .. 5).  The two-step form
+// is described here; each longer pass adds one stage, described at
+// sweep2_tile.
 //
 //  * A workgroup owns an output tile of OJ x OK (j, k) points.  Step one
 //    (u_{n+1}) runs on the tile widened by R on every side (E1: EJ rows x 64
@@ -93,13 +93,12 @@ __host__ __device__ constexpr int gcen(int j) {
 // single-grid kernel (one item per workgroup) and the batch kernel (items
 // of many grids drawn from a queue; LDS is reused between items).
 //
-// S = 3 adds a third step: step two runs on the tile widened by R (E2) at
-// plane i = p-R-1, its u_{n+2} values go to a third register queue and, for
-// the plane i2 = i-R-1 of step three, to LDS; step three writes u_{n+2} and
-// u_{n+3} of the output tile at plane i2.  S = 4 adds a fourth in the same
-// way: step two on the tile widened by 2R, step three on the tile widened
-// by R at plane i-R-1 (a fourth register queue and LDS plane), step four
-// on the output tile at plane i-2R-2, writing u_{n+3} and u_{n+4}.
+// A pass of S steps: step one runs on the output tile widened by (S-1)R
+// (E1) at plane p; step M (2 <= M <= S) on the tile widened by (S-M)R at
+// plane p-(M-1)(R+1), from the register queue of step M-1's values (its i
+// neighbours and in-wave j neighbours) and an LDS plane of them (the other
+// j and k neighbours), into a register queue of its own; the last step
+// writes u_{n+S-1} and u_{n+S} of the output tile.
 template <int NPTS, int MJ, int NW, int S = 2>
 __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, const int tj,
                                             const int ch) {
