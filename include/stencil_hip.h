@@ -226,6 +226,9 @@ typedef struct sth_batch sth_batch;
 #define STH_BATCH_THREE_STEP 3 /* 7 and 13 points: one persistent launch per three steps
                                   (the one or two steps left over run as
                                   STH_BATCH_PERSISTENT); 25 points: STH_ERR_UNSUPPORTED */
+#define STH_BATCH_FUSED 4      /* one persistent launch per pass of the stencil's single-grid
+                                  default length (7 points: 5 steps, 13: 3, 25: 2); steps
+                                  left over run as STH_BATCH_PERSISTENT */
 #define STH_BATCH_GRAPH 0x100  /* flag: capture the call's time loop in one hipGraph */
 
 int sth_batch_create(const uint32_t *dims, uint32_t nshots, uint32_t npts, uint32_t dtype,
@@ -244,7 +247,8 @@ int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,
                          uint32_t *grid_blocks);
 /* `steps` LEAPFROG steps of every shot (c2: per-step source coefficients,
  * NULL = 0); method = STH_BATCH_PERSISTENT, STH_BATCH_SEQUENTIAL,
- * STH_BATCH_TWO_STEP or STH_BATCH_THREE_STEP, optionally | STH_BATCH_GRAPH.
+ * STH_BATCH_TWO_STEP, STH_BATCH_THREE_STEP or STH_BATCH_FUSED, optionally |
+ * STH_BATCH_GRAPH.
  * The fused methods allocate two more buffers per shot on first use (the
  * twins of u and v).  Asynchronous. */
 int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2,

@@ -111,7 +111,8 @@ SIGNATURES = {
     "sth_batch_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
 }
 
-BATCH_PERSISTENT, BATCH_SEQUENTIAL, BATCH_TWO_STEP, BATCH_THREE_STEP, BATCH_GRAPH = 0, 1, 2, 3, 0x100
+BATCH_PERSISTENT, BATCH_SEQUENTIAL, BATCH_TWO_STEP, BATCH_THREE_STEP, BATCH_FUSED = 0, 1, 2, 3, 4
+BATCH_GRAPH = 0x100
 
 _lib = None
 

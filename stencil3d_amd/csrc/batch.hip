@@ -8,7 +8,7 @@
 //    queue (an atomic ticket counter), largest items first, and runs each with
 //    the same tile engine as the single-grid sweep (sweep_tile).
 //  * sequential: one plain sweep launch per shot per step.
-//  * two or three steps per pass: one persistent launch per pass, items of
+//  * two to five steps per pass: one persistent launch per pass, items of
 //    the fused sweep (sweep2_tile) of every shot from a queue.
 // Either way the time loop can be captured into one hipGraph per call.
 #include <hip/hip_runtime.h>
@@ -235,7 +235,7 @@ int build_tables(sth_batch *b) {
 // The fused kernel's shape for S steps per pass: the stencil's default
 // (rows per wave, waves).
 void shape2(uint32_t npts, int S, int &mj, int &nw) {
-  mj = npts == 25 || S == 3 ? 3 : 4;
+  mj = S >= 4 ? 2 : npts == 25 || S == 3 ? 3 : 4;
   nw = 16;
 }
 
@@ -358,7 +358,9 @@ int enqueue_passes(sth_batch *b, int S, uint32_t passes, float c0, float c1, con
     const float *cq = c2 ? c2 + (size_t)S * q : nullptr;
     a.c2a = cq ? cq[0] : 0.0f;
     a.c2b = cq ? cq[1] : 0.0f;
-    a.c2c = cq && S == 3 ? cq[2] : 0.0f;
+    a.c2c = cq && S >= 3 ? cq[2] : 0.0f;
+    a.c2d = cq && S >= 4 ? cq[3] : 0.0f;
+    a.c2e = cq && S >= 5 ? cq[4] : 0.0f;
     STH_TRY(launch_batch2((int)b->npts, mj, nw, S, G, b->stream, a));
     b->ticket_base += per_launch;
     (*launches)++;
@@ -559,12 +561,16 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
   const int base = (int)(method & 0xff);
   const bool graph = (method & STH_BATCH_GRAPH) != 0;
   if (base != STH_BATCH_PERSISTENT && base != STH_BATCH_SEQUENTIAL &&
-      base != STH_BATCH_TWO_STEP && base != STH_BATCH_THREE_STEP)
+      base != STH_BATCH_TWO_STEP && base != STH_BATCH_THREE_STEP && base != STH_BATCH_FUSED)
     return STH_ERR_ARG;
   if (base == STH_BATCH_THREE_STEP && b->npts == 25) return STH_ERR_UNSUPPORTED;
   if (steps == 0) return STH_OK;
   const int parity0 = b->shots[0]->ubuf;
-  const int S = base == STH_BATCH_TWO_STEP ? 2 : base == STH_BATCH_THREE_STEP ? 3 : 0;
+  // STH_BATCH_FUSED: the single-grid default pass of the stencil
+  const int S = base == STH_BATCH_TWO_STEP     ? 2
+                : base == STH_BATCH_THREE_STEP ? 3
+                : base == STH_BATCH_FUSED      ? (b->npts == 7 ? 5 : b->npts == 13 ? 3 : 2)
+                                               : 0;
   const uint32_t passes = S ? steps / (uint32_t)S : 0;
   int st = STH_OK;
   if (passes) {

@@ -42,7 +42,7 @@ hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hip
 
 // (stencil, rows per wave, waves, steps per pass)
 #define STH_BATCH2_LIST(X) \
-  X(7, 4, 16, 2) X(13, 4, 16, 2) X(25, 3, 16, 2) X(7, 3, 16, 3) X(13, 3, 16, 3)
+  X(7, 4, 16, 2) X(13, 4, 16, 2) X(25, 3, 16, 2) X(7, 3, 16, 3) X(13, 3, 16, 3) X(7, 2, 16, 5)
 
 bool batch2_supported(int npts, int mj, int nw, int steps) {
 #define STH_B2_SUP(N, MJ, NW, S) \

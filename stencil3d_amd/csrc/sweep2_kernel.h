@@ -534,7 +534,7 @@ struct Batch2Args {
   unsigned ticket_base;  // tickets taken by earlier launches since the last reset
   int ub;                // buffer of u: 0 = a, 1 = b
   int phase;             // 0: read the bases, write the twins; 1: the reverse
-  float c0, c1, c2a, c2b, c2c;
+  float c0, c1, c2a, c2b, c2c, c2d, c2e;
   float w[12];
 };
 
@@ -557,10 +557,10 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.v = sh.buf[vb + in];
     a.s = sh.s;
     // the two newest fields go to the twins of the buffers the one-step
-    // scheme writes them to: S = 2, u_{n+1} over v and u_{n+2} over u;
-    // S = 3, u_{n+2} over u and u_{n+3} over v
-    a.o1 = const_cast<void *>(sh.buf[(S == 2 ? vb : b.ub) + out]);
-    a.o2 = const_cast<void *>(sh.buf[(S == 2 ? b.ub : vb) + out]);
+    // scheme writes them to: S even, u_{n+S-1} over v and u_{n+S} over u;
+    // S odd, u_{n+S-1} over u and u_{n+S} over v
+    a.o1 = const_cast<void *>(sh.buf[(S % 2 == 0 ? vb : b.ub) + out]);
+    a.o2 = const_cast<void *>(sh.buf[(S % 2 == 0 ? b.ub : vb) + out]);
     a.pitch = sh.pitch;
     a.plane = sh.plane;
     a.xs = sh.xs;
@@ -576,8 +576,8 @@ __global__ __launch_bounds__(64 * NW) void batch2_kernel(const Batch2Args b) {
     a.c2a = b.c2a;
     a.c2b = b.c2b;
     a.c2c = b.c2c;
-    a.c2d = 0.0f;
-    a.c2e = 0.0f;
+    a.c2d = b.c2d;
+    a.c2e = b.c2e;
 #pragma unroll
     for (int t = 0; t < 12; t++) a.w[t] = b.w[t];
     sweep2_tile<NPTS, MJ, NW, S>(a, it.tk, it.tj, it.ch);

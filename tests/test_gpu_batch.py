@@ -196,6 +196,32 @@ def test_three_step_batch_matches_oracle(gpu, orc, graph, steps, npts):
         assert np.array_equal(got[q][1].view(np.uint32), wv.view(np.uint32)), q
 
 
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("npts,spp", [(7, 5), (13, 3), (25, 2)])
+def test_fused_batch_matches_oracle(gpu, orc, graph, npts, spp):
+    """STH_BATCH_FUSED: passes of the stencil's single-grid default length
+    over all shots, the steps left over one-step; bit-identical to the
+    oracle, halos included."""
+    steps = 12
+    R = orc.radius(npts)
+    fields = shot_fields(orc, DIMS, npts, 70)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-2, 1e-2, steps).astype(np.float32)
+    with gpu.Batch(DIMS, npts=npts) as bt:
+        load(bt, fields, R)
+        bt.timer_start()
+        bt.leapfrog(steps, c0, c1, c2, method=gpu.BATCH_FUSED, graph=graph)
+        _, launches = bt.timer_read()
+        got = [(bt.shot(q).download("u", R), bt.shot(q).download("v", R))
+               for q in range(len(DIMS))]
+    assert launches == steps // spp + steps % spp
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, steps)
+        wu, wv = (a, b) if newest == 0 else (b, a)
+        assert np.array_equal(got[q][0].view(np.uint32), wu.view(np.uint32)), q
+        assert np.array_equal(got[q][1].view(np.uint32), wv.view(np.uint32)), q
+
+
 def test_three_step_batch_rejected_for_25_points(gpu):
     with gpu.Batch(DIMS[:2], npts=25) as bt:
         with pytest.raises(gpu.SthError):
@@ -212,7 +238,8 @@ def test_two_step_batch_mixed_with_other_methods(gpu, orc):
     plan = [(3, gpu.BATCH_TWO_STEP, False), (2, gpu.BATCH_PERSISTENT, False),
             (5, gpu.BATCH_TWO_STEP, True), (1, gpu.BATCH_SEQUENTIAL, False),
             (4, gpu.BATCH_TWO_STEP, False), (7, gpu.BATCH_THREE_STEP, False),
-            (3, gpu.BATCH_THREE_STEP, True), (5, gpu.BATCH_THREE_STEP, False)]
+            (3, gpu.BATCH_THREE_STEP, True), (5, gpu.BATCH_THREE_STEP, False),
+            (4, gpu.BATCH_FUSED, False)]
     c2 = np.linspace(-5e-3, 5e-3, sum(p[0] for p in plan)).astype(np.float32)
     new_v = orc.fill_uniform(fields[1][0], 77, 1, halo=0.5)
     with gpu.Batch(DIMS, npts=npts) as bt:
