@@ -148,3 +148,34 @@ def test_tune_cache_three_step_lines(gpu, tmp_path, monkeypatch):
     lines = cache.read_text().splitlines()
     assert len(lines) == 3 and lines[2].startswith("sth-tune 2 25 0 1 30 34 90")
     assert best["steps_per_pass"] in (1, 2) and ms != pytest.approx(0.01)
+
+
+def test_autotune_7pt_result_is_bit_exact(gpu, orc):
+    """The autotuner times every candidate, the fused passes of two to five
+    steps among them, keeps the fastest; twelve steps under its choice
+    equal the oracle's bits."""
+    npts, shape = 7, (40, 44, 100)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u = orc.fill_uniform(g, 31, 0, halo=0.25)
+    v = orc.fill_uniform(g, 31, 1, halo=-0.25)
+    s = orc.fill_uniform(g, 31, 2, halo=0.5)
+    c1 = np.float32(0.25)
+    c0 = float(np.float32(2.0 - orc.lap_gain(npts) * float(c1)))
+    c2 = np.linspace(-2e-3, 2e-3, 12).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        best, ms = ctx.autotune(leapfrog=True, reps=1)
+        assert ms > 0.0 and 1 <= best["steps_per_pass"] <= 5
+        t = ctx.get_tuning()
+        assert t["steps_per_pass"] == best["steps_per_pass"]
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.leapfrog(12, c0, float(c1), c2)
+        gu = ctx.download("u", R)
+        gv = ctx.download("v", R)
+    a, b = u.copy(), v.copy()
+    newest = orc.leapfrog_f32(npts, g, a, b, s, c0, float(c1), c2, 12)
+    wu, wv = (a, b) if newest == 0 else (b, a)
+    assert np.array_equal(gu.view(np.uint32), wu.view(np.uint32))
+    assert np.array_equal(gv.view(np.uint32), wv.view(np.uint32))
