@@ -323,6 +323,30 @@ def test_four_step_passes_equal_one_step_sweeps_on_gpu(gpu, orc):
     assert same_bits(out[0][1], out[1][1])
 
 
+def test_five_step_passes_equal_one_step_sweeps_non_cubic(gpu, orc):
+    """A non-cubic grid with several tiles and chunks in every direction: 13
+    steps as five-step passes (two passes and a three-step pass) and as
+    one-step sweeps give the same bits for u and v, halos included."""
+    shape = (150, 131, 260)
+    npts, R = 7, 1
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-2e-3, 1e-3, 13).astype(np.float32)
+    out = []
+    for spp in (1, 5):
+        with gpu.Stencil(*shape, npts=npts) as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            ctx.init_uniform("u", 8, 0, -1.0, 1.0, 0.5)
+            ctx.init_uniform("v", 8, 1, -1.0, 1.0, -0.5)
+            ctx.init_uniform("s", 8, 2, -1.0, 1.0, 0.25)
+            ctx.timer_start()
+            ctx.leapfrog(13, c0, c1, c2)
+            _, launches = ctx.timer_read()
+            assert launches == (13 if spp == 1 else 3)
+            out.append((ctx.download("u", R), ctx.download("v", R)))
+    assert same_bits(out[0][0], out[1][0])
+    assert same_bits(out[0][1], out[1][1])
+
+
 @pytest.mark.parametrize("spp", [4, 5])
 def test_four_and_five_steps_only_for_7_points(gpu, spp):
     for npts in (13, 25):
