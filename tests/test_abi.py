@@ -28,6 +28,18 @@ def test_header_symbols_exported(sth):
     assert sorted(sth.SIGNATURES) == names
 
 
+def test_header_constants_match_the_binding(sth):
+    """The batch method codes and status codes the Python binding uses are
+    the header's."""
+    text = open(HEADER).read()
+    defs = {k: int(v, 0) for k, v in re.findall(r"#define (STH_[A-Z0-9_]+)\s+(0x[0-9a-fA-F]+|-?\d+)",
+                                                 text)}
+    for name in ("PERSISTENT", "SEQUENTIAL", "TWO_STEP", "THREE_STEP", "FUSED", "GRAPH"):
+        assert getattr(sth, "BATCH_" + name) == defs["STH_BATCH_" + name], name
+    assert len({defs["STH_BATCH_" + n] for n in ("PERSISTENT", "SEQUENTIAL", "TWO_STEP",
+                                                  "THREE_STEP", "FUSED")}) == 5
+
+
 def test_nm_exports_c_symbols(sth):
     out = subprocess.run(["nm", "-D", "--defined-only", sth.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
