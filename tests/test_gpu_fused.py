@@ -403,3 +403,29 @@ def test_default_schedule_by_grid_size(gpu, orc, npts, n, spp):
     if n <= 160:  # the oracle follows these sizes in seconds
         want_u, _ = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 12)
         assert same_bits(gu, want_u)
+
+
+@pytest.mark.parametrize("npts,shape", [(7, (256, 240, 272)), (13, (256, 240, 272)),
+                                        (25, (160, 330, 340))])
+def test_default_schedule_matches_oracle_mid_size(gpu, orc, npts, shape):
+    """The default FP32 LEAPFROG schedule (five, three or two steps per pass)
+    on grids of tens of tiles and several i-chunks (the 25-point pass needs
+    50 tiles), twelve steps with a source term and nonzero halos:
+    bit-identical to the oracle, halos included."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 29, 0.25)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-3e-3, 2e-3, 12).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts) as ctx:
+        spp = ctx.get_tuning()["steps_per_pass"]
+        assert spp == {7: 5, 13: 3, 25: 2}[npts]
+        ctx.upload("u", u, R)
+        ctx.upload("v", v, R)
+        ctx.upload("s", s, R)
+        ctx.leapfrog(12, c0, c1, c2)
+        gu = ctx.download("u", R)
+        gv = ctx.download("v", R)
+    want_u, want_v = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 12)
+    assert same_bits(gu, want_u)
+    assert same_bits(gv, want_v)
