@@ -238,9 +238,11 @@ sth_ctx *sth_batch_shot(sth_batch *b, uint32_t shot);
 uint32_t sth_batch_size(const sth_batch *b);
 /* Tile knobs of the persistent method, shared by all shots; planes = i-planes
  * per work item (0: the whole x-range); grid_blocks = persistent grid (0: the
- * resident capacity).  The sequential method launches each shot with the
- * shot's own tuning (sth_set_tuning on sth_batch_shot; default: the
- * single-grid default tile), one workgroup per tile. */
+ * resident capacity).  The fused methods (STH_BATCH_TWO_STEP, _THREE_STEP,
+ * _FUSED) take t->chunks i-chunks per shot (0: about six rounds of their
+ * persistent grid) and ignore planes.  The sequential method launches each
+ * shot with the shot's own tuning (sth_set_tuning on sth_batch_shot;
+ * default: the single-grid default tile), one workgroup per tile. */
 int sth_batch_set_tuning(sth_batch *b, const sth_tuning *t, uint32_t planes,
                          uint32_t grid_blocks);
 int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,

@@ -249,9 +249,10 @@ unsigned persistent_grid2(const sth_batch *b, int mj, int nw, int S) {
 }
 
 // Work list of a pass of S steps over all shots, from the shots' current
-// buffers (the twins must exist).  i-chunks: `planes` planes per item when
-// set, else as many as give about six rounds of the persistent grid (at
-// least 8R planes per chunk: each chunk recomputes (S-1)(2R+1) planes).
+// buffers (the twins must exist).  i-chunks per shot: the tuning's chunk
+// count when set, else as many as give about six rounds of the persistent
+// grid (at least 8R planes per chunk: each chunk recomputes (S-1)(2R+1)
+// planes).
 int build_tables2(sth_batch *b, int mj, int nw, int S) {
   const size_t ns = b->shots.size();
   sth_tuning t = b->tune;
@@ -273,9 +274,11 @@ int build_tables2(sth_batch *b, int mj, int nw, int S) {
     sth_ctx *c = b->shots[q];
     const int xs = (int)c->g.xs;
     const int most = std::max(1, xs / (8 * c->R));
-    t.chunks = (uint32_t)std::min(64, b->planes ? std::max(1, (xs + (int)b->planes - 1) /
-                                                                  (int)b->planes)
-                                                : std::min(target, most));
+    // the batch tuning's chunk count, else about six rounds (planes per
+    // item is the one-step persistent method's knob: tuned for that kernel,
+    // it cost the fused passes 8-12 %)
+    t.chunks = (uint32_t)std::min(64, b->tune.chunks ? (int)b->tune.chunks
+                                                     : std::min(target, most));
     const Plan2 p = make_plan2(c, t);
     const size_t o = c->L.origin * c->eb;
     Batch2Shot &sh = shots[q];
