@@ -47,7 +47,6 @@ struct BatchArgs {
 // sequence starts at ticket_base = n * (total_items + grid).
 template <int NPTS, int MODE, int FMT, int MJ, int NW, int V>
 __global__ __launch_bounds__(64 * NW) void batch_kernel(const BatchArgs b) {
-  constexpr int R = Terms<NPTS>::R;
   __shared__ __attribute__((aligned(16))) typename Codec<FMT>::A lds[2][TileLds<NPTS, MJ, NW, V>::SIZE];
   __shared__ float red[NW];
   __shared__ int sh_item;

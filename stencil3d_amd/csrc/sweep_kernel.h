@@ -189,8 +189,6 @@ __device__ __forceinline__ BufRes buf_res_n(Ptr base, uint32_t bytes) {
 }
 template <class T, int AUX>
 __device__ __forceinline__ T buf_load(BufRes r, uint32_t off) {
-  using I2 = int __attribute__((ext_vector_type(2)));
-  using I4 = int __attribute__((ext_vector_type(4)));
   if constexpr (sizeof(T) == 2) {
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, AUX));
   } else if constexpr (sizeof(T) == 4) {
@@ -272,8 +270,6 @@ __device__ __forceinline__ void sweep_tile(
   // global (address space 1) views: plane bases are byte pointers
   using GB = __attribute__((address_space(1))) char;
   using GPtr = GB *;
-  using GV = __attribute__((address_space(1))) VT;
-  using GS = __attribute__((address_space(1))) ST;
   constexpr int LL = 64 / F;  // lanes per row
   constexpr int TJ = NW * MJ * F, TK = LL * V;
   constexpr int CO = TileLds<NPTS, MJ, NW, V, F>::CO, LW = TileLds<NPTS, MJ, NW, V, F>::LW;
