@@ -310,7 +310,8 @@ def test_fp16_conversion_matches_numpy(orc):
     hi = (h + 1).view(np.float16).astype(np.float64)
     xs = np.concatenate([xs, ((lo + hi) / 2).astype(np.float32)])
     for x in xs:
-        want = int(np.float32(x).astype(np.float16).view(np.uint16))
+        with np.errstate(over="ignore"):  # values past 65520 round to inf on purpose
+            want = int(np.float32(x).astype(np.float16).view(np.uint16))
         assert orc.to16(1, float(x)) == want, x
     # every finite fp16 value round-trips
     for q in range(0, 0x10000, 7):
