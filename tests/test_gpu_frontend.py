@@ -52,6 +52,16 @@ def test_frontend_leapfrog_steps_per_pass_and_launch(gpu):
     assert outs[0] == outs[1] == outs[2] == outs[3]
 
 
+@pytest.mark.parametrize("stencil,spp", [("25", "3"), ("13", "4"), ("25", "5")])
+def test_frontend_rejects_a_pass_the_stencil_lacks(gpu, stencil, spp):
+    """A steps-per-pass value the stencil has no pass for is a status from
+    sth_set_tuning: the front end reports it and exits 3, nothing runs."""
+    r = run_fe(gpu, "--xs", "40", "--ys", "40", "--zs", "64", "--iter", "3", "--stencil", stencil,
+               "--mode", "leapfrog", "--steps-per-pass", spp)
+    assert r.returncode == 3
+    assert "ERROR: tuning: unsupported combination" in r.stderr
+
+
 def test_frontend_autotune(gpu):
     r = run_fe(gpu, "--xs", "40", "--ys", "40", "--zs", "64", "--iter", "3", "--stencil", "13",
                "--tune")
