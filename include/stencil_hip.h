@@ -107,6 +107,10 @@ typedef struct {
                               one-step; 1 = one; 0: default */
   uint32_t launch;        /* one-step sweep: 1 = one workgroup per tile, 2 = persistent
                              workgroups drawing tiles from a work queue; 0: default (2) */
+  uint32_t ref_sweeps;    /* REF (FP32, FP64): sweeps per launch, 1..4096 (0: one).  u and s
+                             are read-only in iterate_3d, so one launch computes each
+                             point's stencil once and applies k updates to v in
+                             registers: bit-identical to k sweeps, 1/k of the traffic */
 } sth_tuning;
 
 typedef struct sth_ctx sth_ctx;

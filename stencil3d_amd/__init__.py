@@ -51,7 +51,7 @@ class HostLayout(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("rows_per_wave", C.c_uint32), ("waves", C.c_uint32), ("chunks", C.c_uint32),
                 ("xcd_remap", C.c_uint32), ("vec", C.c_uint32), ("steps_per_pass", C.c_uint32),
-                ("launch", C.c_uint32)]
+                ("launch", C.c_uint32), ("ref_sweeps", C.c_uint32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -308,8 +308,9 @@ class Stencil:
 
     # -- schedule and timing
     def set_tuning(self, rows_per_wave=0, waves=0, chunks=0, xcd_remap=1, vec=0,
-                   steps_per_pass=0, launch=0):
-        t = Tuning(rows_per_wave, waves, chunks, xcd_remap, vec, steps_per_pass, launch)
+                   steps_per_pass=0, launch=0, ref_sweeps=0):
+        t = Tuning(rows_per_wave, waves, chunks, xcd_remap, vec, steps_per_pass, launch,
+                   ref_sweeps)
         _ok(lib().sth_set_tuning(self.handle, C.byref(t)), "sth_set_tuning")
 
     def get_tuning(self):

@@ -113,7 +113,7 @@ int main(int argc, char **argv) {
   unsigned xb = 1, yb = 1, zb = 16;
   double sbm = 12.0;
   unsigned npts = 7, seed = 0, tune = 0, rows = 0, waves = 0, chunks = 0, vec = 0;
-  unsigned spp = 0, launch = 0;
+  unsigned spp = 0, launch = 0, rsweeps = 0;
   int mode = 0, dtype = 0, init = -1;
   double c1 = 0.25, c2 = 1e-6, c0 = -1e30;
   static const char *const kModes[] = {"ref", "leapfrog"};
@@ -155,6 +155,7 @@ int main(int argc, char **argv) {
     args += get_int_arg(argv, iarg, argc, "--vec", vec, 1, 2, 1);
     args += get_int_arg(argv, iarg, argc, "--steps-per-pass", spp, 1, 5, 1);
     args += get_int_arg(argv, iarg, argc, "--launch", launch, 1, 2, 1);
+    args += get_int_arg(argv, iarg, argc, "--sweeps-per-launch", rsweeps, 1, 4096, 1);
     if (args > 0) {
       iarg += args;
     } else {
@@ -215,7 +216,7 @@ int main(int argc, char **argv) {
 
   sth_ctx *ctx = nullptr;
   if ((st = sth_create(&g, &ctx))) die("sth_create", st);
-  sth_tuning tn = {rows, waves, chunks, 1, vec, spp, launch};
+  sth_tuning tn = {rows, waves, chunks, 1, vec, spp, launch, rsweeps};
   if ((st = sth_set_tuning(ctx, &tn))) die("tuning", st);
 
   printf("[rank0] Phase #%i: array initialization\n", ++phase);
@@ -280,11 +281,15 @@ int main(int argc, char **argv) {
   printf("[rank0] Phase #%i: finish compute in %f secs (%f %s, mem: %f GB/s, cache: %f GB/s)\n",
          ++phase, t1 - t0, flops / (t1 - t0), "GFlop/s", mem_bytes / (t1 - t0) / cacheiter,
          cache_bytes / (t1 - t0));
-  const double per = kms / (double)(launches ? launches : 1);
+  // kernel time per sweep: a launch may apply several (fused LEAPFROG
+  // passes, REF sweeps per launch), and each launch streams at least the
+  // four fields of one sweep (three read, one written) through HBM
+  const double per = kms / (double)(maxiter ? maxiter : 1);
   const double gpts = gvol / (per * 1e-3) * 1e-9;
-  printf(" -- GPU: %.4f ms/sweep (HIP events), %.2f Gpts/s, %.1f GB/s algorithmic HBM "
-         "(%.1f%% of 8 TB/s)\n",
-         per, gpts, gpts * mem_pt, 100.0 * gpts * mem_pt / 8000.0);
+  const double hbm = kms > 0 ? mem_pt * gvol * (double)launches / (kms * 1e-3) * 1e-9 : 0.0;
+  printf(" -- GPU: %.4f ms/sweep (HIP events, %llu launches), %.2f Gpts/s, %.1f GB/s "
+         "HBM at four fields per launch (%.1f%% of 8 TB/s)\n",
+         per, (unsigned long long)launches, gpts, hbm, 100.0 * hbm / 8000.0);
   report_check(maxiter + 1);
   if (mode != 0) {
     double mx = 0, l2 = 0;

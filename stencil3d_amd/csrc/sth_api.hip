@@ -359,14 +359,21 @@ int launch_one(sth_ctx *c, const Plan &p, int mode, SweepArgs &a) {
   return STH_OK;
 }
 
+// REF sweeps (iterate_3d, iterators.cpp:130-157): v is updated in place
+// from read-only u and s, so each point's sweeps are independent of every
+// other point's; a launch applies up to tune.ref_sweeps of them.
 int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
   SweepArgs a;
   memset(&a, 0, sizeof(a));
   base_args(c, p, a);
   fill_weights(c, false, a);
-  for (uint32_t n = 0; n < iterations; n++) {
+  const uint32_t k = std::max<uint32_t>(1, c->tune.ref_sweeps);
+  for (uint32_t n = 0; n < iterations;) {
+    const uint32_t m = std::min(k, iterations - n);
+    a.ref_extra = (int)m - 1;
     int st = launch_one(c, p, MODE_REF, a);
     if (st) return st;
+    n += m;
   }
   return STH_OK;
 }
@@ -1080,7 +1087,9 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
 }
 
 int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
-  if (!c || !t || t->steps_per_pass > 5 || t->launch > 2 || t->chunks > 64) return STH_ERR_ARG;
+  if (!c || !t || t->steps_per_pass > 5 || t->launch > 2 || t->chunks > 64 ||
+      t->ref_sweeps > 4096)
+    return STH_ERR_ARG;
   if (t->steps_per_pass >= 2 && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
   if ((t->steps_per_pass == 3 && c->g.npts == 25) || (t->steps_per_pass >= 4 && c->g.npts != 7))
     return STH_ERR_UNSUPPORTED;

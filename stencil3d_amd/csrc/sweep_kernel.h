@@ -48,6 +48,7 @@ struct SweepArgs {
   int ntk, ntj, chunks, chunk_len;
   int remap;
   float coef0, mA;      // REF: coef0 and -A
+  int ref_extra;        // REF: updates of v after the first (u, s are read-only)
   float c0, c1, c2n;    // LEAPFROG coefficients (c2n: this step's source coefficient)
   float w[12];          // per-term weight, evaluation order (LEAPFROG: 2*w)
   Q16Slot *qs;          // 16-bit storage: the 3-slot ring
@@ -517,6 +518,7 @@ __device__ __forceinline__ void sweep_tile(
 #pragma unroll
           for (int t = 1; t < T; t++) p = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
           out = fmaT(s0, p, fmaT((AT)a.mA, u0, v0));
+          for (int r = 0; r < a.ref_extra; r++) out = fmaT(s0, p, fmaT((AT)a.mA, u0, out));
         } else {
           AT Lp = (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll

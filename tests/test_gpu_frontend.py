@@ -52,6 +52,17 @@ def test_frontend_leapfrog_steps_per_pass_and_launch(gpu):
     assert outs[0] == outs[1] == outs[2] == outs[3]
 
 
+def test_frontend_sweeps_per_launch(gpu):
+    """The reference's run (REF, init_arrays, check) with ten sweeps per
+    launch: both checks pass, the timed 20 sweeps take 2 launches and the
+    reported time is per sweep."""
+    r = run_fe(gpu, "--xs", "48", "--ys", "40", "--zs", "96", "--iter", "20", "--stencil", "13",
+               "--sweeps-per-launch", "10")
+    assert r.returncode == 0, r.stderr
+    assert re.findall(r"check: (\d+) errors found", r.stdout) == ["0", "0"]
+    assert re.search(r"-- GPU: [0-9.]+ ms/sweep \(HIP events, 2 launches\)", r.stdout)
+
+
 @pytest.mark.parametrize("stencil,spp", [("25", "3"), ("13", "4"), ("25", "5")])
 def test_frontend_rejects_a_pass_the_stencil_lacks(gpu, stencil, spp):
     """A steps-per-pass value the stencil has no pass for is a status from
