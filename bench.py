@@ -191,9 +191,13 @@ def run_other_stencils(S, n, steps, warmup):
             ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 0)
             ctx.init_gaussian("v", 1.0, 8.0, 1e-3, 1, 1)
             ctx.init_point("s", 0, 0, 0, 0.0)
-            for mode in ("ref", "leapfrog"):
-                run = (lambda k: ctx.iterate(k)) if mode == "ref" else \
-                    (lambda k: ctx.leapfrog(k, c0, c1))
+            for mode in ("ref", "leapfrog", "ref_10_per_launch"):
+                if mode == "ref_10_per_launch":
+                    # opt-in: ten REF sweeps per launch (bit-identical to ten
+                    # launches; u and s are read once per ten sweeps)
+                    ctx.set_tuning(ref_sweeps=10)
+                run = (lambda k: ctx.leapfrog(k, c0, c1)) if mode == "leapfrog" else \
+                    (lambda k: ctx.iterate(k))
                 run(warmup)
                 ctx.sync()
                 ctx.timer_start()
@@ -209,7 +213,8 @@ def run_other_stencils(S, n, steps, warmup):
                     row[mode]["tuning"] = ctx.get_tuning()
         out[str(npts)] = row
         log(f"[bench] fp32 {npts}-pt: REF {row['ref']['gpts']} Gpts/s, "
-            f"LEAPFROG {row['leapfrog']['gpts']} Gpts/s")
+            f"LEAPFROG {row['leapfrog']['gpts']} Gpts/s, "
+            f"REF 10 sweeps per launch {row['ref_10_per_launch']['gpts']} Gpts/s")
     return out
 
 
