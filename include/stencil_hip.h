@@ -43,10 +43,20 @@ typedef enum {
   STH_ERR_UNSUPPORTED = 6
 } sth_status;
 
-/* FP64 storage computes in FP64 with the FP32 stencil constants promoted
- * (the same discrete operator as FP32, for on-GPU error references; the
- * reference's -DUSE_DOUBLE build, config.h:96-100, is the analogue). */
+/* FP64 storage computes in FP64.  Its stencil weights are set per context
+ * (sth_set_weights): by default the FP32 constants promoted (the same
+ * discrete operator as FP32, the on-GPU error reference of the FP32 and
+ * 16-bit forms), or as the reference's -DUSE_DOUBLE build forms them
+ * (config.h:96-100). */
 typedef enum { STH_FP32 = 0, STH_FP16 = 1, STH_BF16 = 2, STH_FP64 = 3 } sth_dtype;
+
+/* Weights of FP64 storage.  STH_WEIGHTS_FP32: each weight is the FP32
+ * constant of stencil_N.h promoted to double.  STH_WEIGHTS_REFERENCE: the
+ * reference's -DUSE_DOUBLE build (config.h:96-100): the float literals
+ * promoted, cy = cx*1.1f and cz = cx*0.9f evaluated in double
+ * (stencil_13.h:13-22, stencil_25.h:19-34); bit-identical REF sweeps to that
+ * build. */
+typedef enum { STH_WEIGHTS_FP32 = 0, STH_WEIGHTS_REFERENCE = 1 } sth_weights;
 
 /* Logical fields.  In REF mode v is updated in place and u, s are read-only.
  * In LEAPFROG mode u is the newest field and v the previous one; each step
@@ -182,12 +192,38 @@ int sth_iterate(sth_ctx *ctx, uint32_t iterations);
  * then u and v swap.  c2 is a host array of `steps` per-step source
  * coefficients (NULL: all zero).  16-bit storage requires zero halos. */
 int sth_leapfrog(sth_ctx *ctx, uint32_t steps, float c0, float c1, const float *c2);
+/* Host-only: the centre coefficient that keeps a constant field constant,
+ * c0 = 2 - c1 * sum 2*|2 c_axis,r| (the sum is 6 for 7 and 13 points, 12 for
+ * 25; for 7 points this is README.md:17-20's c0 = 2 - 6 c1), rounded to FP32. */
+int sth_leapfrog_c0(uint32_t npts, float c1, float *c0);
+/* Host-only: per-step source coefficients of a Ricker wavelet (no reference
+ * counterpart: the reference has no source term),
+ *   c2[n] = (float)(scale * (1 - 2a) exp(-a)),  a = pi^2 f^2 (n dt - t0)^2,
+ * n = 0 .. steps-1 (scale = dt^2 injects the wavelet as the leapfrog's
+ * dt^2 * s term). */
+int sth_ricker(uint32_t steps, double dt, double f, double t0, double scale, float *c2);
 int sth_sync(sth_ctx *ctx);
 
 /* check (iterators.cpp:304-325): interior points with |v - expected| > 0.1. */
 int sth_check(sth_ctx *ctx, float expected, uint64_t *errors);
 /* Interior max |x| and L2 norm of a field, accumulated in FP64 on the GPU. */
 int sth_norms(sth_ctx *ctx, sth_field f, double *maxabs, double *l2);
+
+/* Error of field fa of context a against field fb of context b (the same
+ * geometry and device; storage types may differ, e.g. an FP32 or 16-bit run
+ * against its FP64 twin), over the interior, accumulated in FP64 on the GPU:
+ *   max_rel = max|a - b| / max|b|,  l2_rel = ||a - b||_2 / ||b||_2.
+ * No counterpart in the reference (whose only result test is check());
+ * synchronises both streams. */
+int sth_compare(sth_ctx *a, sth_field fa, sth_ctx *b, sth_field fb, double *max_rel,
+                double *l2_rel);
+
+/* Stencil weights of an FP64-storage context (sth_weights; the default is
+ * STH_WEIGHTS_FP32).  STH_WEIGHTS_REFERENCE on other storage types:
+ * STH_ERR_UNSUPPORTED.  Replaces the -DUSE_DOUBLE choice of the reference's
+ * build (config.h:96-100) with a run-time one. */
+int sth_set_weights(sth_ctx *ctx, int mode);
+int sth_get_weights(const sth_ctx *ctx, int *mode);
 
 /* ---- schedule and timing ---------------------------------------------------- */
 int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
@@ -212,6 +248,18 @@ int sth_timer_read(sth_ctx *ctx, double *kernel_ms, uint64_t *launches);
  * of HBM traffic per launch: the achievable-bandwidth context for the sweep's
  * roofline, and the calibration kernel for the traffic counters. */
 int sth_stream_copy(sth_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms);
+/* Diagnostic with no counterpart in the reference: a streaming kernel that
+ * reads `reads` (0-3) and writes `writes` (0-2) streams of `bytes` each
+ * (reads: u, s, v; writes: v in place and the twin of u, allocated if needed;
+ * fields are modified), lane_bytes = 4 (the sweep's access width) or 16 per
+ * lane access, four accesses per lane in flight, over a grid of `grid`
+ * workgroups of 256 (0: 8 per CU); returns the mean ms per launch.  HBM
+ * traffic per launch: (reads + writes) * bytes.  The achievable-bandwidth
+ * ceilings of the sweep's access mixes (3 reads + 1 write: the one-step
+ * sweep; 3 + 2: the fused pass) and the calibration kernels of the traffic
+ * counters at either access width. */
+int sth_bw_probe(sth_ctx *ctx, uint32_t reads, uint32_t writes, uint32_t lane_bytes,
+                 uint64_t bytes, uint32_t grid, uint32_t reps, double *ms);
 
 /* ---- batching: many independent grids ("shots") advanced together ------ */
 /* The reference runs one grid per process; a batch holds nshots grids of
@@ -260,10 +308,15 @@ int sth_batch_get_tuning(const sth_batch *b, sth_tuning *t, uint32_t *planes,
 int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const float *c2,
                        uint32_t method);
 int sth_batch_sync(sth_batch *b);
-/* Times tile x planes-per-item candidates of the persistent step (fields are
- * modified), keeps the fastest and returns it. */
+/* Times tile x planes-per-item candidates of the persistent one-step launch,
+ * then, with the fastest of those, the i-chunk counts of STH_BATCH_FUSED
+ * (fields are modified); keeps both choices (the chunk count in
+ * best->chunks) and returns the fastest time per step of either method.
+ * sth_batch_best_method then names the faster method (STH_BATCH_PERSISTENT
+ * or STH_BATCH_FUSED). */
 int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *planes,
                        float *best_ms);
+int sth_batch_best_method(const sth_batch *b, uint32_t *method);
 /* Accumulates the GPU time of every sth_batch_leapfrog call (events around
  * the launched work) between start and read. */
 int sth_batch_timer_start(sth_batch *b);

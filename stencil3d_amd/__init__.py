@@ -24,6 +24,7 @@ FRONTEND_PATH = os.path.join(PKG, "hstencil3d_hip")
 STH_OK, STH_ERR_ARG, STH_ERR_HIP, STH_ERR_NOMEM, STH_ERR_STATE, STH_ERR_NODEV, \
     STH_ERR_UNSUPPORTED = range(7)
 DTYPES = {"fp32": 0, "fp16": 1, "bf16": 2, "fp64": 3}
+WEIGHTS = {"fp32": 0, "reference": 1}  # sth_weights (FP64 storage)
 FIELDS = {"v": 0, "s": 1, "u": 2}
 
 
@@ -86,15 +87,23 @@ SIGNATURES = {
     "sth_load": (C.c_int, [_VP, C.c_char_p]),
     "sth_iterate": (C.c_int, [_VP, C.c_uint32]),
     "sth_leapfrog": (C.c_int, [_VP, C.c_uint32, C.c_float, C.c_float, _P(C.c_float)]),
+    "sth_leapfrog_c0": (C.c_int, [C.c_uint32, C.c_float, _P(C.c_float)]),
+    "sth_ricker": (C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
+                             _P(C.c_float)]),
     "sth_sync": (C.c_int, [_VP]),
     "sth_check": (C.c_int, [_VP, C.c_float, _P(C.c_uint64)]),
     "sth_norms": (C.c_int, [_VP, C.c_int, _P(C.c_double), _P(C.c_double)]),
+    "sth_compare": (C.c_int, [_VP, C.c_int, _VP, C.c_int, _P(C.c_double), _P(C.c_double)]),
+    "sth_set_weights": (C.c_int, [_VP, C.c_int]),
+    "sth_get_weights": (C.c_int, [_VP, _P(C.c_int)]),
     "sth_set_tuning": (C.c_int, [_VP, _P(Tuning)]),
     "sth_get_tuning": (C.c_int, [_VP, _P(Tuning)]),
     "sth_autotune": (C.c_int, [_VP, C.c_int, C.c_uint32, _P(Tuning), _P(C.c_float)]),
     "sth_timer_start": (C.c_int, [_VP]),
     "sth_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
     "sth_stream_copy": (C.c_int, [_VP, C.c_uint64, C.c_uint32, _P(C.c_double)]),
+    "sth_bw_probe": (C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
+                               C.c_uint32, _P(C.c_double)]),
     "sth_batch_create": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P(_VP)]),
     "sth_batch_destroy": (C.c_int, [_VP]),
@@ -107,6 +116,7 @@ SIGNATURES = {
     "sth_batch_sync": (C.c_int, [_VP]),
     "sth_batch_autotune": (C.c_int, [_VP, C.c_uint32, _P(Tuning), _P(C.c_uint32),
                                      _P(C.c_float)]),
+    "sth_batch_best_method": (C.c_int, [_VP, _P(C.c_uint32)]),
     "sth_batch_timer_start": (C.c_int, [_VP]),
     "sth_batch_timer_read": (C.c_int, [_VP, _P(C.c_double), _P(C.c_uint64)]),
 }
@@ -153,6 +163,24 @@ def layout(xs, ys, zs, npts, dtype="fp32"):
     g = Geom(xs, ys, zs, npts, DTYPES[dtype])
     out = LayoutInfo()
     _ok(lib().sth_layout(C.byref(g), C.byref(out)), "sth_layout")
+    return out
+
+
+def leapfrog_c0(npts, c1):
+    """Centre coefficient that keeps constant fields constant (FP32)."""
+    out = C.c_float()
+    _ok(lib().sth_leapfrog_c0(npts, c1, C.byref(out)), "sth_leapfrog_c0")
+    return out.value
+
+
+def ricker(steps, dt=1e-3, f=15.0, t0=None, scale=None):
+    """Per-step Ricker source coefficients (float32): scale * wavelet(n dt - t0);
+    defaults t0 = 1/f, scale = dt^2."""
+    out = np.zeros(steps, np.float32)
+    t0 = 1.0 / f if t0 is None else t0
+    scale = dt * dt if scale is None else scale
+    _ok(lib().sth_ricker(steps, dt, f, t0, scale, out.ctypes.data_as(_P(C.c_float))),
+        "sth_ricker")
     return out
 
 
@@ -306,6 +334,25 @@ class Stencil:
         _ok(lib().sth_norms(self.handle, FIELDS[field], C.byref(mx), C.byref(l2)), "sth_norms")
         return mx.value, l2.value
 
+    def compare(self, field, ref, ref_field=None):
+        """(max-rel, L2-rel) of this context's field against `ref`'s (same
+        geometry; e.g. an FP64 twin run), computed on the GPU."""
+        mr, lr = C.c_double(), C.c_double()
+        _ok(lib().sth_compare(self.handle, FIELDS[field], ref.handle,
+                              FIELDS[ref_field or field], C.byref(mr), C.byref(lr)),
+            "sth_compare")
+        return mr.value, lr.value
+
+    def set_weights(self, mode):
+        """FP64 storage: 'fp32' (FP32 constants promoted, the default) or
+        'reference' (the reference's -DUSE_DOUBLE weights)."""
+        _ok(lib().sth_set_weights(self.handle, WEIGHTS[mode]), "sth_set_weights")
+
+    def get_weights(self):
+        m = C.c_int()
+        _ok(lib().sth_get_weights(self.handle, C.byref(m)), "sth_get_weights")
+        return {v: k for k, v in WEIGHTS.items()}[m.value]
+
     # -- schedule and timing
     def set_tuning(self, rows_per_wave=0, waves=0, chunks=0, xcd_remap=1, vec=0,
                    steps_per_pass=0, launch=0, ref_sweeps=0):
@@ -331,6 +378,14 @@ class Stencil:
         """Plain copy kernel u -> v of nbytes; returns mean ms per copy."""
         ms = C.c_double()
         _ok(lib().sth_stream_copy(self.handle, nbytes, reps, C.byref(ms)), "sth_stream_copy")
+        return ms.value
+
+    def bw_probe(self, reads, writes, lane_bytes, nbytes, reps=5, grid=0):
+        """Streaming probe (reads of u, s, v; writes of v and u's twin) of
+        nbytes per stream; returns mean ms per launch."""
+        ms = C.c_double()
+        _ok(lib().sth_bw_probe(self.handle, reads, writes, lane_bytes, nbytes, grid, reps,
+                               C.byref(ms)), "sth_bw_probe")
         return ms.value
 
     def timer_read(self):
@@ -409,8 +464,23 @@ class Batch:
         d.update(planes=p.value)
         return d, ms.value
 
+    def best_method(self):
+        """The method the last autotune found fastest (BATCH_PERSISTENT or
+        BATCH_FUSED)."""
+        m = C.c_uint32()
+        _ok(lib().sth_batch_best_method(self._h, C.byref(m)), "sth_batch_best_method")
+        return m.value
+
     def timer_start(self):
         _ok(lib().sth_batch_timer_start(self._h), "sth_batch_timer_start")
+
+    def bw_probe(self, reads, writes, lane_bytes, nbytes, reps=5, grid=0):
+        """Streaming probe (reads of u, s, v; writes of v and u's twin) of
+        nbytes per stream; returns mean ms per launch."""
+        ms = C.c_double()
+        _ok(lib().sth_bw_probe(self.handle, reads, writes, lane_bytes, nbytes, grid, reps,
+                               C.byref(ms)), "sth_bw_probe")
+        return ms.value
 
     def timer_read(self):
         ms, n = C.c_double(), C.c_uint64()

@@ -122,6 +122,7 @@ struct sth_batch {
   size_t shot2_cap = 0, items2_cap = 0;
   int total_items2 = 0;
   hipGraphExec_t exec = nullptr;
+  uint32_t best_method = STH_BATCH_PERSISTENT;  // the autotuner's fastest method
   bool timing = false;
   double timed_ms = 0.0;
   uint64_t launches = 0;
@@ -536,7 +537,11 @@ int sth_batch_set_tuning(sth_batch *b, const sth_tuning *t, uint32_t planes, uin
   b->planes = planes;
   b->grid_blocks = grid_blocks;
   b->tables_ok = false;
-  for (sth_ctx *c : b->shots) c->tune = *t;
+  // the shots' own tuning (the sequential method's) takes the tile; the
+  // chunk count is the fused passes' knob, counted over the whole batch
+  sth_tuning st = *t;
+  st.chunks = 0;
+  for (sth_ctx *c : b->shots) c->tune = st;
   return STH_OK;
 }
 
@@ -567,7 +572,12 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
     return STH_ERR_ARG;
   if (base == STH_BATCH_THREE_STEP && b->npts == 25) return STH_ERR_UNSUPPORTED;
   if (steps == 0) return STH_OK;
+  // One u/v parity for the whole batch (its launches pick every shot's u
+  // buffer from it): a shot stepped on its own since the last batch call
+  // (sth_leapfrog on sth_batch_shot) holds u in the other buffer.
   const int parity0 = b->shots[0]->ubuf;
+  for (const sth_ctx *c : b->shots)
+    if (c->ubuf != parity0) return STH_ERR_STATE;
   // STH_BATCH_FUSED: the single-grid default pass of the stencil
   const int S = base == STH_BATCH_TWO_STEP     ? 2
                 : base == STH_BATCH_THREE_STEP ? 3
@@ -704,6 +714,35 @@ int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *
         bp = pl;
       }
     }
+  // The fused pass of the stencil's default length (STH_BATCH_FUSED): its
+  // i-chunks per shot (0: about six rounds of its persistent grid), timed
+  // over whole passes.
+  static const uint32_t chunk_opts[] = {0, 1, 2, 3, 4, 6, 8, 12};
+  const uint32_t S = b->npts == 7 ? 5 : b->npts == 13 ? 3 : 2;
+  const uint32_t nsteps = (std::max(reps, S) + S - 1) / S * S;
+  float fms = 1e30f;
+  uint32_t fch = 0;
+  for (uint32_t ch : chunk_opts) {
+    sth_tuning t = bt;
+    t.chunks = ch;
+    int st = sth_batch_set_tuning(b, &t, bp, saved_grid);
+    if (st) return st;
+    b->timing = false;
+    st = sth_batch_leapfrog(b, S, 1.0f, 0.0f, nullptr, STH_BATCH_FUSED);
+    if (st) return st;
+    b->timing = true;
+    b->timed_ms = 0.0;
+    b->launches = 0;
+    st = sth_batch_leapfrog(b, nsteps, 1.0f, 0.0f, nullptr, STH_BATCH_FUSED);
+    if (st) return st;
+    const float ms = (float)(b->timed_ms / nsteps);
+    if (ms < fms) {
+      fms = ms;
+      fch = ch;
+    }
+  }
+  bt.chunks = fch;
+  b->best_method = fms < bms ? STH_BATCH_FUSED : STH_BATCH_PERSISTENT;
   b->timing = saved_timing;
   b->timed_ms = 0.0;
   b->launches = 0;
@@ -711,7 +750,13 @@ int sth_batch_autotune(sth_batch *b, uint32_t reps, sth_tuning *best, uint32_t *
   if (st) return st;
   if (best) *best = bt;
   if (best_planes) *best_planes = bp;
-  if (best_ms) *best_ms = bms;
+  if (best_ms) *best_ms = std::min(bms, fms);
+  return STH_OK;
+}
+
+int sth_batch_best_method(const sth_batch *b, uint32_t *method) {
+  if (!b || !method) return STH_ERR_ARG;
+  *method = b->best_method;
   return STH_OK;
 }
 

@@ -35,7 +35,13 @@ hipError_t launch_check(const FieldDesc &d, const void *v, int fmt, int e, float
                         unsigned long long *errors, hipStream_t st);
 hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, double *partial,
                         uint32_t *maxbits, double *out, hipStream_t st);
+hipError_t launch_compare(const FieldDesc &dx, const void *x, int fx, int ex,
+                          const FieldDesc &dr, const void *r, int fr, int er, double *pdiff,
+                          double *pref, uint32_t *maxdiff, uint32_t *maxref, double *sums,
+                          hipStream_t st);
 hipError_t launch_zero_u32(uint32_t *p, int n, hipStream_t st);
 hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+hipError_t launch_bw_probe(int nr, int nw, int lane_bytes, const void *const *r, void *const *w,
+                           uint64_t bytes, float *sink, unsigned grid, hipStream_t st);
 
 }  // namespace sth

@@ -126,26 +126,33 @@ __global__ void check_kernel(FieldDesc d, const void *v, int fmt, int e, float e
   if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(errors, cnt);
 }
 
+// Largest of non-negative doubles by their bit patterns (which order like
+// the values), and its FP32 rounding (rounding is monotone) for the callers
+// that keep FP32 maxima.
+__device__ inline void max_both(uint32_t *maxbits, double m) {
+  atomicMax((unsigned long long *)(maxbits + 2), (unsigned long long)__double_as_longlong(m));
+  atomicMax(maxbits, __float_as_uint((float)m));
+}
+
 // Interior max |x| and per-block sums of x^2 in FP64 (summed in block order
-// by norms_finish, so the L2 norm is deterministic).
+// by norms_finish, so the L2 norm is deterministic).  maxbits[0]: the max as
+// FP32 bits; maxbits[2..3]: as FP64 bits.
 __global__ void norms_kernel(FieldDesc d, const void *x, int fmt, int e, double *partial,
                              uint32_t *maxbits) {
-  __shared__ double ss[16];
-  __shared__ float sm[16];
-  double s = 0.0;
-  float mx = 0.0f;
+  __shared__ double ss[16], sm[16];
+  double s = 0.0, mx = 0.0;
   for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < d.elems;
        idx += (uint64_t)gridDim.x * blockDim.x) {
     const Cell c = decode(d, idx);
     if (c.interior) {
       const double val = load_dec(x, idx, fmt, e);
       s += val * val;
-      mx = fmaxf(mx, (float)fabs(val));
+      mx = fmax(mx, fabs(val));
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
     s += __shfl_xor(s, o, 64);
-    mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    mx = fmax(mx, __shfl_xor(mx, o, 64));
   }
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
@@ -154,14 +161,64 @@ __global__ void norms_kernel(FieldDesc d, const void *x, int fmt, int e, double 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    float m = 0.0f;
+    double t = 0.0, m = 0.0;
     for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
       t += ss[q];
-      m = fmaxf(m, sm[q]);
+      m = fmax(m, sm[q]);
     }
     partial[blockIdx.x] = t;
-    atomicMax(maxbits, __float_as_uint(m));
+    max_both(maxbits, m);
+  }
+}
+
+// Interior difference of two fields of one geometry in two layouts (storage
+// types may differ): per-block sums of (x - r)^2 and r^2 in FP64 and the
+// maxima of |x - r| and |r| (FP32 bits).
+__global__ void compare_kernel(FieldDesc dx, const void *x, int fx, int ex, FieldDesc dr,
+                               const void *r, int fr, int er, double *pdiff, double *pref,
+                               uint32_t *maxdiff, uint32_t *maxref) {
+  __shared__ double sd[16], sr[16], md[16], mr[16];
+  double s1 = 0.0, s2 = 0.0, m1 = 0.0, m2 = 0.0;
+  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < dx.elems;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const Cell c = decode(dx, idx);
+    if (c.interior) {
+      const uint64_t jdx = dr.align - dr.R + (uint64_t)(c.i + dr.R) * dr.plane +
+                           (uint64_t)(c.j + dr.R) * dr.pitch + (uint64_t)(c.k + dr.R);
+      const double a = load_dec(x, idx, fx, ex), b = load_dec(r, jdx, fr, er);
+      const double d = a - b;
+      s1 += d * d;
+      s2 += b * b;
+      m1 = fmax(m1, fabs(d));
+      m2 = fmax(m2, fabs(b));
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s1 += __shfl_xor(s1, o, 64);
+    s2 += __shfl_xor(s2, o, 64);
+    m1 = fmax(m1, __shfl_xor(m1, o, 64));
+    m2 = fmax(m2, __shfl_xor(m2, o, 64));
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sd[w] = s1;
+    sr[w] = s2;
+    md[w] = m1;
+    mr[w] = m2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t1 = 0.0, t2 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+      t1 += sd[q];
+      t2 += sr[q];
+      a1 = fmax(a1, md[q]);
+      a2 = fmax(a2, mr[q]);
+    }
+    pdiff[blockIdx.x] = t1;
+    pref[blockIdx.x] = t2;
+    max_both(maxdiff, a1);
+    max_both(maxref, a2);
   }
 }
 
@@ -188,6 +245,72 @@ __global__ void stream_copy_kernel(const float4 *__restrict__ src, float4 *__res
   for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4;
        t += (uint64_t)gridDim.x * blockDim.x)
     dst[t] = src[t];
+}
+
+// Bandwidth probe: NR streams read and NW streams written, element by element
+// (LB bytes per lane: 4 = the sweep's access width, 16 = float4), four
+// elements per thread in flight per grid-stride step.  Read-only probes keep
+// their sum live with a store no input can trigger.
+template <int NR, int NW, class T>
+__global__ __launch_bounds__(256) void bw_probe_kernel(const T *__restrict__ r0,
+                                                       const T *__restrict__ r1,
+                                                       const T *__restrict__ r2,
+                                                       T *__restrict__ w0, T *__restrict__ w1,
+                                                       uint64_t n, float *sink) {
+  constexpr int U = 4;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  T acc{};
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += U * stride) {
+    T x[U];
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint64_t e = t + q * stride;
+      x[q] = T{};
+      if (e < n) {
+        if constexpr (NR >= 1) x[q] = r0[e];
+        if constexpr (NR >= 2) x[q] = x[q] + r1[e];
+        if constexpr (NR >= 3) x[q] = x[q] + r2[e];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < U; q++) {
+      const uint64_t e = t + q * stride;
+      if (e < n) {
+        if constexpr (NW >= 1) w0[e] = x[q];
+        if constexpr (NW >= 2) w1[e] = x[q];
+        if constexpr (NW == 0) acc = acc + x[q];
+      }
+    }
+  }
+  if constexpr (NW == 0) {
+    float a;
+    if constexpr (sizeof(T) == 4)
+      a = acc;
+    else
+      a = acc.x + acc.y + acc.z + acc.w;
+    if (a == -1.2345e38f) sink[0] = a;
+  }
+}
+
+template <class T>
+hipError_t launch_probe_t(int nr, int nw, const void *const *r, void *const *w, uint64_t n,
+                          float *sink, unsigned grid, hipStream_t st) {
+  const T *r0 = (const T *)r[0], *r1 = (const T *)r[1], *r2 = (const T *)r[2];
+  T *w0 = (T *)w[0], *w1 = (T *)w[1];
+#define STH_PROBE(NR, NW)                                                                    if (nr == NR && nw == NW) {                                                                  hipLaunchKernelGGL((bw_probe_kernel<NR, NW, T>), dim3(grid), dim3(256), 0, st, r0, r1, r2,                        w0, w1, n, sink);                                                       return hipGetLastError();                                                                }
+  STH_PROBE(1, 0) STH_PROBE(2, 0) STH_PROBE(3, 0) STH_PROBE(0, 1) STH_PROBE(0, 2)
+  STH_PROBE(1, 1) STH_PROBE(2, 1) STH_PROBE(3, 1) STH_PROBE(1, 2) STH_PROBE(2, 2) STH_PROBE(3, 2)
+#undef STH_PROBE
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_bw_probe(int nr, int nw, int lane_bytes, const void *const *r, void *const *w,
+                           uint64_t bytes, float *sink, unsigned grid, hipStream_t st) {
+  if (lane_bytes == 4)
+    return launch_probe_t<float>(nr, nw, r, w, bytes / 4, sink, grid, st);
+  if (lane_bytes == 16)
+    return launch_probe_t<float4>(nr, nw, r, w, bytes / 16, sink, grid, st);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_stream_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
@@ -217,6 +340,19 @@ hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, doubl
                      maxbits);
   hipLaunchKernelGGL(norms_finish, dim3(1), dim3(64), 0, st, (const double *)partial,
                      kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_compare(const FieldDesc &dx, const void *x, int fx, int ex,
+                          const FieldDesc &dr, const void *r, int fr, int er, double *pdiff,
+                          double *pref, uint32_t *maxdiff, uint32_t *maxref, double *sums,
+                          hipStream_t st) {
+  hipLaunchKernelGGL(compare_kernel, dim3(kReduceBlocks), dim3(256), 0, st, dx, x, fx, ex, dr, r,
+                     fr, er, pdiff, pref, maxdiff, maxref);
+  hipLaunchKernelGGL(norms_finish, dim3(1), dim3(64), 0, st, (const double *)pdiff,
+                     kReduceBlocks, sums);
+  hipLaunchKernelGGL(norms_finish, dim3(1), dim3(64), 0, st, (const double *)pref,
+                     kReduceBlocks, sums + 1);
   return hipGetLastError();
 }
 

@@ -9,8 +9,12 @@
 //   ITERATE(..., maxiter, ...)   main.cpp:307   -> sth_iterate(ctx, maxiter), timed
 //   check(v,grid,maxiter+1)      main.cpp:312   -> sth_check(ctx, maxiter+1)
 // Added options select what the reference fixes at compile time
-// (--stencil = STENCIL_SIZE, config.h:37-39) and the forms the reference does
-// not have (--mode leapfrog, --dtype fp16/bf16, --tune).
+// (--stencil = STENCIL_SIZE, config.h:37-39; --dtype fp64 = -DUSE_DOUBLE,
+// config.h:96-100) and the forms the reference does not have (--mode
+// leapfrog, --dtype fp16/bf16, --tune).  --host-fields keeps the reference's
+// host arrays (main.cpp:244-289: the --xh/--yh/--zh halos, init_arrays over
+// the whole allocation, check on the host) and moves them through
+// sth_upload / sth_download.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -104,6 +108,79 @@ void die(const char *what, int st) {
   exit(st == STH_ERR_ARG ? 2 : 3);
 }
 
+// The reference's three host arrays (main.cpp:244-275): one dense block per
+// field in the s_grid layout with the command line's halos, 64 KiB aligned.
+template <class T>
+struct HostFields {
+  sth_host_layout hl{};
+  uint64_t n = 0;
+  int64_t jm = 0, im = 0, ofs = 0;
+  T *f[3] = {nullptr, nullptr, nullptr};  // indexed by sth_field: v, s, u
+  bool alloc(const sth_geom &g, unsigned xh, unsigned yh, unsigned zh) {
+    hl = {xh, yh, zh};
+    if (sth_host_elems(&g, &hl, &n)) return false;
+    jm = (int64_t)g.zs + 2 * zh;
+    im = jm * ((int64_t)g.ys + 2 * yh);
+    ofs = (int64_t)zh + (int64_t)yh * jm + (int64_t)xh * im;
+    const size_t bytes = ((n * sizeof(T) + 65535) / 65536) * 65536;
+    for (int q = 0; q < 3; q++)
+      if (!(f[q] = (T *)aligned_alloc(65536, bytes))) return false;
+    return true;
+  }
+  ~HostFields() {
+    for (int q = 0; q < 3; q++) free(f[q]);
+  }
+  // init_arrays (iterators.cpp:292-301): every element of the allocation
+  void init_arrays() {
+    for (uint64_t t = 0; t < n; t++) {
+      f[STH_FIELD_V][t] = (T)0;
+      f[STH_FIELD_S][t] = (T)1;
+      f[STH_FIELD_U][t] = (T)1;
+    }
+  }
+  // check (iterators.cpp:304-325): interior points with |v - expected| > 0.1
+  uint64_t check(const sth_geom &g, double expected) const {
+    uint64_t err = 0;
+    for (int64_t i = 0; i < (int64_t)g.xs; i++)
+      for (int64_t j = 0; j < (int64_t)g.ys; j++) {
+        const T *row = f[STH_FIELD_V] + ofs + i * im + j * jm;
+        for (int64_t k = 0; k < (int64_t)g.zs; k++)
+          if (fabs((double)row[k] - expected) > 0.1) err++;
+      }
+    return err;
+  }
+  double maxabs(const sth_geom &g, int field) const {
+    double m = 0.0;
+    for (int64_t i = 0; i < (int64_t)g.xs; i++)
+      for (int64_t j = 0; j < (int64_t)g.ys; j++) {
+        const T *row = f[field] + ofs + i * im + j * jm;
+        for (int64_t k = 0; k < (int64_t)g.zs; k++) m = fmax(m, fabs((double)row[k]));
+      }
+    return m;
+  }
+};
+
+template <class T>
+int upload_host(sth_ctx *ctx, HostFields<T> &h, int field);
+template <>
+int upload_host<float>(sth_ctx *ctx, HostFields<float> &h, int field) {
+  return sth_upload(ctx, (sth_field)field, h.f[field], &h.hl);
+}
+template <>
+int upload_host<double>(sth_ctx *ctx, HostFields<double> &h, int field) {
+  return sth_upload_f64(ctx, (sth_field)field, h.f[field], &h.hl);
+}
+template <class T>
+int download_host(sth_ctx *ctx, HostFields<T> &h, int field);
+template <>
+int download_host<float>(sth_ctx *ctx, HostFields<float> &h, int field) {
+  return sth_download(ctx, (sth_field)field, h.f[field], &h.hl);
+}
+template <>
+int download_host<double>(sth_ctx *ctx, HostFields<double> &h, int field) {
+  return sth_download_f64(ctx, (sth_field)field, h.f[field], &h.hl);
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -113,11 +190,11 @@ int main(int argc, char **argv) {
   unsigned xb = 1, yb = 1, zb = 16;
   double sbm = 12.0;
   unsigned npts = 7, seed = 0, tune = 0, rows = 0, waves = 0, chunks = 0, vec = 0;
-  unsigned spp = 0, launch = 0, rsweeps = 0;
+  unsigned spp = 0, launch = 0, rsweeps = 0, host_fields = 0;
   int mode = 0, dtype = 0, init = -1;
   double c1 = 0.25, c2 = 1e-6, c0 = -1e30;
   static const char *const kModes[] = {"ref", "leapfrog"};
-  static const char *const kTypes[] = {"fp32", "fp16", "bf16"};
+  static const char *const kTypes[] = {"fp32", "fp16", "bf16", "fp64"};
   static const char *const kInits[] = {"const", "uniform", "gauss"};
 
   int iarg = 1;
@@ -142,7 +219,7 @@ int main(int argc, char **argv) {
     // GPU build options
     args += get_int_arg(argv, iarg, argc, "--stencil", npts, 7, 25, 1);
     args += get_choice(argv, iarg, argc, "--mode", kModes, 2, mode);
-    args += get_choice(argv, iarg, argc, "--dtype", kTypes, 3, dtype);
+    args += get_choice(argv, iarg, argc, "--dtype", kTypes, 4, dtype);
     args += get_choice(argv, iarg, argc, "--init", kInits, 3, init);
     args += get_int_arg(argv, iarg, argc, "--seed", seed, 0, 1u << 30, 1);
     args += get_real_arg(argv, iarg, argc, "--c1", c1, 0.0, 4.0);
@@ -156,6 +233,7 @@ int main(int argc, char **argv) {
     args += get_int_arg(argv, iarg, argc, "--steps-per-pass", spp, 1, 5, 1);
     args += get_int_arg(argv, iarg, argc, "--launch", launch, 1, 2, 1);
     args += get_int_arg(argv, iarg, argc, "--sweeps-per-launch", rsweeps, 1, 4096, 1);
+    args += get_flag(argv, iarg, "--host-fields", host_fields);
     if (args > 0) {
       iarg += args;
     } else {
@@ -171,8 +249,12 @@ int main(int argc, char **argv) {
     fprintf(stderr, "ERROR: --ys must by an exact multiple of --yb\n");
     exit(2);
   }
-  if (mode == 0 && dtype != 0) {
-    fprintf(stderr, "ERROR: --mode ref runs FP32 storage only\n");
+  if (mode == 0 && dtype != 0 && dtype != 3) {
+    fprintf(stderr, "ERROR: --mode ref runs FP32 or FP64 storage only\n");
+    exit(2);
+  }
+  if (host_fields && dtype != 0 && dtype != 3) {
+    fprintf(stderr, "ERROR: --host-fields needs FP32 or FP64 storage\n");
     exit(2);
   }
   if (init < 0) init = mode == 0 ? 0 : 1;
@@ -195,7 +277,7 @@ int main(int argc, char **argv) {
   // the report's constants (config.h:73-91)
   const double flops_pt = npts == 7 ? 12 : npts == 13 ? 29 : 41;
   const double cache_pt = npts == 7 ? 40 : npts == 13 ? 64 : 112;
-  const double eb = dtype == 0 ? 4.0 : 2.0;
+  const double eb = dtype == 0 ? 4.0 : dtype == 3 ? 8.0 : 2.0;
   const double mem_pt = 4 * eb;
   const double gvol = (double)xs * ys * zs;
   const double flops = flops_pt * 1.0e-9 * gvol * maxiter * cacheiter;
@@ -218,22 +300,63 @@ int main(int argc, char **argv) {
   if ((st = sth_create(&g, &ctx))) die("sth_create", st);
   sth_tuning tn = {rows, waves, chunks, 1, vec, spp, launch, rsweeps};
   if ((st = sth_set_tuning(ctx, &tn))) die("tuning", st);
+  // FP64 is the reference's -DUSE_DOUBLE build: its weights too
+  if (dtype == 3 && (st = sth_set_weights(ctx, STH_WEIGHTS_REFERENCE))) die("weights", st);
+
+  HostFields<float> hf32;
+  HostFields<double> hf64;
+  if (host_fields) {
+    const bool ok = dtype == 3 ? hf64.alloc(g, xh, yh, zh) : hf32.alloc(g, xh, yh, zh);
+    if (!ok) die("host arrays", STH_ERR_NOMEM);
+    printf(" -- host arrays: halo %ux%ux%u, %f GB per field\n", xh, yh, zh,
+           1e-9 * (double)(dtype == 3 ? hf64.n * 8 : hf32.n * 4));
+  }
+  // Device fields from the host arrays (the reference's whole-allocation
+  // init_arrays, or a device initialiser round-tripped through the host
+  // layout).
+  auto via_host = [&](bool const_init) {
+    int s2 = 0;
+    for (int q = 0; q < 3 && !s2; q++) {
+      if (dtype == 3) {
+        if (const_init) {
+          if (q == 0) hf64.init_arrays();
+        } else {
+          s2 = download_host(ctx, hf64, q);
+        }
+        if (!s2) s2 = upload_host(ctx, hf64, q);
+      } else {
+        if (const_init) {
+          if (q == 0) hf32.init_arrays();
+        } else {
+          s2 = download_host(ctx, hf32, q);
+        }
+        if (!s2) s2 = upload_host(ctx, hf32, q);
+      }
+    }
+    return s2;
+  };
+  auto initialise = [&]() {
+    int s2;
+    if (init == 0) {
+      if (host_fields) return via_host(true);
+      s2 = sth_init_const(ctx, 0.0f, 1.0f, 1.0f);  // init_arrays, iterators.cpp:292-301
+    } else if (init == 1) {
+      s2 = sth_init_uniform(ctx, STH_FIELD_U, seed, 0, -1.0, 1.0, 0.0f);
+      if (!s2) s2 = sth_init_uniform(ctx, STH_FIELD_S, seed, 1, -1.0, 1.0, 0.0f);
+      if (!s2) s2 = sth_init_uniform(ctx, STH_FIELD_V, seed, 2, -1.0, 1.0, 0.0f);
+    } else {
+      s2 = sth_init_gaussian(ctx, STH_FIELD_U, 1.0, 8.0, 1e-3, seed, 0);
+      if (!s2) s2 = sth_init_gaussian(ctx, STH_FIELD_V, 1.0, 8.0, 1e-3, seed, 1);
+      if (!s2) s2 = sth_init_point(ctx, STH_FIELD_S, 0, 0, 0, 0.0f);
+    }
+    if (!s2 && host_fields) s2 = via_host(false);
+    if (!s2) s2 = sth_sync(ctx);
+    return s2;
+  };
 
   printf("[rank0] Phase #%i: array initialization\n", ++phase);
   double t0 = seconds();
-  if (init == 0) {
-    st = sth_init_const(ctx, 0.0f, 1.0f, 1.0f);  // init_arrays, iterators.cpp:292-301
-  } else if (init == 1) {
-    st = sth_init_uniform(ctx, STH_FIELD_U, seed, 0, -1.0, 1.0, 0.0f);
-    if (!st) st = sth_init_uniform(ctx, STH_FIELD_S, seed, 1, -1.0, 1.0, 0.0f);
-    if (!st) st = sth_init_uniform(ctx, STH_FIELD_V, seed, 2, -1.0, 1.0, 0.0f);
-  } else {
-    st = sth_init_gaussian(ctx, STH_FIELD_U, 1.0, 8.0, 1e-3, seed, 0);
-    if (!st) st = sth_init_gaussian(ctx, STH_FIELD_V, 1.0, 8.0, 1e-3, seed, 1);
-    if (!st) st = sth_init_point(ctx, STH_FIELD_S, 0, 0, 0, 0.0f);
-  }
-  if (!st) st = sth_sync(ctx);
-  if (st) die("initialization", st);
+  if ((st = initialise())) die("initialization", st);
   double t1 = seconds();
   printf("[rank0] Phase #%i: initialization took %f secs\n", ++phase, t1 - t0);
 
@@ -243,13 +366,17 @@ int main(int argc, char **argv) {
     if ((st = sth_autotune(ctx, mode, 3, &best, &ms))) die("autotune", st);
     printf(" -- autotune: rows/wave %u, waves %u, k/lane %u, chunks %u: %.3f ms/sweep\n",
            best.rows_per_wave, best.waves, best.vec, best.chunks, ms);
-    // the tuning sweeps modified the fields: initialise again
-    if (init == 0) st = sth_init_const(ctx, 0.0f, 1.0f, 1.0f);
-    if (st) die("re-initialization", st);
+    // the tuning sweeps modified the fields (and 16-bit scales): the timed
+    // run starts from the same state as without --tune
+    if ((st = initialise())) die("re-initialization", st);
   }
 
   const float fc1 = (float)c1;
-  const float fc0 = c0 > -1e29 ? (float)c0 : (float)(2.0 - 6.0 * c1);
+  // default centre weight: 2 - c1 * (the neighbour sum's constant-mode gain),
+  // which keeps a constant field constant (6 for 7 and 13 points, 12 for 25)
+  float c0_default = 0.0f;
+  if ((st = sth_leapfrog_c0(npts, fc1, &c0_default))) die("c0", st);
+  const float fc0 = c0 > -1e29 ? (float)c0 : c0_default;
   std::vector<float> c2v(maxiter + 1, (float)c2);
   auto sweep = [&](unsigned n) {
     return mode == 0 ? sth_iterate(ctx, n) : sth_leapfrog(ctx, n, fc0, fc1, c2v.data());
@@ -257,7 +384,14 @@ int main(int argc, char **argv) {
   auto report_check = [&](unsigned expected) {
     if (mode != 0) return;
     uint64_t errors = 0;
-    int s2 = sth_check(ctx, (float)expected, &errors);
+    int s2 = 0;
+    if (host_fields) {
+      // the reference's check on its host array (iterators.cpp:304-325)
+      s2 = dtype == 3 ? download_host(ctx, hf64, STH_FIELD_V) : download_host(ctx, hf32, STH_FIELD_V);
+      if (!s2) errors = dtype == 3 ? hf64.check(g, expected) : hf32.check(g, expected);
+    } else {
+      s2 = sth_check(ctx, (float)expected, &errors);
+    }
     if (s2) die("check", s2);
     printf("check: %llu errors found\n", (unsigned long long)errors);
   };
@@ -295,6 +429,13 @@ int main(int argc, char **argv) {
     double mx = 0, l2 = 0;
     if ((st = sth_norms(ctx, STH_FIELD_U, &mx, &l2))) die("norms", st);
     printf("norms: max |u| = %.9e, ||u||_2 = %.9e\n", mx, l2);
+    if (host_fields) {
+      const int s2 = dtype == 3 ? download_host(ctx, hf64, STH_FIELD_U)
+                                : download_host(ctx, hf32, STH_FIELD_U);
+      if (s2) die("download", s2);
+      printf("host: max |u| = %.9e\n", dtype == 3 ? hf64.maxabs(g, STH_FIELD_U)
+                                                   : hf32.maxabs(g, STH_FIELD_U));
+    }
   }
   printf("freeing data arrays\n");
   sth_destroy(ctx);

@@ -209,7 +209,7 @@ int device_fill(sth_ctx *c, int buf, const FillSpec &f, bool halo_zero) {
   STH_TRY(launch_fill(d, f, c->base[buf], c->fmt, e, 1, c->maxbits_dev, c->stream));
   if (is16(c)) {
     // the buffer's max is max |q| of the stored numbers (exponent 0 decode)
-    STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
+    STH_TRY(launch_zero_u32(c->maxbits_dev, 4, c->stream));
     STH_TRY(launch_norms(d, c->base[buf], c->fmt, 0, c->partial_dev, c->maxbits_dev, c->sum_dev,
                          c->stream));
     STH_TRY(hipMemcpyAsync(&mbits, c->maxbits_dev, 4, hipMemcpyDeviceToHost, c->stream));
@@ -224,12 +224,20 @@ int device_fill(sth_ctx *c, int buf, const FillSpec &f, bool halo_zero) {
 void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
   Coeffs cf;
   stencil_coeffs(c->g.npts, cf);
+  double wref[3][4];
+  stencil_coeffs_double(c->g.npts, wref);
+  const bool refd = c->fmt == FMT_F64 && c->weights == STH_WEIGHTS_REFERENCE;
   int ax[12], rr[12];
   const int T = term_order(c->g.npts, ax, rr);
-  for (int t = 0; t < 12; t++) a.w[t] = 0.0f;
+  for (int t = 0; t < 12; t++) {
+    a.w[t] = 0.0f;
+    a.wd[t] = 0.0;
+  }
   for (int t = 0; t < T; t++) {
     const float w = cf.w[ax[t]][rr[t] - 1];
+    const double wd = refd ? wref[ax[t]][rr[t] - 1] : (double)w;
     a.w[t] = leap ? 2.0f * w : w;
+    a.wd[t] = leap ? 2.0 * wd : wd;
   }
   a.coef0 = cf.coef0;
   a.mA = -cf.A;
@@ -1046,6 +1054,22 @@ int sth_leapfrog(sth_ctx *c, uint32_t steps, float c0, float c1, const float *c2
   return run_leap(c, make_plan(c, c->tune), steps, c0, c1, c2);
 }
 
+int sth_leapfrog_c0(uint32_t npts, float c1, float *c0) {
+  if (!c0 || radius_of(npts) < 0) return STH_ERR_ARG;
+  *c0 = (float)(2.0 - lap_gain(npts) * (double)c1);
+  return STH_OK;
+}
+
+int sth_ricker(uint32_t steps, double dt, double f, double t0, double scale, float *c2) {
+  if (!c2 && steps) return STH_ERR_ARG;
+  for (uint32_t n = 0; n < steps; n++) {
+    const double tau = (double)n * dt - t0;
+    const double a = M_PI * M_PI * f * f * tau * tau;
+    c2[n] = (float)(scale * ((1.0 - 2.0 * a) * exp(-a)));
+  }
+  return STH_OK;
+}
+
 int sth_sync(sth_ctx *c) {
   if (!c) return STH_ERR_ARG;
   STH_TRY(hipStreamSynchronize(c->stream));
@@ -1073,16 +1097,63 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
   if (st) return st;
   const int buf = buf_of(c, fld);
   const int e = is16(c) ? c->host_slot.exp[buf] : 0;
-  uint32_t mb = 0;
+  uint32_t mb[4] = {0, 0, 0, 0};
   double ss = 0.0;
-  STH_TRY(launch_zero_u32(c->maxbits_dev, 1, c->stream));
+  STH_TRY(launch_zero_u32(c->maxbits_dev, 4, c->stream));
   STH_TRY(launch_norms(desc_of(c), c->base[buf], c->fmt, e, c->partial_dev, c->maxbits_dev,
                        c->sum_dev, c->stream));
-  STH_TRY(hipMemcpyAsync(&mb, c->maxbits_dev, 4, hipMemcpyDeviceToHost, c->stream));
+  STH_TRY(hipMemcpyAsync(mb, c->maxbits_dev, 16, hipMemcpyDeviceToHost, c->stream));
   STH_TRY(hipMemcpyAsync(&ss, c->sum_dev, 8, hipMemcpyDeviceToHost, c->stream));
   STH_TRY(hipStreamSynchronize(c->stream));
-  if (maxabs) *maxabs = (double)u2f(mb);
+  double m64;
+  memcpy(&m64, &mb[2], 8);
+  if (maxabs) *maxabs = m64;
   if (l2) *l2 = sqrt(ss);
+  return STH_OK;
+}
+
+int sth_compare(sth_ctx *a, sth_field fa, sth_ctx *b, sth_field fb, double *max_rel,
+                double *l2_rel) {
+  if (!a || !b || fa < 0 || fa > 2 || fb < 0 || fb > 2 || a == b) return STH_ERR_ARG;
+  if (a->g.xs != b->g.xs || a->g.ys != b->g.ys || a->g.zs != b->g.zs || a->g.npts != b->g.npts ||
+      a->device != b->device)
+    return STH_ERR_ARG;
+  int st = fetch_slot(a);
+  if (!st) st = fetch_slot(b);
+  if (st) return st;
+  const int xa = buf_of(a, fa), xb = buf_of(b, fb);
+  const int ea = is16(a) ? a->host_slot.exp[xa] : 0, eb = is16(b) ? b->host_slot.exp[xb] : 0;
+  STH_TRY(hipStreamSynchronize(b->stream));  // b's field is complete
+  // a's scratch holds the difference's partials, b's those of the reference
+  STH_TRY(launch_zero_u32(a->maxbits_dev, 4, a->stream));
+  STH_TRY(launch_zero_u32(b->maxbits_dev, 4, a->stream));
+  STH_TRY(launch_compare(desc_of(a), a->base[xa], a->fmt, ea, desc_of(b), b->base[xb], b->fmt, eb,
+                         a->partial_dev, b->partial_dev, a->maxbits_dev, b->maxbits_dev,
+                         a->sum_dev, a->stream));
+  uint32_t md[4] = {0, 0, 0, 0}, mr[4] = {0, 0, 0, 0};
+  double sums[2] = {0.0, 0.0};
+  STH_TRY(hipMemcpyAsync(md, a->maxbits_dev, 16, hipMemcpyDeviceToHost, a->stream));
+  STH_TRY(hipMemcpyAsync(mr, b->maxbits_dev, 16, hipMemcpyDeviceToHost, a->stream));
+  STH_TRY(hipMemcpyAsync(sums, a->sum_dev, 16, hipMemcpyDeviceToHost, a->stream));
+  STH_TRY(hipStreamSynchronize(a->stream));
+  double dmax, rmax;
+  memcpy(&dmax, &md[2], 8);
+  memcpy(&rmax, &mr[2], 8);
+  if (max_rel) *max_rel = rmax > 0.0 ? dmax / rmax : dmax;
+  if (l2_rel) *l2_rel = sums[1] > 0.0 ? sqrt(sums[0] / sums[1]) : sqrt(sums[0]);
+  return STH_OK;
+}
+
+int sth_set_weights(sth_ctx *c, int mode) {
+  if (!c || (mode != STH_WEIGHTS_FP32 && mode != STH_WEIGHTS_REFERENCE)) return STH_ERR_ARG;
+  if (mode == STH_WEIGHTS_REFERENCE && c->fmt != FMT_F64) return STH_ERR_UNSUPPORTED;
+  c->weights = mode;
+  return STH_OK;
+}
+
+int sth_get_weights(const sth_ctx *c, int *mode) {
+  if (!c || !mode) return STH_ERR_ARG;
+  *mode = c->weights;
   return STH_OK;
 }
 
@@ -1268,6 +1339,36 @@ int sth_stream_copy(sth_ctx *c, uint64_t bytes, uint32_t reps, double *ms) {
   STH_TRY(launch_stream_copy(src, dst, bytes, c->stream));  // warm-up
   STH_TRY(hipEventRecord(c->ev0, c->stream));
   for (uint32_t r = 0; r < reps; r++) STH_TRY(launch_stream_copy(src, dst, bytes, c->stream));
+  STH_TRY(hipEventRecord(c->ev1, c->stream));
+  STH_TRY(hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  STH_TRY(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  if (ms) *ms = (double)t / reps;
+  return STH_OK;
+}
+
+int sth_bw_probe(sth_ctx *c, uint32_t reads, uint32_t writes, uint32_t lane_bytes,
+                 uint64_t bytes, uint32_t grid, uint32_t reps, double *ms) {
+  if (!c || reps == 0 || bytes == 0 || reads > 3 || writes > 2 || reads + writes == 0 ||
+      (lane_bytes != 4 && lane_bytes != 16) || bytes % 16 != 0 || bytes > c->L.bytes)
+    return STH_ERR_ARG;
+  if (writes == 2) {
+    int st = ensure_twins(c);
+    if (st) return st;
+  }
+  // read u, s, v; write v (in place, as the one-step sweep does) and the
+  // twin of u (the fused pass's second output)
+  const void *r[3] = {c->base[c->ubuf], c->base[2], c->base[1 - c->ubuf]};
+  void *w[2] = {c->base[1 - c->ubuf], writes == 2 ? c->twin[c->ubuf] : nullptr};
+  c->twins_ok = false;
+  const unsigned g = grid ? grid : (unsigned)std::max(1, c->cus) * 8;
+  float *sink = (float *)c->sum_dev;
+  STH_TRY(launch_bw_probe((int)reads, (int)writes, (int)lane_bytes, r, w, bytes, sink, g,
+                          c->stream));  // warm-up
+  STH_TRY(hipEventRecord(c->ev0, c->stream));
+  for (uint32_t q = 0; q < reps; q++)
+    STH_TRY(launch_bw_probe((int)reads, (int)writes, (int)lane_bytes, r, w, bytes, sink, g,
+                            c->stream));
   STH_TRY(hipEventRecord(c->ev1, c->stream));
   STH_TRY(hipEventSynchronize(c->ev1));
   float t = 0.0f;

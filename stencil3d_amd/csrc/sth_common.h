@@ -88,6 +88,27 @@ inline int stencil_coeffs(uint32_t npts, Coeffs &c) {
   return R;
 }
 
+// The same weights as the reference's -DUSE_DOUBLE build forms them
+// (config.h:96-100 makes `real` double): the float literals are promoted
+// (cx1 = 0.4f, stencil_25.h:22), and cy = cx*1.1f, cz = cx*0.9f are products
+// rounded in double (stencil_13.h:18-21, stencil_25.h:26-33).  The 7-point
+// weights are the literal 0.5 (stencil_7.h:4-8).
+inline int stencil_coeffs_double(uint32_t npts, double w[3][4]) {
+  Coeffs c;
+  const int R = stencil_coeffs(npts, c);
+  if (R < 0) return -1;
+  for (int a = 0; a < 3; a++)
+    for (int r = 0; r < 4; r++) w[a][r] = (double)c.w[a][r];
+  if (npts == 7) return R;
+  volatile double f11 = (double)1.1f, f09 = (double)0.9f;
+  for (int r = 0; r < R; r++) {
+    const double cx = (double)c.w[0][r];  // the literal itself
+    w[1][r] = cx * f11;
+    w[2][r] = cx * f09;
+  }
+  return R;
+}
+
 // Evaluation order of the neighbour pairs, as the reference's expressions
 // are written (stencil_7.h:20; stencil_13.h:40-43; stencil_25.h:64-67):
 // 7-point (i,1),(j,1),(k,1); 13/25-point k(R..1), j(R..1), i(R..1).

@@ -44,6 +44,7 @@ struct sth_ctx {
   void *staging = nullptr;
   size_t staging_bytes = 0;
   sth_tuning tune{};
+  int weights = STH_WEIGHTS_FP32;  // FP64 storage: how its weights are formed
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timing = false;
   uint64_t launches = 0;

@@ -51,6 +51,7 @@ struct SweepArgs {
   int ref_extra;        // REF: updates of v after the first (u, s are read-only)
   float c0, c1, c2n;    // LEAPFROG coefficients (c2n: this step's source coefficient)
   float w[12];          // per-term weight, evaluation order (LEAPFROG: 2*w)
+  double wd[12];        // FP64 storage: the same weights as FP64 numbers
   Q16Slot *qs;          // 16-bit storage: the 3-slot ring
   int slot;             // slot read by this step
   int ubuf, vbuf;       // ring indices (0 = a, 1 = b) of u and v
@@ -358,6 +359,16 @@ __device__ __forceinline__ void sweep_tile(
   auto vload_nt = [](BufRes p, uint32_t ob) { return buf_load<VT, 2>(p, ob); };
   auto sload = [](BufRes p, uint32_t ob) { return buf_load<ST, 0>(p, ob); };
 
+  // Per-term weight in the arithmetic type: FP64 storage takes its own FP64
+  // weights (the FP32 constants promoted, or the reference's -DUSE_DOUBLE
+  // constants), FP32 and 16-bit storage the FP32 ones.
+  auto wt = [&](int t) -> AT {
+    if constexpr (FMT == FMT_F64)
+      return a.wd[t];
+    else
+      return a.w[t];
+  };
+
   // Register queue: q[m][t].get(c) = u at plane i - R + t.
   QCell<AT, V> q[MJ][2 * R + 1];
 #pragma unroll
@@ -514,15 +525,15 @@ __device__ __forceinline__ void sweep_tile(
         if constexpr (MODE == MODE_REF) {
           // stencil_N_ref_1: p = coef0*u0 + sum c*pair (the first product
           // rounded, the rest fused), then v0 - A*u0 + s0*p.
-          AT p = fmaT((AT)a.coef0, u0, (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]));
+          AT p = fmaT((AT)a.coef0, u0, wt(0) * pair(Tm::axis[0], Tm::r[0]));
 #pragma unroll
-          for (int t = 1; t < T; t++) p = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), p);
+          for (int t = 1; t < T; t++) p = fmaT(wt(t), pair(Tm::axis[t], Tm::r[t]), p);
           out = fmaT(s0, p, fmaT((AT)a.mA, u0, v0));
           for (int r = 0; r < a.ref_extra; r++) out = fmaT(s0, p, fmaT((AT)a.mA, u0, out));
         } else {
-          AT Lp = (AT)a.w[0] * pair(Tm::axis[0], Tm::r[0]);
+          AT Lp = wt(0) * pair(Tm::axis[0], Tm::r[0]);
 #pragma unroll
-          for (int t = 1; t < T; t++) Lp = fmaT((AT)a.w[t], pair(Tm::axis[t], Tm::r[t]), Lp);
+          for (int t = 1; t < T; t++) Lp = fmaT(wt(t), pair(Tm::axis[t], Tm::r[t]), Lp);
           if constexpr (FMT == FMT_F16 || FMT == FMT_BF16)
             out = fmaT(kc2, s0, fmaT(kc1, Lp, fmaT(kc0, u0, -(v0 * kv))));
           else

@@ -136,7 +136,9 @@ def run_fe(sth, *args):
     (["--ys", "10", "--yb", "3"], "--ys must by an exact multiple of --yb"),
     (["--iter"], "--iter requires a numeric argument"),
     (["--sbm", "20"], "--sbm must be smaller or equal to"),
-    (["--mode", "ref", "--dtype", "fp16"], "--mode ref runs FP32 storage only"),
+    (["--mode", "ref", "--dtype", "fp16"], "--mode ref runs FP32 or FP64 storage only"),
+    (["--host-fields", "--mode", "leapfrog", "--dtype", "bf16"],
+     "--host-fields needs FP32 or FP64 storage"),
     (["--stencil", "9"], "--stencil must be 7, 13 or 25"),
     (["--stencil", "25", "--zh", "2"], "halos must be at least the stencil radius 4"),
 ])

@@ -263,3 +263,59 @@ def test_two_step_batch_mixed_with_other_methods(gpu, orc):
             if k == 1 and q == 1:
                 b = new_v.copy()
         assert np.array_equal(got[q].view(np.uint32), a.view(np.uint32)), q
+
+
+def test_batch_refuses_mixed_parities(gpu, orc):
+    """A shot stepped on its own between batch calls holds u in its other
+    buffer: the next batch call refuses (STH_ERR_STATE) instead of stepping
+    that shot on swapped buffers; once every shot has the same parity again
+    the batch runs and matches the oracle."""
+    npts = 13
+    R = orc.radius(npts)
+    dims = DIMS[:3]
+    fields = shot_fields(orc, dims, npts, 40)
+    c0, c1 = coeffs(orc, npts)
+    with gpu.Batch(dims, npts=npts) as bt:
+        load(bt, fields, R)
+        bt.leapfrog(2, c0, c1, None)
+        bt.sync()
+        bt.shot(1).leapfrog(1, c0, c1)
+        bt.shot(1).sync()
+        with pytest.raises(gpu.SthError) as e:
+            bt.leapfrog(2, c0, c1, None)
+        assert e.value.status == gpu.STH_ERR_STATE
+        for q in (0, 2):
+            bt.shot(q).leapfrog(1, c0, c1)
+            bt.shot(q).sync()
+        bt.leapfrog(3, c0, c1, None, method=gpu.BATCH_TWO_STEP)
+        got = [bt.shot(q).download("u", R) for q in range(len(dims))]
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, None, 6)
+        want = a if newest == 0 else b
+        assert np.array_equal(got[q].view(np.uint32), want.view(np.uint32)), q
+
+
+def test_batch_autotune_picks_fused_chunks_and_stays_exact(gpu, orc):
+    """The batch autotuner times the one-step tiles and the fused pass's
+    i-chunk counts, names the faster method, and a run with its choice
+    matches the oracle per shot."""
+    npts = 13
+    R = orc.radius(npts)
+    dims = [(40, 60, 100), (64, 48, 96), (30, 96, 128)]
+    fields = shot_fields(orc, dims, npts, 50)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-2, 1e-2, 9).astype(np.float32)
+    with gpu.Batch(dims, npts=npts) as bt:
+        best, ms = bt.autotune(reps=3)
+        assert ms > 0
+        method = bt.best_method()
+        assert method in (gpu.BATCH_PERSISTENT, gpu.BATCH_FUSED)
+        assert best["chunks"] in (0, 1, 2, 3, 4, 6, 8, 12)
+        assert bt.get_tuning()["chunks"] == best["chunks"]
+        load(bt, fields, R)
+        bt.leapfrog(9, c0, c1, c2, method=method, graph=True)
+        got = [bt.shot(q).download("u", R) for q in range(len(dims))]
+    for q, (g, a, b, s) in enumerate(fields):
+        newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, 9)
+        want = a if newest == 0 else b
+        assert np.array_equal(got[q].view(np.uint32), want.view(np.uint32)), q

@@ -200,3 +200,90 @@ def test_autotune_7pt_result_is_bit_exact(gpu, orc):
     wu, wv = (a, b) if newest == 0 else (b, a)
     assert np.array_equal(gu.view(np.uint32), wu.view(np.uint32))
     assert np.array_equal(gv.view(np.uint32), wv.view(np.uint32))
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("dtype", ["fp32", "fp64"])
+def test_frontend_host_fields_with_reference_halos(gpu, npts, dtype):
+    """README.md:92's halos (--xh 7 --yh 7 --zh 16) on the reference's host
+    arrays: init_arrays on the host over the whole allocation, upload with
+    that layout, sweeps, download and the reference's check on the host
+    array; FP64 is the -DUSE_DOUBLE build's arithmetic."""
+    r = run_fe(gpu, "--xs", "40", "--ys", "26", "--zs", "70", "--xh", "7", "--yh", "7",
+               "--zh", "16", "--iter", "10", "--stencil", str(npts), "--host-fields",
+               "--dtype", dtype)
+    assert r.returncode == 0, r.stderr
+    assert "host arrays: halo 7x7x16" in r.stdout
+    assert re.findall(r"check: (\d+) errors found", r.stdout) == ["0", "0"]
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp64"])
+def test_frontend_host_fields_leapfrog_norms_agree(gpu, dtype):
+    """A LEAPFROG run through the host arrays prints the same device norms as
+    the same run on device-only fields, and the host's max |u| of the
+    downloaded field equals the device's."""
+    args = ["--xs", "30", "--ys", "33", "--zs", "80", "--iter", "7", "--stencil", "13",
+            "--mode", "leapfrog", "--init", "gauss", "--c1", "0.2", "--dtype", dtype]
+    a = run_fe(gpu, *args)
+    b = run_fe(gpu, *args, "--host-fields", "--xh", "5", "--yh", "6", "--zh", "9")
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    na = re.search(r"norms: max \|u\| = ([0-9.e+-]+), \|\|u\|\|_2 = ([0-9.e+-]+)", a.stdout)
+    nb = re.search(r"norms: max \|u\| = ([0-9.e+-]+), \|\|u\|\|_2 = ([0-9.e+-]+)", b.stdout)
+    assert na and nb and na.groups() == nb.groups()
+    h = re.search(r"host: max \|u\| = ([0-9.e+-]+)", b.stdout)
+    assert h and h.group(1) == nb.group(1)
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp16"])
+def test_frontend_tune_keeps_the_physics(gpu, dtype):
+    """--tune times schedules on the live fields; the timed run must still
+    start from the initial state: LEAPFROG on the Gaussian pulse prints the
+    same norms with and without --tune."""
+    args = ["--xs", "40", "--ys", "40", "--zs", "64", "--iter", "8", "--stencil", "25",
+            "--mode", "leapfrog", "--init", "gauss", "--c1", "0.16", "--dtype", dtype]
+    a = run_fe(gpu, *args)
+    b = run_fe(gpu, *args, "--tune")
+    assert a.returncode == 0 and b.returncode == 0, (a.stderr, b.stderr)
+    assert "autotune:" in b.stdout
+    na = re.search(r"norms: .*", a.stdout)
+    nb = re.search(r"norms: .*", b.stdout)
+    assert na and nb and na.group(0) == nb.group(0)
+
+
+def test_frontend_default_c0_is_constant_preserving(gpu):
+    """Without --c0 the 25-point LEAPFROG takes c0 = 2 - 12 c1 (the value
+    sth_leapfrog_c0 gives), not 2 - 6 c1."""
+    c0 = gpu.leapfrog_c0(25, 0.16)
+    args = ["--xs", "36", "--ys", "30", "--zs", "70", "--iter", "6", "--stencil", "25",
+            "--mode", "leapfrog", "--init", "gauss", "--c1", "0.16"]
+    outs = []
+    for extra in ([], ["--c0", repr(float(c0))], ["--c0", repr(2.0 - 6.0 * 0.16)]):
+        r = run_fe(gpu, *args, *extra)
+        assert r.returncode == 0, r.stderr
+        outs.append(re.search(r"norms: .*", r.stdout).group(0))
+    assert outs[0] == outs[1] and outs[0] != outs[2]
+
+
+@pytest.mark.parametrize("lane", [4, 16])
+def test_bw_probe_streams(gpu, orc, lane):
+    """The bandwidth probe moves what it says: one read and one write copy u
+    into v; three reads and one write store u + s + v into v."""
+    g = orc.Grid(20, 22, 64, 1, 1, 1)
+    u, s, v = (orc.fill_uniform(g, 9, f) for f in range(3))
+    with gpu.Stencil(20, 22, 64, npts=7) as ctx:
+        nbytes = ctx.layout().bytes // 16 * 16
+        for name, x in (("u", u), ("s", s), ("v", v)):
+            ctx.upload(name, x, 1)
+        assert ctx.bw_probe(1, 1, lane, nbytes, reps=1) > 0
+        y = ctx.download("v", 1)
+        assert np.array_equal(y[g.interior], u[g.interior])
+        ctx.upload("v", v, 1)
+        ctx.bw_probe(3, 1, lane, nbytes, reps=1)  # warm-up + 1: v <- u + s + v twice
+        y = ctx.download("v", 1)
+        w = u + s + v
+        w = u + s + w
+        assert np.array_equal(y[g.interior], w[g.interior])
+        assert ctx.bw_probe(2, 2, lane, nbytes, reps=1) > 0
+        assert ctx.bw_probe(3, 0, lane, nbytes, reps=1) > 0
+        with pytest.raises(gpu.SthError):
+            ctx.bw_probe(0, 0, lane, nbytes)

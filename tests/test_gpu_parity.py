@@ -254,3 +254,35 @@ def test_launch_modes_agree(gpu, orc, dtype, npts):
                 ctx.leapfrog(3, c0, c1, 1e-3)
             out.append(ctx.download("u", R, dtype=np.float64))
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "fp16"])
+def test_compare_matches_numpy(gpu, orc, dtype):
+    """sth_compare (the error of a run against its FP64 twin, on the GPU)
+    against the same figures computed with numpy from the downloads."""
+    shape, npts = (23, 30, 77), 13
+    R = orc.radius(npts)
+    c1 = 0.16
+    c0 = gpu.leapfrog_c0(npts, c1)
+    runs = {}
+    ctxs = [gpu.Stencil(*shape, npts=npts, dtype=dt) for dt in (dtype, "fp64")]
+    try:
+        for ctx in ctxs:
+            ctx.init_gaussian("u", 1.0, 4.0, 1e-3, 1, 0)
+            ctx.init_gaussian("v", 1.0, 4.0, 1e-3, 1, 1)
+            ctx.init_point("s", 11, 15, 38, 1.0)
+            ctx.leapfrog(9, c0, c1, 1e-2)
+        mr, lr = ctxs[0].compare("u", ctxs[1])
+        x = ctxs[0].download("u", R, dtype=np.float64)
+        r = ctxs[1].download("u", R, dtype=np.float64)
+        with pytest.raises(gpu.SthError):
+            ctxs[0].compare("u", ctxs[0])
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+    err = np.abs(x - r)
+    want_mr = err.max() / np.abs(r).max()
+    want_lr = np.sqrt((err ** 2).sum() / (r ** 2).sum())
+    assert mr == want_mr  # the maxima are exact (FP64 on the GPU)
+    assert lr == pytest.approx(want_lr, rel=1e-9)
+    assert 0 < mr < (1e-5 if dtype == "fp32" else 1e-2)
