@@ -52,7 +52,7 @@ template <int NPTS, int MJ, int NW, int S = 2>
 struct Fused2Shape {
   static constexpr int R = Terms<NPTS>::R;
   static constexpr int WD = (S - 1) * R;
-  static constexpr int EJ = NW * MJ, EK = 64;  // step-one tile
+  static constexpr int EJ = NW * MJ, EK = 64;  // rows x columns the lanes hold
   // Output tile.  Its rows are 48 points (192 B): every tile writes whole
   // 64-byte segments that no other tile touches.  Rows of 64 - 2R points
   // would share partly written segments with the next tile, which at 1040^3
@@ -60,10 +60,15 @@ struct Fused2Shape {
   // more tiles) and made the time depend on where the allocator placed the
   // fields.
   static constexpr int OJ = EJ - 2 * WD;
-  static constexpr int OK = ((EK - 2 * WD) / 16) * 16;
-  static constexpr int LW = EK + 2 * R;  // u plane + R halo
-  static constexpr int LU = (EJ + 2 * R) * LW;
-  static constexpr int LR = EJ * EK;  // one plane of u_{n+1}, u_{n+2} or s on E1
+  static constexpr int OK = 48;
+  // The 64 lanes of a row hold the output columns centred, KO = 8 columns
+  // on each side: step one's columns (the output widened by WD) and their
+  // R-wide k-halo all lie inside the lanes' own 64 columns, so the u plane
+  // needs no separate k-halo (OK + 2 (WD + R) <= 64, i.e. S R <= 8).
+  static constexpr int KO = (EK - OK) / 2;
+  static constexpr int LW = EK;  // LDS u plane row: the lanes' 64 columns
+  static constexpr int LU = (EJ + 2 * R) * LW;  // plus the R-wide j-halo rows
+  static constexpr int LR = EJ * EK;  // one plane of u_{n+1}, u_{n+2} or s
   static constexpr int SR = (S - 1) * (R + 1);  // s ring slots
   static constexpr int LDS_BYTES = (2 * LU + (2 * (S - 1) + SR) * LR) * 4;
 };
@@ -105,7 +110,8 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   using Tm = Terms<NPTS>;
   using F = Fused2Shape<NPTS, MJ, NW, S>;
   constexpr int R = Tm::R, T = Tm::T;
-  constexpr int EJ = F::EJ, LW = F::LW, OJ = F::OJ, OK = F::OK, WD = F::WD, SR = F::SR;
+  constexpr int EJ = F::EJ, LW = F::LW, OJ = F::OJ, OK = F::OK, KO = F::KO, WD = F::WD,
+                SR = F::SR;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // u halo rows per wave
   constexpr int Q = 2 * R + 2;               // u queue: planes p-R-1 .. p+R
   // Queue j (1 <= j < S) holds the lane's u_{n+j} values: 2R+2 entries for
@@ -117,7 +123,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   constexpr int U = (S - 1) % 2 == 0 ? (S - 1) * (R + 1) : 2 * (S - 1) * (R + 1);
   static_assert(S >= 2 && S <= 5, "two to five steps per pass");
   static_assert(U % SR == 0 && U % 2 == 0, "ring slots and buffers repeat with the loop");
-  static_assert(MJ * 2 * R <= 64, "k-halo needs at most one entry per thread");
+  static_assert(KO >= WD + R, "step one and its k-halo inside the lanes' columns");
   static_assert(F::LDS_BYTES <= 160 * 1024, "LDS");
   // One LDS array: the two u planes (LU), two planes each of u_{n+1} ..
   // u_{n+S-1} (step j+1's j and k neighbours), then SR planes of s, each
@@ -133,7 +139,8 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // compiles to a branch rather than to exec-mask updates.
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tid = threadIdx.x;
-  const int je = tj * OJ - WD, ke = tk * OK - WD;  // E1 origin (global j, k)
+  // E1's first row; the lanes' first column (the output's first is lane KO)
+  const int je = tj * OJ - WD, ke = tk * OK - KO;
   const int64_t P = a.plane, Pj = a.pitch;
 
   // Byte offsets from the corner (i, -R, -R) of a plane.  Reads clamp to the
@@ -168,7 +175,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // row is (rowout, uniform) and its column is (kout, per lane).
   const int kg = ke + lane;
   const bool kin = kg >= 0 && kg < a.zs;
-  const bool kout = kin && lane >= WD && lane < WD + OK;
+  const bool kout = kin && lane >= KO && lane < KO + OK;
   uint32_t off[MJ];
   bool inner[MJ];
   // rowst[M][m]: the row is one of step M's (2 <= M < S: the tile widened by
@@ -195,24 +202,14 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     hrow[h] = hh < 2 * R ? (hh < R ? hh : EJ + hh) : -1;
     hoff[h] = hrow[h] >= 0 ? boff(je + trow, kg) : kNone;
   }
-  // the first EJ*2R threads (a wave-uniform condition when they fill whole
-  // waves)
-  const bool has_kh = EJ * 2 * R % 64 == 0 ? wv < EJ * 2 * R / 64 : tid < EJ * 2 * R;
-  uint32_t khoff = kNone;
-  int khpos = 0;
-  if (has_kh) {
-    const int row = tid / (2 * R), c = tid % (2 * R);
-    const int tcol = c < R ? c - R : 64 + c - R;
-    khoff = boff(je + row, ke + tcol);
-    khpos = (row + R) * LW + (c < R ? c : 64 + c);
-  }
 
   // LDS addressing.  Every LDS access of a step is a per-lane base plus a
   // compile-time offset below 64 KiB.  The bases pass through an empty asm
   // statement at each step, so the compiler cannot hoist one register per
   // distinct address out of the loop.
   //   lbL: LU[0] at row wv*MJ, column lane (offsets: buffer, row m+R+dr,
-  //        column R+dc, all >= 0);
+  //        column dc; the k-neighbour reads have dr = 0, so m+R >= 1 rows
+  //        keep row*64+dc >= 0);
   //   lbW: W[0] at E1 row wv*MJ-R, column lane-R (offsets: buffer, row
   //        m+R+dr, column R+dc);
   //   lbS: s slot 0 at E1 row wv*MJ, column lane (offsets: slot, row m).
@@ -228,8 +225,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   uint32_t lbS = sbase + (uint32_t)((wv * MJ * 64 + lane) * 4);
   uint32_t hadr[HJ];
 #pragma unroll
-  for (int h = 0; h < HJ; h++) hadr[h] = lu0 + (uint32_t)((max(hrow[h], 0) * LW + lane + R) * 4);
-  const uint32_t kadr = lu0 + (uint32_t)(khpos * 4);
+  for (int h = 0; h < HJ; h++) hadr[h] = lu0 + (uint32_t)((max(hrow[h], 0) * LW + lane) * 4);
   auto lds = [](uint32_t adr) -> LF & { return *(LF *)(uintptr_t)adr; };
 
   // Register queues of the lane's points:
@@ -240,7 +236,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   //              reads entries gcen(j) -+ R (its plane p_{j+1} and i
   //              neighbours), step j+2 entry 0 (u_{n+j} at p_{j+2}).
   float q[MJ][Q], G[S][MJ][GL];
-  float hq[HJ][R + 1], kq[R + 1];
+  float hq[HJ][R + 1];
   // Step one from p0 (its planes past ie-1+WD are never used), the last
   // step (output plane p - (S-1)(R+1)) up to plast.
   const int p0 = ib - WD, plast = ie + (S - 1) * (R + 1) - 1;
@@ -263,7 +259,6 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     GB *Up = pbase(a.u, p0 + t);
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][t] = ld(Up, hoff[h]);
-    kq[t] = ld(Up, khoff);
   }
 
   // Staged loads of step p: u(p+R) with its halo, v(p), s(p); two sets used
@@ -272,7 +267,6 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
   // carried over a skipped load.
   struct Staged {
     float u[MJ], v[MJ], s[MJ], hj[HJ];
-    float hk;
   };
   Staged SA, SB;
   auto prefetch = [&](GB *Up, GB *Vp, GB *Sp, Staged &n) __attribute__((always_inline)) {
@@ -284,7 +278,6 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++) n.hj[h] = ld(Up, hoff[h]);
-    n.hk = ld(Up, khoff);
   };
   prefetch(pbase(a.u, p0 + R), pbase(a.v, p0), pbase(a.s, p0), SA);
   prefetch(pbase(a.u, p0 + 1 + R), pbase(a.v, p0 + 1), pbase(a.s, p0 + 1), SB);
@@ -316,7 +309,6 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][R] = n.hj[h];
-    kq[R] = n.hk;
     prefetch(Un, Vn, Sn, n);
     // Each pointer stays at its next plane clamped to [-R, last]: it moves
     // on only while that plane is inside.  The first step of a chunk is WD
@@ -342,7 +334,7 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     constexpr uint32_t LO = (k & 1) * LUB;  // this step's u buffer
     // LU element (row m+R+dr, column R+dc of this lane's rows)
     auto Lat = [&](int m, int dr, int dc) -> LF & {
-      return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + R + dc) * 4));
+      return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + dc) * 4));
     };
     // This step's buffer of u_{n+j} (E1 row of row m shifted by dr, column
     // lane+dc)
@@ -351,8 +343,8 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
                  (uint32_t)(((m + R + dr) * 64 + R + dc) * 4));
     };
 
-    // u(p) into LDS: the E1 rows, the halo rows, the k-halo columns; u_{n+j}
-    // at the plane of step j+1 (p-j(R+1)), j = 1 .. S-1, of the E1 rows.
+    // u(p) into LDS: the lanes' rows and the j-halo rows; u_{n+j} at the
+    // plane of step j+1 (p-j(R+1)), j = 1 .. S-1, of the lanes' rows.
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       Lat(m, 0, 0) = q[m][R + 1];
@@ -362,7 +354,6 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
 #pragma unroll
     for (int h = 0; h < HJ; h++)
       if (hrow[h] >= 0) lds(hadr[h] + LO) = hq[h][0];
-    if (has_kh) lds(kadr + LO) = kq[0];
     __syncthreads();
 
     // u_{n+M-2} at step M's plane: u_n from the u queue, else the oldest
@@ -487,7 +478,6 @@ __device__ __forceinline__ void sweep2_tile(const Sweep2Args &a, const int tk, c
     for (int t = 0; t < R; t++) {
 #pragma unroll
       for (int h = 0; h < HJ; h++) hq[h][t] = hq[h][t + 1];
-      kq[t] = kq[t + 1];
     }
   };
   for (int p = p0; p <= plast; p += U) {
