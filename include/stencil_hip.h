@@ -192,6 +192,24 @@ int sth_iterate(sth_ctx *ctx, uint32_t iterations);
  * then u and v swap.  c2 is a host array of `steps` per-step source
  * coefficients (NULL: all zero).  16-bit storage requires zero halos. */
 int sth_leapfrog(sth_ctx *ctx, uint32_t steps, float c0, float c1, const float *c2);
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  `steps` LEAPFROG steps as sth_leapfrog computes them (the
+ * one-step sweep, bit-identical to every other schedule; FP32 or FP64
+ * storage, FP64 with the FP32 weights promoted) on fields the caller
+ * allocated in the device layout of sth_layout for *g: u, v, s each
+ * layout.bytes, element (i,j,k) at origin + k + j*pitch + i*plane, halos
+ * included.  Queued on `stream` (a hipStream_t; NULL: the null stream),
+ * asynchronous.  u and v swap names every step as in sth_leapfrog: after an
+ * odd step count the newest field is in v's buffer.  The library allocates
+ * nothing and calls no allocating routine: the only memory it uses beyond
+ * the fields is a static device array of STH_NOALLOC_SCRATCH_BYTES per GPU
+ * (the work queue's counters, kept until the process ends), which every
+ * call clears with a kernel; calls that overlap on one GPU race on it.
+ * 16-bit storage: STH_ERR_UNSUPPORTED (its scale state lives in a context). */
+#define STH_NOALLOC_SCRATCH_BYTES 512
+int sth_leapfrog_noalloc(const sth_geom *g, void *u, void *v, const void *s, uint32_t steps,
+                         float c0, float c1, const float *c2, void *stream);
 /* Host-only: the centre coefficient that keeps a constant field constant,
  * c0 = 2 - c1 * sum 2*|2 c_axis,r| (the sum is 6 for 7 and 13 points, 12 for
  * 25; for 7 points this is README.md:17-20's c0 = 2 - 6 c1), rounded to FP32. */

@@ -87,6 +87,8 @@ SIGNATURES = {
     "sth_load": (C.c_int, [_VP, C.c_char_p]),
     "sth_iterate": (C.c_int, [_VP, C.c_uint32]),
     "sth_leapfrog": (C.c_int, [_VP, C.c_uint32, C.c_float, C.c_float, _P(C.c_float)]),
+    "sth_leapfrog_noalloc": (C.c_int, [_P(Geom), _VP, _VP, _VP, C.c_uint32, C.c_float,
+                                       C.c_float, _P(C.c_float), _VP]),
     "sth_leapfrog_c0": (C.c_int, [C.c_uint32, C.c_float, _P(C.c_float)]),
     "sth_ricker": (C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_double,
                              _P(C.c_float)]),

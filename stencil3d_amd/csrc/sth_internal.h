@@ -107,6 +107,10 @@ int ensure_twins(sth_ctx *c);
 void base_args(const sth_ctx *c, const Plan &p, SweepArgs &a);
 void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a);
 int push_slot(sth_ctx *c);
+// `steps` LEAPFROG steps with plan p (fused passes where they apply), and
+// one launch of plan p (persistent: the ticket base advances on the host).
+int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, const float *c2);
+int launch_one(sth_ctx *c, const Plan &p, int mode, SweepArgs &a);
 
 }  // namespace detail
 }  // namespace sth
