@@ -87,16 +87,22 @@ class Twin:
         self.ctx.close()
 
 
-def run_throughput(S, n, dtype, steps, warmup, twin=None, tuning=None, copy_bytes=0):
+def run_throughput(S, n, dtype, steps, warmup, twin=None, tuning=None, copy_bytes=0, place=0):
     """Returns wall seconds of `steps` timed steps, the HIP-event kernel time
     and launch count, the tuning used, the u norms, (copy_bytes > 0) the mean
-    time of a plain copy kernel of copy_bytes, and (twin) the error of u
-    against the FP64 twin after warmup + steps."""
+    time of a plain copy kernel of copy_bytes, (twin) the error of u against
+    the FP64 twin after warmup + steps, and (place > 0) the placement search:
+    the fields kept are the fastest of the first allocation and `place`
+    more (sth_place), before the fields are initialised."""
     c0, c1 = leap_coeffs(S, NPTS)
     with S.Stencil(n, n, n, npts=NPTS, dtype=dtype) as ctx:
-        init_pulse(ctx)
         if tuning:
             ctx.set_tuning(**tuning)
+        placed = None
+        if place:
+            ms, chosen = ctx.place(place)
+            placed = {"tries": place, "ms_per_step": [round(x, 4) for x in ms], "chosen": chosen}
+        init_pulse(ctx)
         ctx.leapfrog(warmup, c0, c1)
         ctx.sync()
         t0 = time.perf_counter()
@@ -111,7 +117,7 @@ def run_throughput(S, n, dtype, steps, warmup, twin=None, tuning=None, copy_byte
             twin.advance_to(warmup + steps)
             err = err_dict(ctx, twin.ctx)
         copy_ms = ctx.stream_copy(copy_bytes, 5) if copy_bytes else None
-    return t1 - t0, kms, launches, used, (mx, l2), copy_ms, err
+    return t1 - t0, kms, launches, used, (mx, l2), copy_ms, err, placed
 
 
 def run_error_growth(S, dtypes=("fp32", "fp16", "bf16"), n=256, checkpoints=(100, 250, 500, 1000)):
@@ -239,14 +245,15 @@ def run_batch(S, steps=100, nshots=64, seed=2, sample=4):
     return out
 
 
-def run_other_stencils(S, n, steps, warmup, traffic_of):
+def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
     """Every stencil size at n^3, FP32: REF (the reference's update, what
     hstencil3d runs), LEAPFROG (the default schedule) and REF at ten sweeps
     per launch; Gpts/s from HIP events, and the error against the FP64 twin.
     `alg_equiv_gbs` is 16 B per point-update over the time: the bandwidth a
     one-step sweep would need to match the row (above 8 TB/s for the
     multi-step schedules); `traffic_gbs` is the counted traffic of the
-    kernel (profiles/traffic.json) over the same time, where profiled."""
+    kernel (profiles/traffic.json) over the same time, where profiled.
+    LEAPFROG rows keep the fastest of `place` + 1 field allocations."""
     out = {}
     for npts in (7, 13, 25):
         c0, c1 = leap_coeffs(S, npts)
@@ -255,6 +262,10 @@ def run_other_stencils(S, n, steps, warmup, traffic_of):
             with S.Stencil(n, n, n, npts=npts, dtype="fp32") as ctx:
                 twin = Twin(S, n, npts, "leapfrog" if mode == "leapfrog" else "ref")
                 try:
+                    placed = None
+                    if mode == "leapfrog" and place:
+                        ms_c, kept = ctx.place(place)
+                        placed = {"ms_per_step": [round(x, 4) for x in ms_c], "chosen": kept}
                     init_pulse(ctx)
                     if mode == "ref_10_per_launch":
                         # opt-in: ten REF sweeps per launch (bit-identical to ten
@@ -285,6 +296,7 @@ def run_other_stencils(S, n, steps, warmup, traffic_of):
                 r["traffic_gbs"] = round(tb / (ms / nl * 1e-3) / 1e9, 1)
             if mode == "leapfrog":
                 r["tuning"] = tune
+                r["placement"] = placed
             row[mode] = r
         out[str(npts)] = row
         log(f"[bench] fp32 {npts}-pt: REF {row['ref']['gpts']} Gpts/s, "
@@ -431,6 +443,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-other", action="store_true")
     ap.add_argument("--no-probes", action="store_true")
+    ap.add_argument("--place", type=int, default=3,
+                    help="FP32 runs: keep the fastest of the first field allocation and this "
+                         "many more (sth_place; 0: the first allocation)")
     ap.add_argument("--only-probes", action="store_true",
                     help="run the bandwidth probes alone (for counter runs)")
     args = ap.parse_args()
@@ -451,8 +466,11 @@ def main():
     twin = None if args.no_error else Twin(S, n, NPTS)
     log(f"[bench] fp32 {NPTS}-pt {n}^3: {W} warm-up + {K} timed steps")
     copy_bytes = (n * n * n * 4) // 16 * 16
-    wall, kms, launches, tune, nrm, copy_ms, err32 = run_throughput(
-        S, n, "fp32", K, W, twin=twin, copy_bytes=copy_bytes)
+    wall, kms, launches, tune, nrm, copy_ms, err32, placed = run_throughput(
+        S, n, "fp32", K, W, twin=twin, copy_bytes=copy_bytes, place=args.place)
+    if placed:
+        log(f"[bench] placement: ms/step of the candidates {placed['ms_per_step']}, "
+            f"kept {placed['chosen']}")
     copy_gbs = 2.0 * copy_bytes / (copy_ms * 1e-3) / 1e9
     gpts = pts * K / wall / 1e9
     # A launch advances `units` steps (2 to 5 for a fused pass):
@@ -467,20 +485,21 @@ def main():
     one_step = None
     if tune.get("steps_per_pass", 0) >= 2:
         # the one-step sweep on the same fields: the baseline of the FP16 ratio
-        w1, k1, l1, t1, _, _, e1 = run_throughput(S, n, "fp32", K, W, twin=twin,
-                                                  tuning={"steps_per_pass": 1})
+        w1, k1, l1, t1, _, _, e1, p1 = run_throughput(S, n, "fp32", K, W, twin=twin,
+                                                      tuning={"steps_per_pass": 1},
+                                                      place=args.place)
         g1 = pts * K / w1 / 1e9
         one_step = {"gpts": round(g1, 3), "ms_per_step": round(w1 / K * 1e3, 4),
                     "kernel_ms": round(k1 / l1, 4),
                     "alg_gbs": round(16.0 * pts / (k1 / l1 * 1e-3) / 1e9, 1), "tuning": t1,
-                    "err_vs_fp64": e1}
+                    "err_vs_fp64": e1, "placement": p1}
         log(f"[bench] fp32 one step per pass: {g1:.2f} Gpts/s")
 
     extra = {}
     if not args.no_16bit:
         for dt in ("fp16", "bf16"):
             log(f"[bench] {dt}: {W} warm-up + {K} timed steps")
-            w2, k2, l2, t2, _, _, e2 = run_throughput(S, n, dt, K, W, twin=twin)
+            w2, k2, l2, t2, _, _, e2, _ = run_throughput(S, n, dt, K, W, twin=twin)
             g2 = pts * K / w2 / 1e9
             a2 = 8.0 * pts / (k2 / l2 * 1e-3) / 1e9
             extra[dt] = {"gpts": round(g2, 3), "ms_per_step": round(w2 / K * 1e3, 4),
@@ -507,7 +526,7 @@ def main():
     if not args.no_other:
         def traffic_of(nn, npts, dcode, mode, tn):
             return measured_traffic(nn, dcode, tn, npts=npts, mode=mode)[0]
-        extra["fp32_by_stencil"] = run_other_stencils(S, n, 24, 6, traffic_of)
+        extra["fp32_by_stencil"] = run_other_stencils(S, n, 24, 6, traffic_of, place=args.place)
     if not args.no_error:
         extra["physics_7pt_128"] = run_physics_7pt(S)
         log(f"[bench] 7-pt 128^3 physics: {extra['physics_7pt_128']}")
@@ -557,6 +576,7 @@ def main():
                      "copy_kernel_gbs": round(copy_gbs, 1),
                      "copy_kernel_bytes": 2 * copy_bytes},
         "cpu_baseline": cpu,
+        "placement": placed,
         "norms_u": {"max": nrm[0], "l2": nrm[1]},
         "err_vs_fp64_1040": err32,
     }

@@ -278,6 +278,24 @@ int sth_stream_copy(sth_ctx *ctx, uint64_t bytes, uint32_t reps, double *ms);
  * counters at either access width. */
 int sth_bw_probe(sth_ctx *ctx, uint32_t reads, uint32_t writes, uint32_t lane_bytes,
                  uint64_t bytes, uint32_t grid, uint32_t reps, double *ms);
+/* Placement of the field buffers, the counterpart of the reference's shifted
+ * array starts (main.cpp:243-276, "to minimize CL bank conflicts").  On
+ * MI355X the sweep's time depends on the physical memory behind its fields
+ * (a shift of the virtual start changes nothing measurable): the same FP32
+ * 25-point pass at 1040^3 runs about 2.62 or about 2.82 ms per step depending
+ * on the allocation, with the mixed read/write streaming rate and the memory
+ * latency of the slow allocations lower and higher.  sth_place allocates up
+ * to `tries` (0-8) further sets of the context's buffers (a, b, s and the
+ * twins when the LEAPFROG default is a fused pass), as many as fit while a
+ * sixteenth of the device (at least 4 GiB) stays free, all held at once;
+ * times the context's LEAPFROG default on each and on the current set; keeps
+ * the fastest and frees the others.  ms[0 .. tries] (if not null) receives
+ * the ms per step of the current set and of each candidate (0: not
+ * allocated), *chosen (if not null) the index kept.  Call it right after
+ * sth_create and after sth_set_tuning: field contents are undefined
+ * afterwards.  FP32 and FP64 storage (else STH_ERR_UNSUPPORTED); not for a
+ * batch's shots. */
+int sth_place(sth_ctx *ctx, uint32_t tries, float *ms, uint32_t *chosen);
 
 /* ---- batching: many independent grids ("shots") advanced together ------ */
 /* The reference runs one grid per process; a batch holds nshots grids of

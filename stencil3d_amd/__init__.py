@@ -106,6 +106,7 @@ SIGNATURES = {
     "sth_stream_copy": (C.c_int, [_VP, C.c_uint64, C.c_uint32, _P(C.c_double)]),
     "sth_bw_probe": (C.c_int, [_VP, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32,
                                C.c_uint32, _P(C.c_double)]),
+    "sth_place": (C.c_int, [_VP, C.c_uint32, _P(C.c_float), _P(C.c_uint32)]),
     "sth_batch_create": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32,
                                    _P(_VP)]),
     "sth_batch_destroy": (C.c_int, [_VP]),
@@ -390,6 +391,16 @@ class Stencil:
                                C.byref(ms)), "sth_bw_probe")
         return ms.value
 
+    def place(self, tries=3):
+        """Keeps the fastest of the current field buffers and up to `tries`
+        further allocations of them (the LEAPFROG default timed on each);
+        returns (ms per step of each set, 0 where none was allocated; the
+        index kept).  Field contents are undefined afterwards."""
+        ms = (C.c_float * (tries + 1))()
+        chosen = C.c_uint32()
+        _ok(lib().sth_place(self.handle, tries, ms, C.byref(chosen)), "sth_place")
+        return list(ms), chosen.value
+
     def timer_read(self):
         ms, n = C.c_double(), C.c_uint64()
         _ok(lib().sth_timer_read(self.handle, C.byref(ms), C.byref(n)), "sth_timer_read")
@@ -475,14 +486,6 @@ class Batch:
 
     def timer_start(self):
         _ok(lib().sth_batch_timer_start(self._h), "sth_batch_timer_start")
-
-    def bw_probe(self, reads, writes, lane_bytes, nbytes, reps=5, grid=0):
-        """Streaming probe (reads of u, s, v; writes of v and u's twin) of
-        nbytes per stream; returns mean ms per launch."""
-        ms = C.c_double()
-        _ok(lib().sth_bw_probe(self.handle, reads, writes, lane_bytes, nbytes, grid, reps,
-                               C.byref(ms)), "sth_bw_probe")
-        return ms.value
 
     def timer_read(self):
         ms, n = C.c_double(), C.c_uint64()

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <new>
 #include <string>
 #include <vector>
@@ -1374,6 +1375,113 @@ int sth_bw_probe(sth_ctx *c, uint32_t reads, uint32_t writes, uint32_t lane_byte
   float t = 0.0f;
   STH_TRY(hipEventElapsedTime(&t, c->ev0, c->ev1));
   if (ms) *ms = (double)t / reps;
+  return STH_OK;
+}
+
+// ---- placement of the field buffers ------------------------------------------
+
+// The LEAPFROG time of one candidate set of field buffers (a, b, s and, when
+// the pass is fused, the twins of a and b): ms per step over four passes
+// after one, on whatever the buffers hold.
+static int time_fields(sth_ctx *c, void *const set[5], bool twins, float *ms) {
+  for (int i = 0; i < 3; i++) c->base[i] = set[i];
+  c->twin[0] = twins ? set[3] : nullptr;
+  c->twin[1] = twins ? set[4] : nullptr;
+  c->ubuf = 0;
+  c->slot = 0;
+  c->twins_ok = true;  // the halos do not matter to the time
+  const float c1 = 0.1f, c0 = (float)(2.0 - lap_gain(c->g.npts) * c1);
+  const uint32_t spp = twins ? (uint32_t)make_plan2(c, c->tune).s : 1;
+  const Plan p = make_plan(c, c->tune);
+  int st = run_leap(c, p, spp, c0, c1, nullptr);
+  if (st) return st;
+  STH_TRY(hipEventRecord(c->ev0, c->stream));
+  st = run_leap(c, p, 4 * spp, c0, c1, nullptr);
+  if (st) return st;
+  STH_TRY(hipEventRecord(c->ev1, c->stream));
+  STH_TRY(hipEventSynchronize(c->ev1));
+  float t = 0.0f;
+  STH_TRY(hipEventElapsedTime(&t, c->ev0, c->ev1));
+  *ms = t / (float)(4 * spp);
+  return STH_OK;
+}
+
+int sth_place(sth_ctx *c, uint32_t tries, float *ms, uint32_t *chosen) {
+  if (!c || tries > 8) return STH_ERR_ARG;
+  if (c->fmt != FMT_F32 && c->fmt != FMT_F64) return STH_ERR_UNSUPPORTED;
+  if (c->no_fuse) return STH_ERR_STATE;  // a batch's shot: its tables hold the buffers
+  const bool twins = fused_applies(c, c->tune);
+  const int nb = twins ? 5 : 3;
+  if (twins) {
+    const int st = ensure_twins(c);
+    if (st) return st;
+  } else {
+    // twins left from an earlier tuning are rebuilt when next needed
+    for (int b = 0; b < 2; b++)
+      if (c->twin[b]) {
+        (void)hipFree(c->twin[b]);
+        c->twin[b] = nullptr;
+      }
+  }
+  // Candidates only where they fit, leaving a sixteenth of the device (at
+  // least 4 GiB) free.
+  size_t fr = 0, tot = 0;
+  STH_TRY(hipMemGetInfo(&fr, &tot));
+  const size_t spare = std::max<size_t>(tot / 16, (size_t)4 << 30), per = nb * c->L.bytes;
+  uint32_t n = 0;
+  while (n < tries && (size_t)(n + 1) * per + spare <= fr) n++;
+  std::vector<std::array<void *, 5>> sets(n + 1);
+  sets[0] = {c->base[0], c->base[1], c->base[2], c->twin[0], c->twin[1]};
+  // Every candidate stays allocated until the choice is made: a buffer freed
+  // and allocated again tends to come back on the same memory.
+  uint32_t got = 0;
+  for (uint32_t t = 1; t <= n; t++) {
+    sets[t] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool ok = true;
+    for (int i = 0; i < nb && ok; i++)
+      if (hipMalloc(&sets[t][i], c->L.bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        sets[t][i] = nullptr;
+        ok = false;
+      }
+    if (!ok) {
+      for (void *q : sets[t])
+        if (q) (void)hipFree(q);
+      break;
+    }
+    got = t;
+  }
+  const bool timing = c->timing;
+  c->timing = false;
+  std::vector<float> t_ms(got + 1, 0.0f);
+  int st = STH_OK;
+  // Three rounds over the sets, the least time of each: clocks ramp up over
+  // the first passes after an idle spell and drift under load, which a
+  // single pass over the sets in order would charge to the sets.
+  for (int round = 0; round < 3 && st == STH_OK; round++)
+    for (uint32_t t = 0; t <= got && st == STH_OK; t++) {
+      float x = 0.0f;
+      st = time_fields(c, sets[t].data(), twins, &x);
+      if (st == STH_OK && (round == 0 || x < t_ms[t])) t_ms[t] = x;
+    }
+  uint32_t best = 0;
+  if (st == STH_OK)
+    for (uint32_t t = 1; t <= got; t++)
+      if (t_ms[t] < t_ms[best]) best = t;
+  for (uint32_t t = 0; t <= got; t++)
+    if (t != best)
+      for (int i = 0; i < nb; i++) (void)hipFree(sets[t][i]);
+  for (int i = 0; i < 3; i++) c->base[i] = sets[best][i];
+  c->twin[0] = twins ? sets[best][3] : nullptr;
+  c->twin[1] = twins ? sets[best][4] : nullptr;
+  c->ubuf = 0;
+  c->slot = 0;
+  c->twins_ok = false;
+  c->timing = timing;
+  if (st) return st;
+  if (ms)
+    for (uint32_t t = 0; t <= tries; t++) ms[t] = t <= got ? t_ms[t] : 0.0f;
+  if (chosen) *chosen = best;
   return STH_OK;
 }
 

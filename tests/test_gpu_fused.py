@@ -429,3 +429,31 @@ def test_default_schedule_matches_oracle_mid_size(gpu, orc, npts, shape):
     want_u, want_v = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, 12)
     assert same_bits(gu, want_u)
     assert same_bits(gv, want_v)
+
+
+@pytest.mark.parametrize("npts,shape,steps", [(25, (128, 640, 816), 5), (13, (96, 640, 816), 7),
+                                              (7, (128, 288, 1056), 11)])
+def test_more_tiles_than_cus(gpu, orc, npts, shape, steps):
+    """More output tiles than CUs (272 or 264 at the default shapes), so the
+    work items of a pass take several rounds of the grid, with the default
+    and with an explicit i-chunk count: bit-identical to the oracle."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 31, 0.25)
+    c0, c1 = coeffs(orc, npts)
+    c2 = np.linspace(-1e-3, 2e-3, steps).astype(np.float32)
+    want_u, want_v = oracle_run(orc, npts, g, u, v, s, c0, c1, c2, steps)
+    for chunks in (0, 3):
+        with gpu.Stencil(*shape, npts=npts) as ctx:
+            ctx.set_tuning(chunks=chunks)
+            ctx.upload("u", u, R)
+            ctx.upload("v", v, R)
+            ctx.upload("s", s, R)
+            ctx.timer_start()
+            ctx.leapfrog(steps, c0, c1, c2)
+            _, launches = ctx.timer_read()
+            assert launches == 3  # two passes + one one-step sweep
+            gu = ctx.download("u", R)
+            gv = ctx.download("v", R)
+        assert same_bits(gu, want_u), chunks
+        assert same_bits(gv, want_v), chunks
