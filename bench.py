@@ -443,6 +443,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-other", action="store_true")
     ap.add_argument("--no-probes", action="store_true")
+    ap.add_argument("--only-placement", type=int, default=0, metavar="N",
+                    help="only the placement search of the headline context with N more "
+                         "allocations (its launches: 3 rounds over the sets, 5 passes each)")
     ap.add_argument("--place", type=int, default=3,
                     help="FP32 runs: keep the fastest of the first field allocation and this "
                          "many more (sth_place; 0: the first allocation)")
@@ -461,6 +464,13 @@ def main():
     pts = float(n) ** 3
     if args.only_probes:
         print(json.dumps({"probes": run_probes(S, n)}), flush=True)
+        return
+    if args.only_placement:
+        with S.Stencil(n, n, n, npts=NPTS) as ctx:
+            ms, chosen = ctx.place(args.only_placement)
+            print(json.dumps({"placement": {"tries": args.only_placement, "ms_per_step": ms,
+                                            "chosen": chosen, "tuning": ctx.get_tuning()}}),
+                  flush=True)
         return
 
     twin = None if args.no_error else Twin(S, n, NPTS)

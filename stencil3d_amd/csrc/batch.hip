@@ -121,7 +121,7 @@ struct sth_batch {
   Batch2Item *items2 = nullptr;
   size_t shot2_cap = 0, items2_cap = 0;
   int total_items2 = 0;
-  hipGraphExec_t exec = nullptr;
+  std::vector<hipGraphExec_t> execs;  // the graphs of the last graph-mode call
   uint32_t best_method = STH_BATCH_PERSISTENT;  // the autotuner's fastest method
   bool timing = false;
   double timed_ms = 0.0;
@@ -129,6 +129,9 @@ struct sth_batch {
 };
 
 namespace {
+
+// Kernel launches per captured graph (see sth_batch_leapfrog).
+constexpr uint32_t kGraphLaunches = 1024;
 
 hipError_t launch_batch(uint32_t npts, int mj, int nw, int vec, unsigned grid, hipStream_t st,
                         const BatchArgs &b) {
@@ -466,7 +469,7 @@ extern "C" {
 int sth_batch_destroy(sth_batch *b) {
   if (!b) return STH_ERR_ARG;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  for (hipGraphExec_t x : b->execs) (void)hipGraphExecDestroy(x);
   for (sth_ctx *c : b->shots)
     if (c) sth_destroy(c);
   if (b->shot_args) (void)hipFree(b->shot_args);
@@ -612,32 +615,68 @@ int sth_batch_leapfrog(sth_batch *b, uint32_t steps, float c0, float c1, const f
     return st;
   }
   if ((st = join_shots(b))) return st;
-  if (b->exec) {  // the previous call's graph has been launched before: free it
+  if (!b->execs.empty()) {  // the previous call's graphs have been launched: free them
     STH_TRY(hipStreamSynchronize(b->stream));
-    STH_TRY(hipGraphExecDestroy(b->exec));
-    b->exec = nullptr;
+    for (hipGraphExec_t x : b->execs) STH_TRY(hipGraphExecDestroy(x));
+    b->execs.clear();
   }
   uint64_t launches = 0;
   if (graph) {
-    hipGraph_t g = nullptr;
-    // a fresh ticket count for every replay of the graph
+    // The call is captured in segments of at most kGraphLaunches launches,
+    // one graph each (whole steps, and an even number of fused passes so
+    // every segment starts on the call's first buffer phase), then the
+    // graphs are launched in order.  Each graph starts from a zeroed ticket
+    // counter, so every replay draws the same tickets.  (One graph of 6400
+    // launches, 64 shots x 100 sequential steps, crashed rocprofv3's kernel
+    // trace on the host; segments of 1024 run under it.)
     const unsigned saved_base = b->ticket_base;
-    STH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = launch_zero_u32(b->ticket, 1, b->stream);
-    b->ticket_base = 0;
-    if (e == hipSuccess)
-      st = enqueue_all(b, S, passes, single, c0, c1, c2, base, parity0, &launches);
-    const hipError_t ec = hipStreamEndCapture(b->stream, &g);
-    if (e != hipSuccess || st != STH_OK || ec != hipSuccess) {
-      if (g) (void)hipGraphDestroy(g);
-      b->ticket_base = saved_base;
-      return st != STH_OK ? st : hip_status(e != hipSuccess ? e : ec);
+    auto capture = [&](auto &&enqueue) -> int {
+      hipGraph_t g = nullptr;
+      STH_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = launch_zero_u32(b->ticket, 1, b->stream);
+      b->ticket_base = 0;
+      const int sq = e == hipSuccess ? enqueue() : STH_OK;
+      const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+      if (e != hipSuccess || sq != STH_OK || ec != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return sq != STH_OK ? sq : hip_status(e != hipSuccess ? e : ec);
+      }
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e != hipSuccess) return hip_status(e);
+      b->execs.push_back(x);
+      return STH_OK;
+    };
+    const uint32_t seg_passes = kGraphLaunches & ~1u;
+    for (uint32_t q = 0; q < passes && st == STH_OK;) {
+      const uint32_t np = std::min(passes - q, seg_passes), done = (uint32_t)S * q;
+      st = capture([&] {
+        return enqueue_passes(b, S, np, c0, c1, c2 ? c2 + done : nullptr,
+                              (parity0 + (int)done) & 1, &launches);
+      });
+      q += np;
     }
-    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) return hip_status(e);
+    if (single) {
+      const int method = S ? STH_BATCH_PERSISTENT : base;
+      const uint32_t done = (uint32_t)S * passes;
+      const uint32_t per_step = method == STH_BATCH_SEQUENTIAL ? (uint32_t)b->shots.size() : 1;
+      const uint32_t seg = std::max<uint32_t>(1, kGraphLaunches / per_step);
+      for (uint32_t k = 0; k < single && st == STH_OK;) {
+        const uint32_t nk = std::min(single - k, seg), at = done + k;
+        st = capture([&] {
+          return enqueue_steps(b, nk, c0, c1, c2 ? c2 + at : nullptr, method,
+                               (parity0 + (int)at) & 1, &launches);
+        });
+        k += nk;
+      }
+    }
+    if (st != STH_OK) {
+      b->ticket_base = saved_base;
+      return st;
+    }
     if (b->timing) STH_TRY(hipEventRecord(b->ev0, b->stream));
-    STH_TRY(hipGraphLaunch(b->exec, b->stream));
+    for (hipGraphExec_t x : b->execs) STH_TRY(hipGraphLaunch(x, b->stream));
     if (b->timing) STH_TRY(hipEventRecord(b->ev1, b->stream));
   } else {
     if (b->timing) STH_TRY(hipEventRecord(b->ev0, b->stream));

@@ -319,3 +319,32 @@ def test_batch_autotune_picks_fused_chunks_and_stays_exact(gpu, orc):
         newest = orc.leapfrog_f32(npts, g, a, b, s, c0, c1, c2, 9)
         want = a if newest == 0 else b
         assert np.array_equal(got[q].view(np.uint32), want.view(np.uint32)), q
+
+
+def test_graph_segments_match_eager(gpu, orc):
+    """Graphed calls of more than 1024 launches are captured in several
+    graphs: 48 shots x 23 sequential steps (1104 launches: 21 steps, then 2),
+    and 1027 five-step passes plus 3 persistent steps on two small 7-point
+    shots (1030 launches: 1024 passes, then 3 passes and the steps).  Each
+    equals the same call made eagerly, bit for bit."""
+    npts, R = 7, 1
+    c0, c1 = coeffs(orc, npts)
+    dims = [(8 + q % 5, 10 + q % 3, 16 + q % 7) for q in range(48)]
+    cases = [(dims, 23, gpu.BATCH_SEQUENTIAL), (dims[:2], 5 * 1027 + 3, gpu.BATCH_FUSED)]
+    for dd, steps, method in cases:
+        c2 = np.linspace(-1e-3, 1e-3, steps).astype(np.float32)
+        outs = []
+        for graph in (False, True):
+            fields = shot_fields(orc, dd, npts, 70)
+            with gpu.Batch(dd, npts=npts) as bt:
+                load(bt, fields, R)
+                bt.timer_start()
+                bt.leapfrog(steps, c0, c1, c2, method=method, graph=graph)
+                _, launches = bt.timer_read()
+                outs.append([(bt.shot(q).download("u", R), bt.shot(q).download("v", R))
+                             for q in range(len(dd))])
+            assert launches > 1024
+        for q in range(len(dd)):
+            for f in range(2):
+                assert np.array_equal(outs[0][q][f].view(np.uint32), outs[1][q][f].view(np.uint32)), \
+                    (method, q, f)
