@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 HIP_SOURCES = ["sth_api.hip", "fields.hip", "sweep_7.hip", "sweep_13.hip", "sweep_25.hip",
-               "sweep2.hip", "batch.hip", "noalloc.hip"]
+               "sweep16.hip", "sweep2.hip", "batch.hip", "noalloc.hip"]
 
 
 def _run(cmd, quiet):

@@ -246,10 +246,18 @@ void fill_weights(const sth_ctx *c, bool leap, SweepArgs &a) {
 }
 
 
+// A one-step tile the context's sweeps can run: a compiled sweep_kernel
+// tile, or (FP16, two k points per lane) a packed tile.
+bool shape_supported(const sth_ctx *c, int mj, int nw, int vec) {
+  if (packed16(c->fmt, MODE_LEAP, vec)) return tile16_supported(mj, nw);
+  return tile_supported(mj, nw, vec);
+}
+
 Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   Plan p;
   TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, t);
-  if (!tile_supported(ts.mj, ts.nw, ts.vec)) ts = default_tile(c->fmt, c->g.ys, c->g.zs, sth_tuning{});
+  if (!shape_supported(c, ts.mj, ts.nw, ts.vec))
+    ts = default_tile(c->fmt, c->g.ys, c->g.zs, sth_tuning{});
   p.mj = ts.mj;
   p.nw = ts.nw;
   p.vec = ts.vec;
@@ -270,6 +278,12 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   }
   ch = std::min(ch, xs);
   p.chunk_len = (xs + ch - 1) / ch;
+  if (packed16(c->fmt, MODE_LEAP, p.vec)) {
+    // the packed FP16 sweep runs whole turns of its rings: chunks of the
+    // nearest multiple of a turn leave idle steps in the last chunk only
+    const int u = sweep16_turn(c->R);
+    p.chunk_len = std::min(xs, std::max(u, (p.chunk_len + u / 2) / u * u));
+  }
   p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
   p.grid = (unsigned)(p.ntk * p.ntj * p.chunks);
   p.items = p.grid;
@@ -279,7 +293,9 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
     // 10 % faster than one workgroup per tile -- no partly filled last round,
     // and a workgroup's next tile starts without a launch)
     int per_cu = 0;
-    const void *fn = sweep_fn((int)c->g.npts, MODE_LEAP, c->fmt, p.mj, p.nw, p.vec, 0);
+    const void *fn = packed16(c->fmt, MODE_LEAP, p.vec)
+                         ? sweep16_fn((int)c->g.npts, p.mj, p.nw)
+                         : sweep_fn((int)c->g.npts, MODE_LEAP, c->fmt, p.mj, p.nw, p.vec, 0);
     if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * p.nw, 0) !=
                    hipSuccess || per_cu < 1) {
       (void)hipGetLastError();
@@ -354,7 +370,10 @@ int launch_one(sth_ctx *c, const Plan &p, int mode, SweepArgs &a) {
     a.band = p.band;
   }
   const hipError_t e =
-      launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.edge, p.grid, c->stream, a);
+      packed16(c->fmt, mode, p.vec)
+          ? launch_sweep16((int)c->g.npts, p.mj, p.nw, p.grid, c->stream, a)
+          : launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.edge, p.grid,
+                         c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (p.persist && p.band >= (int)p.items) {
     c->tick_next[0] += p.items + p.grid;  // one queue
@@ -1169,7 +1188,7 @@ int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
       t->steps_per_pass >= 2 && fused2_supported((int)c->g.npts, (int)t->rows_per_wave,
                                                  (int)t->waves, (int)t->steps_per_pass);
   const TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, *t);
-  if (!fused_shape && !tile_supported(ts.mj, ts.nw, ts.vec)) return STH_ERR_ARG;
+  if (!fused_shape && !shape_supported(c, ts.mj, ts.nw, ts.vec)) return STH_ERR_ARG;
   c->tune = *t;
   return STH_OK;
 }
@@ -1220,7 +1239,7 @@ bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning 
       continue;
     const bool ok = spp >= 2 ? (c->fmt == FMT_F32 && !c->no_fuse &&
                                 fused2_supported((int)npts, (int)mj, (int)nw, (int)spp))
-                             : tile_supported((int)mj, (int)nw, (int)vec);
+                             : shape_supported(c, (int)mj, (int)nw, (int)vec);
     if (!ok) continue;
     t.rows_per_wave = mj;
     t.waves = nw;

@@ -73,16 +73,6 @@ struct Fused2Shape {
   static constexpr int LDS_BYTES = (2 * LU + (2 * (S - 1) + SR) * LR) * 4;
 };
 
-// Calls f(integral_constant<int, 0>) .. f(integral_constant<int, N-1>).
-template <class Fn, int... K>
-__device__ __forceinline__ void static_for_seq(Fn &&f, std::integer_sequence<int, K...>) {
-  (f(std::integral_constant<int, K>{}), ...);
-}
-template <int N, class Fn>
-__device__ __forceinline__ void static_for(Fn &&f) {
-  static_for_seq(f, std::make_integer_sequence<int, N>{});
-}
-
 // Length of the u_{n+j} register queue of an S-step pass, and its entry
 // at the plane of step j+1.
 template <int S, int R>

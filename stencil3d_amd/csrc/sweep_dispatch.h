@@ -1,6 +1,7 @@
 // sweep_dispatch.h -- selects the sweep_kernel instance for a run.
 #pragma once
 
+#include "sweep16_kernel.h"
 #include "sweep_kernel.h"
 
 namespace sth {
@@ -116,6 +117,23 @@ hipError_t launch_sweep_25(int mode, int fmt, int mj, int nw, int vec, int ef, u
 const void *sweep_fn_7(int mode, int fmt, int mj, int nw, int vec, int ef);
 const void *sweep_fn_13(int mode, int fmt, int mj, int nw, int vec, int ef);
 const void *sweep_fn_25(int mode, int fmt, int mj, int nw, int vec, int ef);
+
+// The packed FP16 LEAPFROG sweep (sweep16_kernel.h, sweep16.hip): the tiles
+// of two k points per lane listed above.
+hipError_t launch_sweep16(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+                          const SweepArgs &a);
+const void *sweep16_fn(int npts, int mj, int nw);
+bool tile16_supported(int mj, int nw);
+// Steps per loop turn of the packed sweep: a tile's i-chunk of a multiple of
+// it runs no idle step.
+inline int sweep16_turn(int R) {
+  const int d = Tile16<7, 1, 1>::D;
+  return (2 * R + d + 2) / 2 * 2;
+}
+// An FP16 LEAPFROG sweep of two k points per lane runs the packed kernel.
+inline bool packed16(int fmt, int mode, int vec) {
+  return fmt == FMT_F16 && mode == MODE_LEAP && vec == 2;
+}
 
 inline const void *sweep_fn(int npts, int mode, int fmt, int mj, int nw, int vec, int ef) {
   if (npts == 7) return sweep_fn_7(mode, fmt, mj, nw, vec, ef);

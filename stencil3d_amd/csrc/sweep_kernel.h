@@ -24,6 +24,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "sth_common.h"
 
 namespace sth {
@@ -67,6 +70,16 @@ struct SweepArgs {
   unsigned tick_base[kXcds];
   int items, band;
 };
+
+// Calls f(integral_constant<int, 0>) .. f(integral_constant<int, N-1>).
+template <class Fn, int... K>
+__device__ __forceinline__ void static_for_seq(Fn &&f, std::integer_sequence<int, K...>) {
+  (f(std::integral_constant<int, K>{}), ...);
+}
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 template <int FMT>
 struct Codec;
@@ -613,6 +626,37 @@ __device__ inline void q16_publish(const SweepArgs &a) {
   }
 }
 
+// Runs tile(item) for the work items of one launch: one item per workgroup
+// (through the XCD-contiguous remap when a.remap), or, when a.tickets is
+// set, persistent workgroups drawing items from the ticket counters; within
+// an XCD's bands the tiles go in item order (k fastest, then j, then the
+// i-chunk).  One call site of the tile body for both launch forms keeps the
+// kernel's register budget that of one tile.  sh_item: a __shared__ int.
+template <class TileFn>
+__device__ __forceinline__ void for_items(const SweepArgs &a, int &sh_item, TileFn &&tile) {
+  const bool persistent = a.tickets != nullptr;
+  const bool one = a.band >= a.items;  // a single queue (counter 0)
+  const int x = one ? 0 : (int)(blockIdx.x % kXcds);
+  const int n = persistent ? (one ? a.items : xcd_band_items(a.items, a.band, x)) : 0;
+  unsigned *cnt = persistent ? a.tickets + 16 * x : nullptr;
+  for (;;) {
+    int item;
+    if (persistent) {
+      if (threadIdx.x == 0) sh_item = (int)(atomicAdd(cnt, 1u) - a.tick_base[x]);
+      __syncthreads();
+      const int t = sh_item;
+      __syncthreads();
+      if (t < 0 || t >= n) break;
+      item = one ? t : (x + kXcds * (t / a.band)) * a.band + t % a.band;
+    } else {
+      item = a.remap ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    }
+    tile(item);
+    if (!persistent) break;
+    __syncthreads();  // LDS is reused by the next tile
+  }
+}
+
 // EF > 0: the last k-tile of every row is the narrow remainder of the row
 // (at most 64/EF points) and runs folded (EF rows per wave, one k point per
 // lane) instead of as a full, mostly masked tile.
@@ -645,30 +689,7 @@ __global__ __launch_bounds__(64 * NW) void sweep_kernel(const SweepArgs a) {
     }
     sweep_tile<NPTS, MODE, FMT, MJ, NW, V>(a, k0, j0, ib, ie, lds, red);
   };
-  // Persistent launch: within an XCD's bands the tiles go in item order (k
-  // fastest, then j, then the i-chunk).  One call site of the tile body for
-  // both launch forms keeps the kernel's register budget that of one tile.
-  const bool persistent = a.tickets != nullptr;
-  const bool one = a.band >= a.items;  // a single queue (counter 0)
-  const int x = one ? 0 : (int)(blockIdx.x % kXcds);
-  const int n = persistent ? (one ? a.items : xcd_band_items(a.items, a.band, x)) : 0;
-  unsigned *cnt = persistent ? a.tickets + 16 * x : nullptr;
-  for (;;) {
-    int item;
-    if (persistent) {
-      if (threadIdx.x == 0) sh_item = (int)(atomicAdd(cnt, 1u) - a.tick_base[x]);
-      __syncthreads();
-      const int t = sh_item;
-      __syncthreads();
-      if (t < 0 || t >= n) break;
-      item = one ? t : (x + kXcds * (t / a.band)) * a.band + t % a.band;
-    } else {
-      item = a.remap ? xcd_remap((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
-    }
-    tile(item);
-    if (!persistent) break;
-    __syncthreads();  // LDS is reused by the next tile
-  }
+  for_items(a, sh_item, tile);
 }
 
 }  // namespace sth
