@@ -112,9 +112,10 @@ typedef struct {
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
   uint32_t steps_per_pass; /* LEAPFROG, FP32: 2 = two steps per pass over memory
                               (temporal blocking), 3 = three (7 and 13 points),
-                              4 or 5 = four or five (7 points); steps left over
-                              run as the longest shorter pass that fits, then
-                              one-step; 1 = one; 0: default */
+                              4 or 5 = four or five (7 points); FP16: 2 (the second
+                              step's scale from the bound of the first's output);
+                              steps left over run as the longest shorter pass that
+                              fits, then one-step; 1 = one; 0: default */
   uint32_t launch;        /* one-step sweep: 1 = one workgroup per tile, 2 = persistent
                              workgroups drawing tiles from a work queue; 0: default (2) */
   uint32_t ref_sweeps;    /* REF (FP32, FP64): sweeps per launch, 1..4096 (0: one).  u and s
@@ -248,7 +249,8 @@ int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
 int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
 /* Times the candidate schedules on this context's geometry (REF or LEAPFROG
  * sweeps, 2*reps steps rounded up to a multiple of 60 each, fields are
- * modified; FP32 LEAPFROG also tries two to five steps per pass), keeps
+ * modified; FP32 LEAPFROG also tries two to five steps per pass, FP16 LEAPFROG
+ * the two-step pass), keeps
  * the fastest and returns it with its mean time
  * per step in ms.  If the environment variable
  * STH_TUNE_CACHE names a file, a result stored there for the same (stencil,

@@ -90,6 +90,9 @@ def lib():
             "orc_dequantize": (None, [C.c_int, G, u16p, C.c_int, f32p]),
             "orc_leapfrog_q16": (C.c_int, [C.c_int, C.c_int, G, u16p, u16p, u16p, P(C.c_int),
                                            f32p, C.c_float, C.c_float, f32p, C.c_int, C.c_int]),
+            "orc_leapfrog_q16_pass2": (C.c_int, [C.c_int, C.c_int, G, u16p, u16p, u16p,
+                                                 P(C.c_int), f32p, C.c_float, C.c_float, f32p,
+                                                 C.c_int, C.c_int]),
             "orc_splitmix64_mix": (C.c_uint64, [C.c_uint64]),
             "orc_uniform01": (C.c_double, [C.c_uint64, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
             "orc_fill_uniform": (None, [G, f32p, C.c_uint64, C.c_int, C.c_double, C.c_double,
@@ -210,13 +213,16 @@ def dequantize(fmt, g: Grid, q, e):
     return x
 
 
-def leapfrog_q16(npts, fmt, g: Grid, a, b, s, exps, maxs, c0, c1, c2, steps, nthreads=0):
-    """exps/maxs: lists of 3 (a, b, s), updated in place.  Returns newest index."""
+def leapfrog_q16(npts, fmt, g: Grid, a, b, s, exps, maxs, c0, c1, c2, steps, nthreads=0,
+                 pass2=False):
+    """exps/maxs: lists of 3 (a, b, s), updated in place.  Returns newest index.
+    pass2: the steps in passes of two (orc_leapfrog_q16_pass2)."""
     ce = (C.c_int * 3)(*exps)
     cm = (C.c_float * 3)(*maxs)
     c2a = None if c2 is None else np.ascontiguousarray(c2, dtype=np.float32)
-    rc = lib().orc_leapfrog_q16(npts, fmt, C.byref(g.c()), u16p(a), u16p(b), u16p(s), ce, cm,
-                                c0, c1, None if c2a is None else f32p(c2a), steps, nthreads)
+    fn = lib().orc_leapfrog_q16_pass2 if pass2 else lib().orc_leapfrog_q16
+    rc = fn(npts, fmt, C.byref(g.c()), u16p(a), u16p(b), u16p(s), ce, cm,
+            c0, c1, None if c2a is None else f32p(c2a), steps, nthreads)
     assert rc >= 0
     exps[:] = list(ce)
     maxs[:] = list(cm)

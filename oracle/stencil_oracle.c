@@ -350,46 +350,97 @@ static int q16_out_exp(const int exps[3], const float maxs[3], int cu, int cv, f
                        fabs((double)c2n) * Ms);
 }
 
+/* One 16-bit LEAPFROG step with output exponent eo: v <- update(u, v, s) on
+ * the stored numbers; returns the largest stored magnitude of the interior. */
+static float q16_step(const term_tab *tb, int fmt, const orc_geom *g, const uint16_t *u,
+                      uint16_t *v, const uint16_t *s, const int exps[3], int cur, int eo,
+                      float c0, float c1, float c2n) {
+  const int64_t im = orc_imult(g), jm = orc_jmult(g), o = orc_ofs(g);
+  const int T = tb->T;
+  /* the update on stored numbers: powers of two folded into the coefficients */
+  const float k0 = ldexpf(c0, exps[cur] - eo), k1 = ldexpf(c1, exps[cur] - eo);
+  const float k2 = ldexpf(c2n, exps[2] - eo), kv = ldexpf(1.0f, exps[1 - cur] - eo);
+  float mx = 0.0f;
+#pragma omp parallel for schedule(static) reduction(max : mx)
+  for (int64_t i = 0; i < g->xs; i++)
+    for (int64_t j = 0; j < g->ys; j++)
+      for (int64_t k = 0; k < g->zs; k++) {
+        const int64_t c = o + i * im + j * jm + k;
+        float L = tb->w[0] * (orc_16_to_f32(fmt, u[c - tb->off[0]]) +
+                              orc_16_to_f32(fmt, u[c + tb->off[0]]));
+        for (int t = 1; t < T; t++)
+          L = fmaf(tb->w[t],
+                   orc_16_to_f32(fmt, u[c - tb->off[t]]) + orc_16_to_f32(fmt, u[c + tb->off[t]]),
+                   L);
+        const float u0 = orc_16_to_f32(fmt, u[c]);
+        const float v0 = orc_16_to_f32(fmt, v[c]);
+        const float s0 = orc_16_to_f32(fmt, s[c]);
+        const float out = fmaf(k2, s0, fmaf(k1, L, fmaf(k0, u0, -(v0 * kv))));
+        const uint16_t q = orc_f32_to_16(fmt, out);
+        v[c] = q;
+        const float d = fabsf(orc_16_to_f32(fmt, q));
+        if (d > mx) mx = d;
+      }
+  return mx;
+}
+
 int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
                      const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
                      const float *c2, int steps, int nthreads) {
   term_tab tb;
   if (make_tab(npts, g, &tb, 1, 0) < 0 || steps < 0 || (fmt != 1 && fmt != 2)) return -1;
   set_threads(nthreads);
-  const int64_t im = orc_imult(g), jm = orc_jmult(g), o = orc_ofs(g);
-  const int T = tb.T;
   const double gain = orc_lap_gain(npts);
   uint16_t *bufs[2] = {a, b};
   int cur = 0;
   for (int n = 0; n < steps; n++) {
-    const uint16_t *u = bufs[cur];
-    uint16_t *v = bufs[1 - cur];
     const float c2n = c2 ? c2[n] : 0.0f;
     const int eo = q16_out_exp(exps, maxs, cur, 1 - cur, c0, c1, c2n, gain);
-    /* the update on stored numbers: powers of two folded into the coefficients */
-    const float k0 = ldexpf(c0, exps[cur] - eo), k1 = ldexpf(c1, exps[cur] - eo);
-    const float k2 = ldexpf(c2n, exps[2] - eo), kv = ldexpf(1.0f, exps[1 - cur] - eo);
-    float mx = 0.0f;
-#pragma omp parallel for schedule(static) reduction(max : mx)
-    for (int64_t i = 0; i < g->xs; i++)
-      for (int64_t j = 0; j < g->ys; j++)
-        for (int64_t k = 0; k < g->zs; k++) {
-          const int64_t c = o + i * im + j * jm + k;
-          float L = tb.w[0] * (orc_16_to_f32(fmt, u[c - tb.off[0]]) +
-                               orc_16_to_f32(fmt, u[c + tb.off[0]]));
-          for (int t = 1; t < T; t++)
-            L = fmaf(tb.w[t],
-                     orc_16_to_f32(fmt, u[c - tb.off[t]]) + orc_16_to_f32(fmt, u[c + tb.off[t]]),
-                     L);
-          const float u0 = orc_16_to_f32(fmt, u[c]);
-          const float v0 = orc_16_to_f32(fmt, v[c]);
-          const float s0 = orc_16_to_f32(fmt, s[c]);
-          const float out = fmaf(k2, s0, fmaf(k1, L, fmaf(k0, u0, -(v0 * kv))));
-          const uint16_t q = orc_f32_to_16(fmt, out);
-          v[c] = q;
-          const float d = fabsf(orc_16_to_f32(fmt, q));
-          if (d > mx) mx = d;
-        }
+    const float mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, eo, c0, c1,
+                              c2n);
+    exps[1 - cur] = eo;
+    maxs[1 - cur] = mx;
+    cur = 1 - cur;
+  }
+  return cur;
+}
+
+/* The same steps in passes of two (the GPU's FP16 two-step pass): within a
+ * pass the second step's output exponent comes from the bound of the first
+ * step's output in place of its maximum, B2 = (|c0| + |c1| gain) B1 + M_u +
+ * |c2[n+1]| M_s with B1 the first step's bound; a step left over runs alone. */
+int orc_leapfrog_q16_pass2(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
+                           const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
+                           const float *c2, int steps, int nthreads) {
+  term_tab tb;
+  if (make_tab(npts, g, &tb, 1, 0) < 0 || steps < 0 || (fmt != 1 && fmt != 2)) return -1;
+  set_threads(nthreads);
+  const double gain = orc_lap_gain(npts);
+  const double gmul = fabs((double)c0) + fabs((double)c1) * gain;
+  uint16_t *bufs[2] = {a, b};
+  int cur = 0, n = 0;
+  for (; n + 2 <= steps; n += 2) {
+    const float c2a = c2 ? c2[n] : 0.0f, c2b = c2 ? c2[n + 1] : 0.0f;
+    const double Mu = ldexp((double)maxs[cur], exps[cur]);
+    const double Mv = ldexp((double)maxs[1 - cur], exps[1 - cur]);
+    const double Ms = ldexp((double)maxs[2], exps[2]);
+    const double B1 = gmul * Mu + Mv + fabs((double)c2a) * Ms;
+    const double B2 = gmul * B1 + Mu + fabs((double)c2b) * Ms;
+    const int e1 = orc_scale_exp(B1), e2 = orc_scale_exp(B2);
+    float mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, e1, c0, c1, c2a);
+    exps[1 - cur] = e1;
+    maxs[1 - cur] = mx;
+    cur = 1 - cur;
+    mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, e2, c0, c1, c2b);
+    exps[1 - cur] = e2;
+    maxs[1 - cur] = mx;
+    cur = 1 - cur;
+  }
+  if (n < steps) {
+    const float c2n = c2 ? c2[n] : 0.0f;
+    const int eo = q16_out_exp(exps, maxs, cur, 1 - cur, c0, c1, c2n, gain);
+    const float mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, eo, c0, c1,
+                              c2n);
     exps[1 - cur] = eo;
     maxs[1 - cur] = mx;
     cur = 1 - cur;

@@ -26,6 +26,10 @@
  *   orc_leapfrog_q16    fp16/bf16 storage emulation: conversions pinned by
  *                       IEEE special cases, exhaustive round trips and numpy's
  *                       float16 rounding; scaling rule pinned by invariants.
+ *   orc_leapfrog_q16_pass2  the same steps in passes of two: pinned by
+ *                       equality with orc_leapfrog_q16 wherever the two
+ *                       exponent rules agree, and by its own invariants (no
+ *                       overflow, exponent never below the one-step rule's).
  *   orc_fill_uniform    SplitMix64 published output vectors.
  */
 #ifndef STENCIL_ORACLE_H
@@ -115,6 +119,15 @@ void orc_dequantize(int fmt, const orc_geom *g, const uint16_t *q, int e, float 
 int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
                      const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
                      const float *c2, int steps, int nthreads);
+/* The same update in passes of two steps (the GPU's FP16 two-step pass):
+ * in a pass the first output's exponent is the one-step rule's (bound B1 of
+ * the stored inputs' maxima), the second's comes from
+ *   B2 = (|c0| + |c1| gain) B1 + M_u + |c2[n+1]| M_s
+ * (B1 in place of the unknown maximum of u_{n+1}); a step left over runs
+ * with the one-step rule. */
+int orc_leapfrog_q16_pass2(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
+                           const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
+                           const float *c2, int steps, int nthreads);
 
 /* ---- initialisers, checks, norms ----------------------------------------- */
 uint64_t orc_splitmix64_mix(uint64_t z);

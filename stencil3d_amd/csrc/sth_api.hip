@@ -411,7 +411,23 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
 // for 7 points, three for 13, two for 25; otherwise, or when the tuning asks
 // for one step per pass, the one-step sweep.
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
-  if (c->fmt != FMT_F32 || c->no_fuse) return false;
+  if (c->no_fuse) return false;
+  if (c->fmt == FMT_F16) {
+    if (t.steps_per_pass) return t.steps_per_pass == 2;
+    // FP16 two-step pass against the packed one-step sweep (cubic grids,
+    // one-step time / pass time per step): 7 points 0.76 at 128^3 (6
+    // output tiles), 1.40 at 256^3 (15), 1.32 at 512^3, 1.62 at 1040^3;
+    // 13 points 0.94 at 512^3 (54 output tiles of the 4 x 16 shape), 1.09 at
+    // 640^3 (77), 1.18 at 768^3, 1.35 at 1040^3; 25 points 0.63-0.98 from 256^3 to 1040^3 (the
+    // pass does 1.47 lane-updates per point-update and is instruction-bound
+    // there, the sweep is held by the HBM read/write mix), so it stays
+    // one-step.
+    const int npts = (int)c->g.npts;
+    if (npts == 25) return false;
+    const Plan2 p = make_plan2(c, t);
+    return p.ntk * p.ntj >= (npts == 7 ? 12 : 70) && (int)c->g.xs >= 16 * c->R;
+  }
+  if (c->fmt != FMT_F32) return false;
   if (t.steps_per_pass) return t.steps_per_pass >= 2;
   // Measured on cubic grids (FP32 LEAPFROG, one-step time / pass time per
   // step): 7 points, four-step pass 1.14-1.22 from 24^3 (1 tile) to 48^3,
@@ -435,25 +451,41 @@ bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
 Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   const int npts = (int)c->g.npts;
   Plan2 p;
-  if (npts == 25 || t.steps_per_pass == 2)
+  Fused2Geom g;
+  if (c->fmt == FMT_F16) {
+    // FP16 storage: two steps per pass (sweep2h_kernel.h)
     p.s = 2;
-  else if (npts == 7 && t.steps_per_pass == 4)
-    p.s = 4;
-  else if (npts == 7 && (t.steps_per_pass == 5 || t.steps_per_pass == 0))
-    p.s = 5;
-  else
-    p.s = 3;
-  if (fused2_supported(npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
-    p.mj = (int)t.rows_per_wave;
-    p.nw = (int)t.waves;
-  } else if (p.s >= 4) {
-    p.mj = 2;
-    p.nw = 16;
+    if (fused2h_supported(npts, (int)t.rows_per_wave, (int)t.waves)) {
+      p.mj = (int)t.rows_per_wave;
+      p.nw = (int)t.waves;
+    } else {
+      // 7 and 13 points 4 x 16 (13 points at 1040^3: 1.42 ms/step against
+      // 1.48 for 3 x 16), 25 points 3 x 16 (4 x 16 exceeds 128 VGPRs)
+      p.mj = npts == 25 ? 3 : 4;
+      p.nw = 16;
+    }
+    g = fused2h_geom(npts, p.mj, p.nw);
   } else {
-    p.mj = npts == 25 || p.s == 3 ? 3 : 4;
-    p.nw = 16;
+    if (npts == 25 || t.steps_per_pass == 2)
+      p.s = 2;
+    else if (npts == 7 && t.steps_per_pass == 4)
+      p.s = 4;
+    else if (npts == 7 && (t.steps_per_pass == 5 || t.steps_per_pass == 0))
+      p.s = 5;
+    else
+      p.s = 3;
+    if (fused2_supported(npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
+      p.mj = (int)t.rows_per_wave;
+      p.nw = (int)t.waves;
+    } else if (p.s >= 4) {
+      p.mj = 2;
+      p.nw = 16;
+    } else {
+      p.mj = npts == 25 || p.s == 3 ? 3 : 4;
+      p.nw = 16;
+    }
+    g = fused2_geom(npts, p.mj, p.nw, p.s);
   }
-  const Fused2Geom g = fused2_geom(npts, p.mj, p.nw, p.s);
   p.oj = g.oj;
   p.ok = g.ok;
   p.ntk = ((int)c->g.zs + p.ok - 1) / p.ok;
@@ -555,13 +587,25 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
   a.c2d = c2n && p.s >= 4 ? c2n[3] : 0.0f;
   a.c2e = c2n && p.s >= 5 ? c2n[4] : 0.0f;
   memcpy(a.w, wts.w, sizeof(a.w));
-  const hipError_t e = launch_sweep2((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a);
+  const bool h16 = c->fmt == FMT_F16;
+  if (h16) {
+    a.qs = c->qs_dev;
+    a.slot = c->slot;
+    a.ubuf = ub;
+    a.vbuf = vb;
+    a.gain = wts.gain;
+  }
+  const hipError_t e =
+      h16 ? launch_sweep2h((int)c->g.npts, p.mj, p.nw, p.grid, c->stream, a)
+          : launch_sweep2((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (c->timing) c->launches++;
   std::swap(c->base[0], c->twin[0]);
   std::swap(c->base[1], c->twin[1]);
   if (p.s % 2) c->ubuf = vb;
-  c->slot = (c->slot + p.s) % 3;
+  // FP16: a pass takes one slot of the scale ring (it publishes the state
+  // after its last step)
+  c->slot = (c->slot + (h16 ? 1 : p.s)) % 3;
   return STH_OK;
 }
 
@@ -1181,12 +1225,16 @@ int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
   if (!c || !t || t->steps_per_pass > 5 || t->launch > 2 || t->chunks > 64 ||
       t->ref_sweeps > 4096)
     return STH_ERR_ARG;
-  if (t->steps_per_pass >= 2 && c->fmt != FMT_F32) return STH_ERR_UNSUPPORTED;
+  if (t->steps_per_pass >= 2 && c->fmt != FMT_F32 && !(c->fmt == FMT_F16 && t->steps_per_pass == 2))
+    return STH_ERR_UNSUPPORTED;
   if ((t->steps_per_pass == 3 && c->g.npts == 25) || (t->steps_per_pass >= 4 && c->g.npts != 7))
     return STH_ERR_UNSUPPORTED;
   const bool fused_shape =
-      t->steps_per_pass >= 2 && fused2_supported((int)c->g.npts, (int)t->rows_per_wave,
-                                                 (int)t->waves, (int)t->steps_per_pass);
+      t->steps_per_pass >= 2 &&
+      (c->fmt == FMT_F16
+           ? fused2h_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves)
+           : fused2_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves,
+                              (int)t->steps_per_pass));
   const TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, *t);
   if (!fused_shape && !shape_supported(c, ts.mj, ts.nw, ts.vec)) return STH_ERR_ARG;
   c->tune = *t;
@@ -1237,9 +1285,13 @@ bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning 
     if (ver != 2 || npts != c->g.npts || dt != c->g.dtype || lf != (unsigned)leap ||
         xs != c->g.xs || ys != c->g.ys || zs != c->g.zs || spp < 1 || spp > 5)
       continue;
-    const bool ok = spp >= 2 ? (c->fmt == FMT_F32 && !c->no_fuse &&
-                                fused2_supported((int)npts, (int)mj, (int)nw, (int)spp))
-                             : shape_supported(c, (int)mj, (int)nw, (int)vec);
+    const bool ok =
+        spp >= 2 ? (!c->no_fuse &&
+                    (c->fmt == FMT_F16
+                         ? spp == 2 && fused2h_supported((int)npts, (int)mj, (int)nw)
+                         : c->fmt == FMT_F32 &&
+                               fused2_supported((int)npts, (int)mj, (int)nw, (int)spp)))
+                 : shape_supported(c, (int)mj, (int)nw, (int)vec);
     if (!ok) continue;
     t.rows_per_wave = mj;
     t.waves = nw;
@@ -1283,7 +1335,7 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
   }
   static const int tiles[][3] = {{1, 4, 1}, {2, 4, 1}, {4, 4, 1}, {8, 4, 1}, {2, 8, 1},
                                  {4, 8, 1}, {4, 2, 1}, {1, 4, 2}, {2, 4, 2}, {4, 4, 2},
-                                 {2, 8, 2}, {4, 8, 2}};
+                                 {2, 8, 2}, {4, 8, 2}, {2, 16, 2}, {3, 8, 2}, {3, 16, 2}};
   static const int chunk_opts[] = {0, 8, 16, 32};
   static const int fused_shapes[][3] = {{3, 16, 2}, {4, 16, 2}, {4, 12, 2}, {4, 8, 2},
                                         {3, 8, 2},  {3, 16, 3}, {4, 12, 3}, {4, 10, 4},
@@ -1299,10 +1351,26 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
       t.vec = (uint32_t)tl[2];
       t.chunks = (uint32_t)ch;
       t.steps_per_pass = 1;
+      if (!shape_supported(c, tl[0], tl[1], tl[2])) continue;
       const Plan p = make_plan(c, t);
       if (ch != 0 && p.chunks != ch) continue;
       cand.push_back(t);
     }
+  // FP16: the two-step pass's shapes
+  if (leapfrog && c->fmt == FMT_F16 && !c->no_fuse)
+    for (int mj : {3, 4})
+      for (int ch : {0, 4, 8, 16}) {
+        if (!fused2h_supported((int)c->g.npts, mj, 16)) continue;
+        sth_tuning t = saved;
+        t.rows_per_wave = (uint32_t)mj;
+        t.waves = 16;
+        t.vec = 2;
+        t.chunks = (uint32_t)ch;
+        t.steps_per_pass = 2;
+        const Plan2 p = make_plan2(c, t);
+        if (ch != 0 && p.chunks != ch) continue;
+        cand.push_back(t);
+      }
   if (leapfrog && c->fmt == FMT_F32 && !c->no_fuse)
     for (const auto &fs : fused_shapes) {
       if (!fused2_supported((int)c->g.npts, fs[0], fs[1], fs[2])) continue;

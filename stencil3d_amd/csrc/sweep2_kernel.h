@@ -44,6 +44,11 @@ struct Sweep2Args {
   float c0, c1;
   float c2a, c2b, c2c, c2d, c2e;  // source coefficients of steps n .. n+4
   float w[12];     // LEAPFROG weights (2 * stencil weight), evaluation order
+  // FP16 storage (sweep2h_kernel.h): the scale ring, the slot this pass
+  // reads, the ring indices of u and v, and the sum of 2|w| over the terms
+  Q16Slot *qs;
+  int slot, ubuf, vbuf;
+  double gain;
 };
 
 // S steps per pass (2, 3 or 4): step one runs on the output tile widened by

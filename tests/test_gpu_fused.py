@@ -370,10 +370,51 @@ def test_three_steps_rejected_for_25_points_and_16bit(gpu):
             ctx.set_tuning(steps_per_pass=3)
 
 
-def test_two_steps_rejected_for_16bit(gpu):
-    with gpu.Stencil(16, 16, 64, npts=25, dtype="fp16") as ctx:
+def test_16bit_pass_lengths(gpu):
+    """FP16 storage has the two-step pass and no longer one; BF16 has none."""
+    with gpu.Stencil(16, 16, 64, npts=25, dtype="bf16") as ctx:
         with pytest.raises(gpu.SthError):
             ctx.set_tuning(steps_per_pass=2)
+    with gpu.Stencil(16, 16, 64, npts=13, dtype="fp16") as ctx:
+        with pytest.raises(gpu.SthError):
+            ctx.set_tuning(steps_per_pass=3)
+        ctx.set_tuning(steps_per_pass=2)
+        assert ctx.get_tuning()["steps_per_pass"] == 2
+
+
+@pytest.mark.parametrize("npts,shape,spp", [(7, (40, 130, 300), 2), (7, (40, 70, 300), 1),
+                                             (25, (40, 130, 300), 1)])
+def test_fp16_default_schedule(gpu, orc, npts, shape, spp):
+    """FP16 LEAPFROG takes the two-step pass by default for 7 points from 12
+    output tiles (and 13 points from 70), and stays one-step for 25 points;
+    five steps are then two passes and a step, bit-identical to the oracle's
+    emulation of the same schedule."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 31, 0.0)  # the 16-bit form needs zero halos
+    s *= np.float32(1e-3)
+    qs, exps, maxs = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(1, g, x, e)
+        qs.append(q)
+        exps.append(e)
+        maxs.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.linspace(0.25, -0.25, 5).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+        assert ctx.get_tuning()["steps_per_pass"] == spp
+        ctx.upload_raw16("u", qs[0], exps[0], R)
+        ctx.upload_raw16("v", qs[1], exps[1], R)
+        ctx.upload_raw16("s", qs[2], exps[2], R)
+        ctx.leapfrog(5, c0, c1, c2)
+        gu, eu, mu = ctx.download_raw16("u", R)
+    a, b = qs[0].copy(), qs[1].copy()
+    newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
+                              np.float32(c1), c2, 5, pass2=spp == 2)
+    want = a if newest == 0 else b
+    assert eu == exps[newest]
+    assert np.array_equal(gu[g.interior], want[g.interior])
 
 
 @pytest.mark.parametrize("npts,n,spp", [(7, 12, 1), (7, 48, 5), (13, 112, 1), (13, 128, 3),

@@ -235,6 +235,51 @@ def test_fp16_packed_tiles_bit_exact(gpu, orc, npts, tile, shape, chunks):
     assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
 
 
+FP16_PASS_SHAPES = {7: [(4, 16)], 13: [(3, 16), (4, 16)], 25: [(3, 16)]}
+
+
+@pytest.mark.parametrize("npts,tile", [(n, t) for n, ts in FP16_PASS_SHAPES.items() for t in ts])
+@pytest.mark.parametrize("shape,chunks", [((30, 61, 201), 2), ((44, 130, 97), 3),
+                                          ((17, 12, 40), 1)])
+def test_fp16_two_step_pass_bit_exact(gpu, orc, npts, tile, shape, chunks):
+    """The FP16 two-step pass: three passes and one step left over, several
+    i-chunks, output tiles cut by the grid's edges in j and k (an odd zs
+    leaves a lane with one point inside), a source term; bits, exponents and
+    maxima equal to the oracle's binary16 emulation in passes of two steps."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    v, s, u = seeded(orc, g, seed=23 + npts)
+    s *= np.float32(1e-3)
+    qs, exps, maxs = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(1, g, x, e)
+        qs.append(q)
+        exps.append(e)
+        maxs.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.linspace(0.5, -0.5, 7).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+        ctx.set_tuning(rows_per_wave=tile[0], waves=tile[1], chunks=chunks, steps_per_pass=2)
+        assert ctx.get_tuning()["steps_per_pass"] == 2
+        ctx.upload_raw16("u", qs[0], exps[0], R)
+        ctx.upload_raw16("v", qs[1], exps[1], R)
+        ctx.upload_raw16("s", qs[2], exps[2], R)
+        ctx.leapfrog(7, c0, c1, c2)
+        gu, eu, mu = ctx.download_raw16("u", R)
+        gv, ev, mv = ctx.download_raw16("v", R)
+    a, b = qs[0].copy(), qs[1].copy()
+    newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
+                              np.float32(c1), c2, 7, pass2=True)
+    (wu, weu, wmu), (wv, wev, wmv) = (
+        ((a, exps[0], maxs[0]), (b, exps[1], maxs[1])) if newest == 0 else
+        ((b, exps[1], maxs[1]), (a, exps[0], maxs[0])))
+    I = g.interior
+    assert (eu, ev) == (weu, wev)
+    assert np.array_equal(gu[I], wu[I]) and np.array_equal(gv[I], wv[I])
+    assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
+
+
 def test_norms_match_oracle(gpu, orc):
     shape = (20, 21, 22)
     g = orc.Grid(*shape, 1, 1, 1)

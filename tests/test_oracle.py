@@ -379,6 +379,58 @@ def test_q16_leapfrog_emulation_invariants(orc, fmt):
     assert mr < (4e-3 if fmt == 1 else 4e-2)
 
 
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_q16_pass2_against_one_step_rule(orc, npts):
+    """Passes of two steps (the second step's exponent from the bound of the
+    first's output) against the one-step rule from the same start: the first
+    step of a pass is the same step, bit for bit; the second uses an exponent
+    at least the one-step rule's, and since scaling by a power of two
+    commutes with rounding its decoded values equal the one-step rule's
+    wherever both stored numbers are normal binary16, within half a
+    subnormal step elsewhere.  A step left over runs with the one-step rule."""
+    R = orc.radius(npts)
+    g = orc.Grid(9, 10, 11, R, R, R)
+    fmt = 1
+    u = orc.fill_gaussian(g, 1.0, 2.0, 1e-3, 1, 0)
+    v = orc.fill_gaussian(g, 1.0, 2.0, 1e-3, 1, 1)
+    s = orc.fill_uniform(g, 3, 2, -1e-3, 1e-3)  # halos 0 (the 16-bit form's boundary)
+    qs, ex, mx = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, m = orc.quantize(fmt, g, x, e)
+        qs.append(q)
+        ex.append(e)
+        mx.append(m)
+    c1 = np.float32(0.16)
+    c0 = np.float32(2 - orc.lap_gain(npts) * 0.16)
+    c2 = np.array([0.3, -0.2, 0.5], np.float32)
+    # one pass (two steps)
+    a1, b1 = qs[0].copy(), qs[1].copy()
+    e1, m1 = list(ex), list(mx)
+    n1 = orc.leapfrog_q16(npts, fmt, g, a1, b1, qs[2], e1, m1, c0, c1, c2, 2)
+    a2, b2 = qs[0].copy(), qs[1].copy()
+    e2, m2 = list(ex), list(mx)
+    n2 = orc.leapfrog_q16(npts, fmt, g, a2, b2, qs[2], e2, m2, c0, c1, c2, 2, pass2=True)
+    assert n1 == n2 == 0
+    I = g.interior
+    assert np.array_equal(b1[I], b2[I]) and e1[1] == e2[1] and m1[1] == m2[1]  # u_{n+1}
+    assert e2[0] >= e1[0]
+    x1 = orc.dequantize(fmt, g, a1, e1[0])[I]
+    x2 = orc.dequantize(fmt, g, a2, e2[0])[I]
+    normal = np.abs(orc.dequantize(fmt, g, a2, 0)[I]) >= 2.0 ** -14
+    assert np.array_equal(x1[normal], x2[normal])
+    assert np.all(np.abs(x1 - x2) <= 2.0 ** (-25 + e2[0]))
+    # a pass and a step left over: the last step is the one-step rule's
+    a3, b3 = qs[0].copy(), qs[1].copy()
+    e3, m3 = list(ex), list(mx)
+    n3 = orc.leapfrog_q16(npts, fmt, g, a3, b3, qs[2], e3, m3, c0, c1, c2, 3, pass2=True)
+    a4, b4 = a2.copy(), b2.copy()
+    e4, m4 = list(e2), list(m2)
+    n4 = orc.leapfrog_q16(npts, fmt, g, a4, b4, qs[2], e4, m4, c0, c1, c2[2:], 1)
+    assert n3 == 1 and n4 == 1
+    assert np.array_equal(b3[I], b4[I]) and e3 == e4 and m3 == m4
+
+
 # ------------------------------------------------------------------ misc
 def test_gaussian_init(orc):
     g = orc.Grid(7, 8, 9, 2, 2, 2)
