@@ -248,19 +248,27 @@ def run_batch(S, steps=100, nshots=64, seed=2, sample=4):
 def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
     """Every stencil size at n^3, FP32: REF (the reference's update, what
     hstencil3d runs), LEAPFROG (the default schedule) and REF at ten sweeps
-    per launch; Gpts/s from HIP events, and the error against the FP64 twin.
-    `alg_equiv_gbs` is 16 B per point-update over the time: the bandwidth a
-    one-step sweep would need to match the row (above 8 TB/s for the
-    multi-step schedules); `traffic_gbs` is the counted traffic of the
+    per launch; and LEAPFROG on FP16 storage (its default schedule: the
+    two-step pass for 7 and 13 points, the packed one-step sweep for 25);
+    Gpts/s from HIP events, and the error against the FP64 twin.
+    `alg_equiv_gbs` is 16 B (FP16: 8 B) per point-update over the time: the
+    bandwidth a one-step sweep would need to match the row (above 8 TB/s for
+    the multi-step schedules); `traffic_gbs` is the counted traffic of the
     kernel (profiles/traffic.json) over the same time, where profiled.
     LEAPFROG rows keep the fastest of `place` + 1 field allocations."""
     out = {}
     for npts in (7, 13, 25):
         c0, c1 = leap_coeffs(S, npts)
         row = {}
-        for mode in ("ref", "leapfrog", "ref_10_per_launch"):
-            with S.Stencil(n, n, n, npts=npts, dtype="fp32") as ctx:
-                twin = Twin(S, n, npts, "leapfrog" if mode == "leapfrog" else "ref")
+        modes = ("ref", "leapfrog", "ref_10_per_launch", "leapfrog_fp16")
+        if npts == 25:
+            # the FP16 two-step pass, which is not 25 points' default
+            modes += ("leapfrog_fp16_two_step",)
+        for mode in modes:
+            dt = "fp16" if mode.startswith("leapfrog_fp16") else "fp32"
+            with S.Stencil(n, n, n, npts=npts, dtype=dt) as ctx:
+                leap = mode.startswith("leapfrog")
+                twin = Twin(S, n, npts, "leapfrog" if leap else "ref")
                 try:
                     placed = None
                     if mode == "leapfrog" and place:
@@ -271,7 +279,9 @@ def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
                         # opt-in: ten REF sweeps per launch (bit-identical to ten
                         # launches; u and s are read once per ten sweeps)
                         ctx.set_tuning(ref_sweeps=10)
-                    run = (lambda k: ctx.leapfrog(k, c0, c1)) if mode == "leapfrog" else \
+                    if mode == "leapfrog_fp16_two_step":
+                        ctx.set_tuning(steps_per_pass=2)
+                    run = (lambda k: ctx.leapfrog(k, c0, c1)) if leap else \
                         (lambda k: ctx.iterate(k))
                     run(warmup)
                     ctx.sync()
@@ -279,7 +289,7 @@ def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
                     run(steps)
                     ms, nl = ctx.timer_read()
                     twin.advance_to(warmup + steps)
-                    field = "u" if mode == "leapfrog" else "v"
+                    field = "u" if leap else "v"
                     mr, lr = ctx.compare(field, twin.ctx)
                     tune = ctx.get_tuning()
                 finally:
@@ -287,21 +297,24 @@ def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
             per = ms / steps  # per step (a pass counts as its steps)
             g = float(n) ** 3 / (per * 1e-3) / 1e9
             r = {"kernel_ms_per_step": round(per, 4), "gpts": round(g, 2),
-                 "alg_equiv_gbs": round(16.0 * g, 1),
+                 "alg_equiv_gbs": round((8.0 if dt == "fp16" else 16.0) * g, 1),
                  "gflops_nominal": round(FLOPS_PT[npts] * g, 1),
                  "steps_per_launch": round(steps / nl, 3),
                  "err_vs_fp64": {"maxrel": mr, "l2rel": lr}}
-            tb = traffic_of(n, npts, 0, 1 if mode == "leapfrog" else 0, tune)
+            tb = traffic_of(n, npts, 1 if dt == "fp16" else 0, 1 if leap else 0, tune)
             if tb is not None and mode != "ref_10_per_launch":
                 r["traffic_gbs"] = round(tb / (ms / nl * 1e-3) / 1e9, 1)
+            if leap:
+                r["tuning"] = tune
             if mode == "leapfrog":
                 r["tuning"] = tune
                 r["placement"] = placed
             row[mode] = r
         out[str(npts)] = row
-        log(f"[bench] fp32 {npts}-pt: REF {row['ref']['gpts']} Gpts/s, "
+        log(f"[bench] {npts}-pt: REF {row['ref']['gpts']} Gpts/s, "
             f"LEAPFROG {row['leapfrog']['gpts']} Gpts/s, "
-            f"REF 10 sweeps per launch {row['ref_10_per_launch']['gpts']} Gpts/s")
+            f"REF 10 sweeps per launch {row['ref_10_per_launch']['gpts']} Gpts/s, "
+            f"LEAPFROG FP16 {row['leapfrog_fp16']['gpts']} Gpts/s")
     return out
 
 
@@ -418,9 +431,15 @@ def measured_traffic(n, dtype_code, tune, npts=NPTS, mode=1):
     t = load_traffic()
     if t is None or t.get("grid") != [n, n, n]:
         return None, None
-    if tune.get("steps_per_pass", 0) >= 2 and mode == 1:
+    if tune.get("steps_per_pass", 0) >= 2 and mode == 1 and dtype_code == 1:
+        # the FP16 two-step pass
+        prefix = f"void sth::sweep2h_kernel<{npts}, {tune['rows_per_wave']}, {tune['waves']}"
+    elif tune.get("steps_per_pass", 0) >= 2 and mode == 1:
         prefix = (f"void sth::sweep2_kernel<{npts}, {tune['rows_per_wave']}, "
                   f"{tune['waves']}, {tune['steps_per_pass']}")
+    elif dtype_code == 1 and mode == 1 and tune.get("vec") == 2:
+        # the packed FP16 one-step sweep
+        prefix = f"void sth::sweep16_kernel<{npts}, {tune['rows_per_wave']}, {tune['waves']}"
     else:
         # the last template argument (fold of the last k-tile) follows the grid
         prefix = (f"void sth::sweep_kernel<{npts}, {mode}, {dtype_code}, "
@@ -517,6 +536,9 @@ def main():
                          "frac_of_peak": round(a2 / PEAK_GBS, 4),
                          "speedup_vs_fp32": round(g2 / gpts, 3), "tuning": t2,
                          "err_vs_fp64_1040": e2}
+            tb2, _ = measured_traffic(n, 1 if dt == "fp16" else 2, t2)
+            if tb2 is not None:
+                extra[dt]["traffic_gbs"] = round(tb2 / (k2 / l2 * 1e-3) / 1e9, 1)
             if one_step:
                 extra[dt]["speedup_vs_fp32_one_step"] = round(g2 / one_step["gpts"], 3)
             log(f"[bench] {dt}: {g2:.2f} Gpts/s ({g2 / gpts:.2f}x fp32)")
