@@ -466,8 +466,8 @@ def main():
                     help="only the placement search of the headline context with N more "
                          "allocations (its launches: 3 rounds over the sets, 5 passes each)")
     ap.add_argument("--place", type=int, default=3,
-                    help="FP32 runs: keep the fastest of the first field allocation and this "
-                         "many more (sth_place; 0: the first allocation)")
+                    help="FP32, FP16 and BF16 runs: keep the fastest of the first field "
+                         "allocation and this many more (sth_place; 0: the first allocation)")
     ap.add_argument("--only-probes", action="store_true",
                     help="run the bandwidth probes alone (for counter runs)")
     args = ap.parse_args()
@@ -528,19 +528,23 @@ def main():
     if not args.no_16bit:
         for dt in ("fp16", "bf16"):
             log(f"[bench] {dt}: {W} warm-up + {K} timed steps")
-            w2, k2, l2, t2, _, _, e2, _ = run_throughput(S, n, dt, K, W, twin=twin)
+            w2, k2, l2, t2, _, _, e2, p2 = run_throughput(S, n, dt, K, W, twin=twin,
+                                                          place=args.place)
             g2 = pts * K / w2 / 1e9
             a2 = 8.0 * pts / (k2 / l2 * 1e-3) / 1e9
             extra[dt] = {"gpts": round(g2, 3), "ms_per_step": round(w2 / K * 1e3, 4),
                          "kernel_ms": round(k2 / l2, 4), "hbm_gbs_alg": round(a2, 1),
                          "frac_of_peak": round(a2 / PEAK_GBS, 4),
                          "speedup_vs_fp32": round(g2 / gpts, 3), "tuning": t2,
-                         "err_vs_fp64_1040": e2}
+                         "err_vs_fp64_1040": e2, "placement": p2}
             tb2, _ = measured_traffic(n, 1 if dt == "fp16" else 2, t2)
             if tb2 is not None:
                 extra[dt]["traffic_gbs"] = round(tb2 / (k2 / l2 * 1e-3) / 1e9, 1)
             if one_step:
                 extra[dt]["speedup_vs_fp32_one_step"] = round(g2 / one_step["gpts"], 3)
+            if p2:
+                log(f"[bench] {dt} placement: ms/step of the candidates {p2['ms_per_step']}, "
+                    f"kept {p2['chosen']}")
             log(f"[bench] {dt}: {g2:.2f} Gpts/s ({g2 / gpts:.2f}x fp32)")
     if twin is not None:
         twin.close()

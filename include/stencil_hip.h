@@ -294,9 +294,9 @@ int sth_bw_probe(sth_ctx *ctx, uint32_t reads, uint32_t writes, uint32_t lane_by
  * the fastest and frees the others.  ms[0 .. tries] (if not null) receives
  * the ms per step of the current set and of each candidate (0: not
  * allocated), *chosen (if not null) the index kept.  Call it right after
- * sth_create and after sth_set_tuning: field contents are undefined
- * afterwards.  FP32 and FP64 storage (else STH_ERR_UNSUPPORTED); not for a
- * batch's shots. */
+ * sth_create and after sth_set_tuning: field contents (and, for 16-bit
+ * storage, the scale state) are undefined afterwards, until the fields are
+ * initialised or uploaded.  Any storage type; not for a batch's shots. */
 int sth_place(sth_ctx *ctx, uint32_t tries, float *ms, uint32_t *chosen);
 
 /* ---- batching: many independent grids ("shots") advanced together ------ */

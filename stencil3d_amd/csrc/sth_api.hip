@@ -1477,6 +1477,14 @@ static int time_fields(sth_ctx *c, void *const set[5], bool twins, float *ms) {
   c->ubuf = 0;
   c->slot = 0;
   c->twins_ok = true;  // the halos do not matter to the time
+  if (is16(c)) {
+    // a scale state for the timing (the work does not depend on the values)
+    for (int b = 0; b < 3; b++) {
+      c->host_slot.exp[b] = 0;
+      c->host_slot.maxbits[b] = 0x3f800000u;  // 1.0
+    }
+    c->host_dirty = true;
+  }
   const float c1 = 0.1f, c0 = (float)(2.0 - lap_gain(c->g.npts) * c1);
   const uint32_t spp = twins ? (uint32_t)make_plan2(c, c->tune).s : 1;
   const Plan p = make_plan(c, c->tune);
@@ -1495,7 +1503,6 @@ static int time_fields(sth_ctx *c, void *const set[5], bool twins, float *ms) {
 
 int sth_place(sth_ctx *c, uint32_t tries, float *ms, uint32_t *chosen) {
   if (!c || tries > 8) return STH_ERR_ARG;
-  if (c->fmt != FMT_F32 && c->fmt != FMT_F64) return STH_ERR_UNSUPPORTED;
   if (c->no_fuse) return STH_ERR_STATE;  // a batch's shot: its tables hold the buffers
   const bool twins = fused_applies(c, c->tune);
   const int nb = twins ? 5 : 3;
@@ -1565,6 +1572,13 @@ int sth_place(sth_ctx *c, uint32_t tries, float *ms, uint32_t *chosen) {
   c->slot = 0;
   c->twins_ok = false;
   c->timing = timing;
+  if (is16(c)) {
+    // the kept buffers hold what the timing left: no scale state, no zero
+    // halo, until the fields are initialised
+    c->host_slot = Q16Slot{};
+    c->host_dirty = true;
+    for (int b = 0; b < 3; b++) c->halo_zero[b] = false;
+  }
   if (st) return st;
   if (ms)
     for (uint32_t t = 0; t <= tries; t++) ms[t] = t <= got ? t_ms[t] : 0.0f;

@@ -58,7 +58,41 @@ def test_place_fp64_and_rejections(gpu, orc):
         with pytest.raises(gpu.SthError) as e:
             ctx.place(9)
         assert e.value.status == gpu.STH_ERR_ARG
+
+
+
+@pytest.mark.parametrize("npts,shape,spp", [(25, (20, 30, 200), 1), (7, (40, 130, 300), 2)])
+def test_place_fp16_then_leapfrog_bit_exact(gpu, orc, npts, shape, spp):
+    """Placement of 16-bit fields: the kept buffers hold no scale state and
+    no zero halo until the fields are set (a LEAPFROG call before that is
+    refused); after an upload the steps (the packed sweep, or for 7 points the
+    two-step pass) are bit-identical to the oracle's emulation."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 47, 0.0)
+    s *= np.float32(1e-3)
+    qs, exps, maxs = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(1, g, x, e)
+        qs.append(q)
+        exps.append(e)
+        maxs.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.array([1e-2, -2e-2, 3e-2, 0.0], np.float32)
     with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+        ms, chosen = ctx.place(2)
+        assert ms[chosen] > 0
         with pytest.raises(gpu.SthError) as e:
-            ctx.place(1)
+            ctx.leapfrog(1, c0, c1)
         assert e.value.status == gpu.STH_ERR_UNSUPPORTED
+        assert ctx.get_tuning()["steps_per_pass"] == spp
+        for f, q, ex in zip(("u", "v", "s"), qs, exps):
+            ctx.upload_raw16(f, q, ex, R)
+        ctx.leapfrog(4, c0, c1, c2)
+        gu, eu, mu = ctx.download_raw16("u", R)
+    a, b = qs[0].copy(), qs[1].copy()
+    newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
+                              np.float32(c1), c2, 4, pass2=spp == 2)
+    assert eu == exps[newest]
+    assert np.array_equal(gu[g.interior], (a if newest == 0 else b)[g.interior])
