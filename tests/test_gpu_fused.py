@@ -498,3 +498,62 @@ def test_more_tiles_than_cus(gpu, orc, npts, shape, steps):
             gv = ctx.download("v", R)
         assert same_bits(gu, want_u), chunks
         assert same_bits(gv, want_v), chunks
+
+
+@pytest.mark.parametrize("npts", [7, 13])
+def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts):
+    """FP16 LEAPFROG split over calls of 3, 2 and 1 steps with the two-step
+    pass (each call its own passes and remainder), then autotuned (the
+    tuner also times the FP16 pass and the taller packed tiles) and stepped
+    on: bit-identical to the oracle's emulation of the same schedule."""
+    shape = (24, 70, 200)
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    u, v, s = seeded(orc, g, 53, 0.0)
+    s *= np.float32(1e-3)
+    qs, exps, maxs = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(1, g, x, e)
+        qs.append(q)
+        exps.append(e)
+        maxs.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.linspace(0.3, -0.3, 6).astype(np.float32)
+    a, b = qs[0].copy(), qs[1].copy()
+    bufs, cur = [a, b], 0
+    with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+        ctx.set_tuning(steps_per_pass=2)
+        for f, q, ex in zip(("u", "v", "s"), qs, exps):
+            ctx.upload_raw16(f, q, ex, R)
+        done = 0
+        for k in (3, 2, 1):
+            ctx.leapfrog(k, c0, c1, c2[done:done + k])
+            e3 = [exps[cur], exps[1 - cur], exps[2]]
+            m3 = [maxs[cur], maxs[1 - cur], maxs[2]]
+            n = orc.leapfrog_q16(npts, 1, g, bufs[cur], bufs[1 - cur], qs[2], e3, m3,
+                                 np.float32(c0), np.float32(c1), c2[done:done + k], k,
+                                 pass2=True)
+            exps[cur], exps[1 - cur] = e3[0], e3[1]
+            maxs[cur], maxs[1 - cur] = m3[0], m3[1]
+            cur = cur if n == 0 else 1 - cur
+            done += k
+        gu, eu, _ = ctx.download_raw16("u", R)
+        assert eu == exps[cur]
+        assert np.array_equal(gu[g.interior], bufs[cur][g.interior])
+        # autotune (it steps the fields), then start again from the same fields
+        t, ms = ctx.autotune(reps=1)
+        assert ms > 0
+        spp = ctx.get_tuning()["steps_per_pass"]
+        for f, q, ex in zip(("u", "v", "s"), qs, (orc.scale_exp(float(np.abs(x).max()))
+                                                 for x in (u, v, s))):
+            ctx.upload_raw16(f, q, ex, R)
+        ctx.leapfrog(5, c0, c1, c2[:5])
+        gu, eu, _ = ctx.download_raw16("u", R)
+    e0 = [orc.scale_exp(float(np.abs(x).max())) for x in (u, v, s)]
+    m0 = [orc.quantize(1, g, x, e)[1] for x, e in zip((u, v, s), e0)]
+    a, b = qs[0].copy(), qs[1].copy()
+    n = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], e0, m0, np.float32(c0), np.float32(c1),
+                         c2[:5], 5, pass2=spp == 2)
+    assert eu == e0[n]
+    assert np.array_equal(gu[g.interior], (a if n == 0 else b)[g.interior])
