@@ -48,11 +48,11 @@ struct Fused2hShape {
   static constexpr int LDS_WORDS = 2 * LU + 2 * LP + SR * LS;
 };
 
-// The coefficients of both steps on the stored numbers, and the exponents.
+// The coefficients of both steps on the stored numbers (e1, e2: the output
+// exponents of the two steps).
 struct Coef2h {
   float k0a, k1a, k2a, kva;  // step one: c0 2^(eu-e1), c1 2^(eu-e1), c2a 2^(es-e1), 2^(ev-e1)
   float k0b, k1b, k2b, kvb;  // step two: c0 2^(e1-e2), c1 2^(e1-e2), c2b 2^(es-e2), 2^(eu-e2)
-  int e1, e2;
 };
 
 // The output exponents of the pass's two steps (see the file comment).
@@ -501,8 +501,6 @@ __global__ __launch_bounds__(64 * NW) void sweep2h_kernel(const Sweep2Args a) {
   }
   const int eu = cur.exp[a.ubuf], ev = cur.exp[a.vbuf], es = cur.exp[2];
   Coef2h K;
-  K.e1 = e1;
-  K.e2 = e2;
   K.k0a = ldexpf(a.c0, eu - e1);
   K.k1a = ldexpf(a.c1, eu - e1);
   K.k2a = ldexpf(a.c2a, es - e1);
