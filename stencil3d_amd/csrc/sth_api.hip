@@ -264,6 +264,29 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
   const int TJ = p.mj * p.nw, TK = 64 * p.vec;
   p.ntk = ((int)c->g.zs + TK - 1) / TK;
   p.ntj = ((int)c->g.ys + TJ - 1) / TJ;
+  // A last k-tile with at most 64/f lanes' columns runs folded (f rows per
+  // lane group): at 1040 columns and 128-column tiles it holds 16 columns,
+  // and unfolded it would cost as much as a full tile.  FP32 folds it onto
+  // the tile's own rows; the packed FP16 sweep runs it f tiles' rows at a
+  // time (measured at 1040^3, 25 points, 2x8: 1.89 ms per step against
+  // 1.93 unfolded).  What the narrow column still costs is mostly its rows'
+  // 128-byte lines, which L2 fetches whole: 1032 to 1088 columns take about
+  // as long per step, 6 to 10 % more than 1024.
+  p.edge = 0;
+  const int we = (int)c->g.zs - (p.ntk - 1) * 64 * p.vec;
+  if (c->fmt == FMT_F32 && we < 64 * p.vec)
+    for (int f : {4, 2, 1})
+      if (64 / f >= we && p.mj % f == 0 && (f > 1 || p.vec == 2) &&
+          edge_supported(p.mj, p.nw, p.vec, f)) {
+        p.edge = f;
+        break;
+      }
+  if (packed16(c->fmt, MODE_LEAP, p.vec) && we < 128)
+    for (int f : {8, 4, 2})
+      if (128 / f >= we && edge16_supported((int)c->g.npts, p.mj, p.nw, f)) {
+        p.edge = f;
+        break;
+      }
   const int xs = (int)c->g.xs;
   int ch = (int)t.chunks;
   if (ch == 0) {
@@ -285,7 +308,12 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
     p.chunk_len = std::min(xs, std::max(u, (p.chunk_len + u / 2) / u * u));
   }
   p.chunks = (xs + p.chunk_len - 1) / p.chunk_len;
-  p.grid = (unsigned)(p.ntk * p.ntj * p.chunks);
+  // items per chunk: every tile; with the packed sweep's folded edge, the
+  // full tiles and one folded tile per p.edge rows of tiles
+  const int per_chunk = packed16(c->fmt, MODE_LEAP, p.vec) && p.edge > 0
+                            ? p.ntj * (p.ntk - 1) + (p.ntj + p.edge - 1) / p.edge
+                            : p.ntk * p.ntj;
+  p.grid = (unsigned)(per_chunk * p.chunks);
   p.items = p.grid;
   p.persist = t.launch != 1 && c->tickets_dev != nullptr;
   if (p.persist) {
@@ -294,7 +322,7 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
     // and a workgroup's next tile starts without a launch)
     int per_cu = 0;
     const void *fn = packed16(c->fmt, MODE_LEAP, p.vec)
-                         ? sweep16_fn((int)c->g.npts, p.mj, p.nw)
+                         ? sweep16_fn((int)c->g.npts, p.mj, p.nw, p.edge)
                          : sweep_fn((int)c->g.npts, MODE_LEAP, c->fmt, p.mj, p.nw, p.vec, 0);
     if (!fn || hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * p.nw, 0) !=
                    hipSuccess || per_cu < 1) {
@@ -314,18 +342,6 @@ Plan make_plan(const sth_ctx *c, const sth_tuning &t) {
     const bool banded = t.xcd_remap && (c->fmt == FMT_F16 || c->fmt == FMT_BF16);
     p.band = banded ? rows * p.ntk : (int)p.items;
   }
-  // A last k-tile with at most 64/f columns runs folded (f rows per wave):
-  // at 1040 columns and 128-column tiles it holds 16 columns, and unfolded it
-  // would cost as much as a full tile.
-  p.edge = 0;
-  const int we = (int)c->g.zs - (p.ntk - 1) * 64 * p.vec;
-  if (c->fmt == FMT_F32 && we < 64 * p.vec)
-    for (int f : {4, 2, 1})
-      if (64 / f >= we && p.mj % f == 0 && (f > 1 || p.vec == 2) &&
-          edge_supported(p.mj, p.nw, p.vec, f)) {
-        p.edge = f;
-        break;
-      }
   return p;
 }
 
@@ -371,7 +387,7 @@ int launch_one(sth_ctx *c, const Plan &p, int mode, SweepArgs &a) {
   }
   const hipError_t e =
       packed16(c->fmt, mode, p.vec)
-          ? launch_sweep16((int)c->g.npts, p.mj, p.nw, p.grid, c->stream, a)
+          ? launch_sweep16((int)c->g.npts, p.mj, p.nw, p.edge, p.grid, c->stream, a)
           : launch_sweep((int)c->g.npts, mode, c->fmt, p.mj, p.nw, p.vec, p.edge, p.grid,
                          c->stream, a);
   if (e != hipSuccess) return hip_status(e);

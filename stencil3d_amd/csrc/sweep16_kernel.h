@@ -43,16 +43,26 @@
 
 namespace sth {
 
-// LDS plane of a packed tile, in 32-bit words: TJ + 2R rows of 64 words
-// (the lanes' pairs) plus KH = ceil(R/2) words of k-halo on each side.
-template <int NPTS, int MJ, int NW>
+// LDS plane of a packed tile, in 32-bit words: TJ + 2R rows of LL words
+// (the lanes' pairs) plus KH = ceil(R/2) words of k-halo on each side.  F
+// is the fold: a wave's 64 lanes form F rows of LL = 64/F lanes, so the tile
+// is TJ = NW MJ F rows by 2 LL points (F = 1: the regular tile, one row of
+// 128 points per lane group).
+template <int NPTS, int MJ, int NW, int F = 1>
 struct Tile16 {
   static constexpr int R = Terms<NPTS>::R;
   static constexpr int KH = (R + 1) / 2;
-  static constexpr int LWW = 64 + 2 * KH;
-  static constexpr int SIZE = (NW * MJ + 2 * R) * LWW;
+  static constexpr int LL = 64 / F;
+  static constexpr int LWW = LL + 2 * KH;
+  static constexpr int TJ = NW * MJ * F;
+  static constexpr int SIZE = (TJ + 2 * R) * LWW;
   static constexpr int D = 3;  // planes loaded ahead
 };
+
+// A folded tile (F > 1) loads its k-halo as one 32-bit word per thread.
+__host__ __device__ constexpr bool fold16_fits(int R, int mj, int f) {
+  return f == 2 || f == 4 || f == 8 ? mj * f * 2 * ((R + 1) / 2) <= 64 : false;
+}
 
 // Slots of the s and v rings: the smallest length of at least d+1 that
 // divides the u ring's length qu.
@@ -105,14 +115,19 @@ struct Coef16 {
   float kc0, kc1, kc2, kv;
 };
 
-template <int NPTS, int MJ, int NW>
+// One tile: lane l of wave w holds the pair at k = k0 + 2 (l % LL) of rows
+// j0 + (w MJ + m) F + l / LL, m < MJ.  F > 1 folds a narrow tile (the last
+// k-tile of rows whose length is not a multiple of 128) onto whole waves: a
+// lane's j-neighbours then come from LDS, and its rows differ across the
+// wave, so every result is stored under a per-lane mask.
+template <int NPTS, int MJ, int NW, int F = 1>
 __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K, int k0, int j0,
                                              int ib, int ie, uint32_t *lds, float *red) {
   using Tm = Terms<NPTS>;
-  using G = Tile16<NPTS, MJ, NW>;
-  constexpr int R = Tm::R, T = Tm::T, KH = G::KH, LWW = G::LWW;
-  constexpr int TJ = NW * MJ, TK = 128;
-  constexpr int HJ = (2 * R + NW - 1) / NW;  // j-halo rows loaded per wave
+  using G = Tile16<NPTS, MJ, NW, F>;
+  constexpr int R = Tm::R, T = Tm::T, KH = G::KH, LWW = G::LWW, LL = G::LL;
+  constexpr int TJ = G::TJ, TK = 2 * LL;
+  constexpr int HJ = (2 * R + NW * F - 1) / (NW * F);  // j-halo rows loaded per wave
   // Loads run D planes ahead.  u ring: the 2R+1 planes of the stencil and
   // the D in flight, rounded up to an even length (the LDS buffers
   // alternate); s / v rings: the smallest length of at least D+1 slots that
@@ -121,12 +136,13 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
   constexpr int QU = (2 * R + D + 2) / 2 * 2;
   constexpr int SV = sv_ring(QU, D);
   constexpr int UL = QU;
-  static_assert(TJ * 2 * R <= 64 * NW, "one k-halo cell per thread");
+  static_assert(F == 1 ? TJ * 2 * R <= 64 * NW : fold16_fits(R, MJ, F), "one k-halo cell per thread");
   using GB = __attribute__((address_space(1))) char;
   using LW32 = __attribute__((address_space(3))) uint32_t;
   using LW16 = __attribute__((address_space(3))) uint16_t;
 
   const int lane = threadIdx.x & 63;
+  const int lc = F == 1 ? lane : lane % LL, lf = F == 1 ? 0 : lane / LL;  // pair, row in the fold
   // the wave index as a scalar: conditions on rows are wave-uniform branches
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int tid = threadIdx.x;
@@ -145,38 +161,56 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
   // aligned 32-bit word.
   const int jmax = a.ys + R - 1 - j0;
   const int kmax = a.zs + R - 1 - k0;
-  const int kc = min(2 * lane, (kmax / 2) * 2);
-  // every lane's pair is inside (a wave-uniform branch)
-  const bool kfull = __builtin_amdgcn_readfirstlane((int)(k0 + TK <= a.zs));
+  const int kc = min(2 * lc, (kmax / 2) * 2);
+  // every lane's pair is inside (a wave-uniform branch; folded tiles always
+  // store under the per-lane mask)
+  const bool kfull = F == 1 && __builtin_amdgcn_readfirstlane((int)(k0 + TK <= a.zs));
   uint32_t off[MJ];
   bool val0[MJ], val1[MJ];  // the lane's first / second point is inside
-  int rowin[MJ];            // the row is inside the interior (uniform)
+  int rowin[MJ];            // the row is inside the interior (uniform; folded: 1)
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
-    const int row = w * MJ + m;
+    const int row = (w * MJ + m) * F + lf;
     off[m] = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (kc + R)) * 2);
-    rowin[m] = __builtin_amdgcn_readfirstlane((int)(j0 + row < a.ys));
-    val0[m] = (j0 + row < a.ys) && (k0 + 2 * lane < a.zs);
-    val1[m] = (j0 + row < a.ys) && (k0 + 2 * lane + 1 < a.zs);
+    rowin[m] = F == 1 ? __builtin_amdgcn_readfirstlane((int)(j0 + row < a.ys)) : 1;
+    val0[m] = (j0 + row < a.ys) && (k0 + 2 * lc < a.zs);
+    val1[m] = (j0 + row < a.ys) && (k0 + 2 * lc + 1 < a.zs);
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
 #pragma unroll
   for (int h = 0; h < HJ; h++) {
-    const int hh = w + NW * h;  // halo row id 0 .. 2R-1
+    const int hh = (w + NW * h) * F + lf;  // halo row id 0 .. 2R-1
     const int trow = hh < R ? hh - R : TJ + hh - R;
     hrow[h] = hh < 2 * R ? (hh < R ? hh : TJ + hh) : -1;  // LDS row, -1: none
     hoff[h] = (uint32_t)(((int64_t)(min(trow, jmax) + R) * Pj + (kc + R)) * 2);
   }
-  // k-halo: thread tid < TJ 2R loads one 16-bit cell (row tid / 2R, R
-  // columns on either side of the tile).
-  const bool has_kh = tid < TJ * 2 * R;
+  // k-halo.  F = 1: thread tid < TJ 2R loads one 16-bit cell (row tid / 2R,
+  // R columns on either side of the tile).  F > 1: thread tid < TJ 2KH loads
+  // one 32-bit word (row tid / 2KH, KH words on either side), clamped like
+  // the lanes' pairs: a word past the last readable column is needed by no
+  // point inside; for odd R the outer half of a left word (column k0 - R - 1)
+  // is never read, and lies inside the allocation (row >= 0 has R rows of
+  // halo above it).
+  bool has_kh;
   uint32_t khoff = 0, khpos = 0;
-  if (has_kh) {
-    const int row = tid / (2 * R), c = tid % (2 * R);
-    const int tcol = c < R ? c - R : TK + c - R;
-    khoff = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R)) * 2);
-    khpos = (uint32_t)((row + R) * 2 * LWW + 2 * KH + tcol);  // half index
+  if constexpr (F == 1) {
+    has_kh = tid < TJ * 2 * R;
+    if (has_kh) {
+      const int row = tid / (2 * R), c = tid % (2 * R);
+      const int tcol = c < R ? c - R : TK + c - R;
+      khoff = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (min(tcol, kmax) + R)) * 2);
+      khpos = (uint32_t)((row + R) * 2 * LWW + 2 * KH + tcol);  // half index
+    }
+  } else {
+    has_kh = tid < TJ * 2 * KH;
+    if (has_kh) {
+      const int row = tid / (2 * KH), c = tid % (2 * KH);
+      const int wc = c < KH ? c - KH : LL + c - KH;  // word column from the tile's first
+      const int tcol = min(2 * wc, (kmax / 2) * 2);
+      khoff = (uint32_t)(((int64_t)(min(row, jmax) + R) * Pj + (tcol + R)) * 2);
+      khpos = (uint32_t)(((row + R) * LWW + KH + wc) * 2);  // half index
+    }
   }
 
   auto uni = [](const void *p) {
@@ -198,6 +232,12 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
   auto ld16 = [&](GB *b, uint32_t o) {
     return (uint32_t)buf_load<uint16_t, 0>(buf_res_n(b, plane_bytes), o);
   };
+  auto ldkh = [&](GB *b, uint32_t o) {  // a k-halo cell
+    if constexpr (F == 1)
+      return ld16(b, o);
+    else
+      return ld32(b, o);
+  };
 #pragma unroll
   for (int h = 0; h < HJ; h++)
     if (hrow[h] < 0) hoff[h] = kNone;
@@ -207,10 +247,10 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
   // word base passes through an empty asm statement at each step so the
   // compiler does not keep one register per distinct address.
   const uint32_t l0 = (uint32_t)(uintptr_t)(LW32 *)&lds[0];
-  uint32_t lb = l0 + (uint32_t)((w * MJ * LWW + lane) * 4);
+  uint32_t lb = l0 + (uint32_t)(((w * MJ * F + lf) * LWW + lc) * 4);
   uint32_t hadr[HJ];
 #pragma unroll
-  for (int h = 0; h < HJ; h++) hadr[h] = l0 + (uint32_t)((max(hrow[h], 0) * LWW + KH + lane) * 4);
+  for (int h = 0; h < HJ; h++) hadr[h] = l0 + (uint32_t)((max(hrow[h], 0) * LWW + KH + lc) * 4);
   const uint32_t khadr = l0 + khpos * 2;
   auto w32 = [](uint32_t adr) -> LW32 & { return *(LW32 *)(uintptr_t)adr; };
   auto w16 = [](uint32_t adr) -> LW16 & { return *(LW16 *)(uintptr_t)adr; };
@@ -235,7 +275,7 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
     GB *Up = pl(Ub, ib + t);
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][R + t] = ld32(Up, hoff[h]);
-    kq[R + t] = ld16(Up, khoff);
+    kq[R + t] = ldkh(Up, khoff);
   }
   // u(i+R) with its halo, s(i), v(i) into the slots of step k's plane + d
   // (d = 0, 1: the first two planes; d = 2: the prefetch of step k).  s and
@@ -252,7 +292,7 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][su] = ld32(Upl, hoff[h]);
-    kq[su] = ld16(Upl, khoff);
+    kq[su] = ldkh(Upl, khoff);
   };
   // Plane pointers of the prefetch (u(i+2+R), s(i+2), v(i+2)), one plane
   // further at every step while the plane is inside the chunk (they stay on
@@ -286,11 +326,16 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
 
     asm volatile("" : "+v"(lb));
 #pragma unroll
-    for (int m = 0; m < MJ; m++) w32(lb + BO + (uint32_t)(((m + R) * LWW + KH) * 4)) = q[m][sC];
+    for (int m = 0; m < MJ; m++) w32(lb + BO + (uint32_t)(((m * F + R) * LWW + KH) * 4)) = q[m][sC];
 #pragma unroll
     for (int h = 0; h < HJ; h++)
       if (hrow[h] >= 0) w32(hadr[h] + BO) = hq[h][sC];
-    if (has_kh) w16(khadr + BO) = (uint16_t)kq[sC];
+    if (has_kh) {
+      if constexpr (F == 1)
+        w16(khadr + BO) = (uint16_t)kq[sC];
+      else
+        w32(khadr + BO) = kq[sC];
+    }
     __syncthreads();
 
     // Steps past the chunk (in the last turn of the rings) keep the barrier
@@ -304,15 +349,16 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
       uint32_t kw[2 * KH + 1];
 #pragma unroll
       for (int p = -KH; p <= KH; p++)
-        kw[KH + p] = p == 0 ? cw : w32(lb + BO + (uint32_t)(((m + R) * LWW + KH + p) * 4));
-      // j-neighbour words of rows m -/+ r outside the wave's rows
+        kw[KH + p] = p == 0 ? cw : w32(lb + BO + (uint32_t)(((m * F + R) * LWW + KH + p) * 4));
+      // j-neighbour words: the lane's own rows m -/+ r from registers, the
+      // rest (every row of a folded tile) from LDS
       uint32_t jl[R + 1], jh[R + 1];
 #pragma unroll
       for (int r = 1; r <= R; r++) {
-        jl[r] = m - r >= 0 ? q[m - r >= 0 ? m - r : 0][sC]
-                           : w32(lb + BO + (uint32_t)(((m + R - r) * LWW + KH) * 4));
-        jh[r] = m + r < MJ ? q[m + r < MJ ? m + r : 0][sC]
-                           : w32(lb + BO + (uint32_t)(((m + R + r) * LWW + KH) * 4));
+        jl[r] = F == 1 && m - r >= 0 ? q[m - r >= 0 ? m - r : 0][sC]
+                                     : w32(lb + BO + (uint32_t)(((m * F + R - r) * LWW + KH) * 4));
+        jh[r] = F == 1 && m + r < MJ ? q[m + r < MJ ? m + r : 0][sC]
+                                     : w32(lb + BO + (uint32_t)(((m * F + R + r) * LWW + KH) * 4));
       }
       float res[2];
       static_for<2>([&](auto cc) __attribute__((always_inline)) {
@@ -382,10 +428,18 @@ __device__ __forceinline__ void sweep16_tile(const SweepArgs &a, const Coef16 &K
   }
 }
 
-template <int NPTS, int MJ, int NW>
+// EF > 0: the last k-tile of every row of tiles holds at most 128/EF points
+// and runs folded (EF rows per lane group), EF tiles' rows at a time.  The
+// items of a chunk come in groups of EF rows of tiles: the rows' ntk - 1
+// full tiles in row order, then one folded tile over the group's rows, so
+// the narrow column costs 1/EF of its tiles and no item is empty (a
+// persistent workgroup's band of items is all work).
+template <int NPTS, int MJ, int NW, int EF = 0>
 __global__ __launch_bounds__(64 * NW) void sweep16_kernel(const SweepArgs a) {
   using G = Tile16<NPTS, MJ, NW>;
-  __shared__ __attribute__((aligned(16))) uint32_t lds[2 * G::SIZE];
+  using GE = Tile16<NPTS, MJ, NW, (EF > 0 ? EF : 1)>;
+  constexpr int LS = GE::SIZE > G::SIZE ? GE::SIZE : G::SIZE;
+  __shared__ __attribute__((aligned(16))) uint32_t lds[2 * LS];
   __shared__ float red[NW];
   __shared__ int sh_item;
   q16_publish<FMT_F16>(a);
@@ -401,17 +455,39 @@ __global__ __launch_bounds__(64 * NW) void sweep16_kernel(const SweepArgs a) {
     K.kc2 = ldexpf(a.c2n, es - eo);
     K.kv = ldexpf(1.0f, ev - eo);
   }
-  auto tile = [&](int item) __attribute__((always_inline)) {
-    const int tk = item % a.ntk;
-    const int t = item / a.ntk;
-    const int tj = t % a.ntj;
-    const int ch = t / a.ntj;
-    // the item is the same in every lane: keep its bounds scalar
-    const int ib = __builtin_amdgcn_readfirstlane(ch * a.chunk_len);
-    const int ie = __builtin_amdgcn_readfirstlane(min(a.xs, ib + a.chunk_len));
+  auto tile = [&](int item_) __attribute__((always_inline)) {
+    // the item is the same in every lane: keep it and its bounds scalar
+    const int item = __builtin_amdgcn_readfirstlane(item_);
+    int tk, tj, ch;
+    if constexpr (EF > 0) {
+      const int m = a.ntk - 1;                              // full tiles per row
+      const int ni = a.ntj * m + (a.ntj + EF - 1) / EF;     // items per chunk
+      ch = item / ni;
+      const int r = item - ch * ni, gs = EF * m + 1;        // group size
+      const int g = r / gs, o = r - g * gs;
+      if (o < min(EF, a.ntj - g * EF) * m) {
+        tj = g * EF + o / m;
+        tk = o % m;
+      } else {
+        tj = g * EF;
+        tk = m;
+      }
+    } else {
+      tk = item % a.ntk;
+      const int t = item / a.ntk;
+      tj = t % a.ntj;
+      ch = t / a.ntj;
+    }
+    const int ib = ch * a.chunk_len;
+    const int ie = min(a.xs, ib + a.chunk_len);
     if (ib >= ie) return;
-    sweep16_tile<NPTS, MJ, NW>(a, K, __builtin_amdgcn_readfirstlane(tk * 128),
-                               __builtin_amdgcn_readfirstlane(tj * NW * MJ), ib, ie, lds, red);
+    if constexpr (EF > 0) {
+      if (tk == a.ntk - 1) {
+        sweep16_tile<NPTS, MJ, NW, EF>(a, K, tk * 128, tj * NW * MJ, ib, ie, lds, red);
+        return;
+      }
+    }
+    sweep16_tile<NPTS, MJ, NW>(a, K, tk * 128, tj * NW * MJ, ib, ie, lds, red);
   };
   for_items(a, sh_item, tile);
 }

@@ -120,10 +120,12 @@ const void *sweep_fn_25(int mode, int fmt, int mj, int nw, int vec, int ef);
 
 // The packed FP16 LEAPFROG sweep (sweep16_kernel.h, sweep16.hip): the tiles
 // of two k points per lane listed above.
-hipError_t launch_sweep16(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+// ef > 0: the kernel with a folded last k-tile (edge16_supported).
+hipError_t launch_sweep16(int npts, int mj, int nw, int ef, unsigned grid, hipStream_t st,
                           const SweepArgs &a);
-const void *sweep16_fn(int npts, int mj, int nw);
+const void *sweep16_fn(int npts, int mj, int nw, int ef);
 bool tile16_supported(int mj, int nw);
+bool edge16_supported(int npts, int mj, int nw, int ef);
 // Steps per loop turn of the packed sweep: a tile's i-chunk of a multiple of
 // it runs no idle step.
 inline int sweep16_turn(int R) {

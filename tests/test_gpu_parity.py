@@ -235,6 +235,52 @@ def test_fp16_packed_tiles_bit_exact(gpu, orc, npts, tile, shape, chunks):
     assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
 
 
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("tile", [(2, 8), (2, 16)])
+@pytest.mark.parametrize("zs", [16, 144, 145, 160, 161, 192, 1040])
+def test_fp16_packed_folded_edge_bit_exact(gpu, orc, npts, tile, zs):
+    """The packed FP16 sweep's last k-tile, when it holds at most 64 of its
+    128 columns, runs folded on the default 2x8 tile (8, 4 or 2 rows per lane
+    group, 8, 4 or 2 tiles' rows per item; 2x16 runs it unfolded): widths 16
+    (a grid of one folded tile), 16, 17 and 32, 33 and 64 past the last full
+    tile, and 1040; 150 rows, so several folded tiles run down j and the last
+    one is partly outside.  Bits, exponents and maxima equal to the oracle's
+    binary16 emulation."""
+    R = orc.radius(npts)
+    shape = (9, 150, zs)
+    g = orc.Grid(*shape, R, R, R)
+    v, s, u = seeded(orc, g, seed=17 + npts)
+    s *= np.float32(1e-3)
+    qs, exps, maxs = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(1, g, x, e)
+        qs.append(q)
+        exps.append(e)
+        maxs.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.linspace(0.5, -0.5, 5).astype(np.float32)
+    with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+        ctx.set_tuning(rows_per_wave=tile[0], waves=tile[1], vec=2, chunks=2,
+                       steps_per_pass=1)
+        ctx.upload_raw16("u", qs[0], exps[0], R)
+        ctx.upload_raw16("v", qs[1], exps[1], R)
+        ctx.upload_raw16("s", qs[2], exps[2], R)
+        ctx.leapfrog(5, c0, c1, c2)
+        gu, eu, mu = ctx.download_raw16("u", R)
+        gv, ev, mv = ctx.download_raw16("v", R)
+    a, b = qs[0].copy(), qs[1].copy()
+    newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
+                              np.float32(c1), c2, 5)
+    (wu, weu, wmu), (wv, wev, wmv) = (
+        ((a, exps[0], maxs[0]), (b, exps[1], maxs[1])) if newest == 0 else
+        ((b, exps[1], maxs[1]), (a, exps[0], maxs[0])))
+    I = g.interior
+    assert (eu, ev) == (weu, wev)
+    assert np.array_equal(gu[I], wu[I]) and np.array_equal(gv[I], wv[I])
+    assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
+
+
 FP16_PASS_SHAPES = {7: [(4, 16)], 13: [(3, 16), (4, 16)], 25: [(3, 16)]}
 
 
