@@ -93,6 +93,9 @@ def lib():
             "orc_leapfrog_q16_pass2": (C.c_int, [C.c_int, C.c_int, G, u16p, u16p, u16p,
                                                  P(C.c_int), f32p, C.c_float, C.c_float, f32p,
                                                  C.c_int, C.c_int]),
+            "orc_leapfrog_q16_passes": (C.c_int, [C.c_int, C.c_int, G, u16p, u16p, u16p,
+                                                  P(C.c_int), f32p, C.c_float, C.c_float, f32p,
+                                                  C.c_int, C.c_int, C.c_int]),
             "orc_splitmix64_mix": (C.c_uint64, [C.c_uint64]),
             "orc_uniform01": (C.c_double, [C.c_uint64, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
             "orc_fill_uniform": (None, [G, f32p, C.c_uint64, C.c_int, C.c_double, C.c_double,
@@ -214,15 +217,20 @@ def dequantize(fmt, g: Grid, q, e):
 
 
 def leapfrog_q16(npts, fmt, g: Grid, a, b, s, exps, maxs, c0, c1, c2, steps, nthreads=0,
-                 pass2=False):
+                 pass2=False, plen=None):
     """exps/maxs: lists of 3 (a, b, s), updated in place.  Returns newest index.
-    pass2: the steps in passes of two (orc_leapfrog_q16_pass2)."""
+    pass2: the steps in passes of two (orc_leapfrog_q16_pass2); plen: in
+    passes of plen steps (orc_leapfrog_q16_passes)."""
     ce = (C.c_int * 3)(*exps)
     cm = (C.c_float * 3)(*maxs)
     c2a = None if c2 is None else np.ascontiguousarray(c2, dtype=np.float32)
-    fn = lib().orc_leapfrog_q16_pass2 if pass2 else lib().orc_leapfrog_q16
-    rc = fn(npts, fmt, C.byref(g.c()), u16p(a), u16p(b), u16p(s), ce, cm,
-            c0, c1, None if c2a is None else f32p(c2a), steps, nthreads)
+    args = (npts, fmt, C.byref(g.c()), u16p(a), u16p(b), u16p(s), ce, cm,
+            c0, c1, None if c2a is None else f32p(c2a), steps)
+    if plen is not None:
+        rc = lib().orc_leapfrog_q16_passes(*args, plen, nthreads)
+    else:
+        fn = lib().orc_leapfrog_q16_pass2 if pass2 else lib().orc_leapfrog_q16
+        rc = fn(*args, nthreads)
     assert rc >= 0
     exps[:] = list(ce)
     maxs[:] = list(cm)

@@ -405,47 +405,50 @@ int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t
   return cur;
 }
 
-/* The same steps in passes of two (the GPU's FP16 two-step pass): within a
- * pass the second step's output exponent comes from the bound of the first
- * step's output in place of its maximum, B2 = (|c0| + |c1| gain) B1 + M_u +
- * |c2[n+1]| M_s with B1 the first step's bound; a step left over runs alone. */
+/* The same steps in passes of plen (the GPU's FP16 multi-step passes): the
+ * output exponent of a pass's step M is orc_scale_exp of the bound
+ *   B_M = (|c0| + |c1| gain) B_{M-1} + B_{M-2} + |c2[n+M-1]| M_s,
+ * B_0 = M_u, B_{-1} = M_v (the stored inputs' maxima), so B_1 is the
+ * one-step rule's bound and each later bound stands in for the maximum of
+ * an output the pass does not know before it ends.  The steps left over run
+ * as one shorter pass (a single step: the one-step rule). */
+int orc_leapfrog_q16_passes(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
+                            const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
+                            const float *c2, int steps, int plen, int nthreads) {
+  term_tab tb;
+  if (make_tab(npts, g, &tb, 1, 0) < 0 || steps < 0 || plen < 1 || plen > 8 ||
+      (fmt != 1 && fmt != 2))
+    return -1;
+  set_threads(nthreads);
+  const double gmul = fabs((double)c0) + fabs((double)c1) * orc_lap_gain(npts);
+  uint16_t *bufs[2] = {a, b};
+  int cur = 0;
+  for (int n = 0; n < steps;) {
+    const int len = steps - n < plen ? steps - n : plen;
+    const double Ms = ldexp((double)maxs[2], exps[2]);
+    double b2 = ldexp((double)maxs[1 - cur], exps[1 - cur]); /* B_{M-2} */
+    double b1 = ldexp((double)maxs[cur], exps[cur]);         /* B_{M-1} */
+    for (int M = 1; M <= len; M++, n++) {
+      const float c2n = c2 ? c2[n] : 0.0f;
+      const double B = gmul * b1 + b2 + fabs((double)c2n) * Ms;
+      const int eo = orc_scale_exp(B);
+      const float mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, eo, c0, c1,
+                                c2n);
+      exps[1 - cur] = eo;
+      maxs[1 - cur] = mx;
+      cur = 1 - cur;
+      b2 = b1;
+      b1 = B;
+    }
+  }
+  return cur;
+}
+
 int orc_leapfrog_q16_pass2(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
                            const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
                            const float *c2, int steps, int nthreads) {
-  term_tab tb;
-  if (make_tab(npts, g, &tb, 1, 0) < 0 || steps < 0 || (fmt != 1 && fmt != 2)) return -1;
-  set_threads(nthreads);
-  const double gain = orc_lap_gain(npts);
-  const double gmul = fabs((double)c0) + fabs((double)c1) * gain;
-  uint16_t *bufs[2] = {a, b};
-  int cur = 0, n = 0;
-  for (; n + 2 <= steps; n += 2) {
-    const float c2a = c2 ? c2[n] : 0.0f, c2b = c2 ? c2[n + 1] : 0.0f;
-    const double Mu = ldexp((double)maxs[cur], exps[cur]);
-    const double Mv = ldexp((double)maxs[1 - cur], exps[1 - cur]);
-    const double Ms = ldexp((double)maxs[2], exps[2]);
-    const double B1 = gmul * Mu + Mv + fabs((double)c2a) * Ms;
-    const double B2 = gmul * B1 + Mu + fabs((double)c2b) * Ms;
-    const int e1 = orc_scale_exp(B1), e2 = orc_scale_exp(B2);
-    float mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, e1, c0, c1, c2a);
-    exps[1 - cur] = e1;
-    maxs[1 - cur] = mx;
-    cur = 1 - cur;
-    mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, e2, c0, c1, c2b);
-    exps[1 - cur] = e2;
-    maxs[1 - cur] = mx;
-    cur = 1 - cur;
-  }
-  if (n < steps) {
-    const float c2n = c2 ? c2[n] : 0.0f;
-    const int eo = q16_out_exp(exps, maxs, cur, 1 - cur, c0, c1, c2n, gain);
-    const float mx = q16_step(&tb, fmt, g, bufs[cur], bufs[1 - cur], s, exps, cur, eo, c0, c1,
-                              c2n);
-    exps[1 - cur] = eo;
-    maxs[1 - cur] = mx;
-    cur = 1 - cur;
-  }
-  return cur;
+  return orc_leapfrog_q16_passes(npts, fmt, g, a, b, s, exps, maxs, c0, c1, c2, steps, 2,
+                                 nthreads);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -26,10 +26,12 @@
  *   orc_leapfrog_q16    fp16/bf16 storage emulation: conversions pinned by
  *                       IEEE special cases, exhaustive round trips and numpy's
  *                       float16 rounding; scaling rule pinned by invariants.
- *   orc_leapfrog_q16_pass2  the same steps in passes of two: pinned by
- *                       equality with orc_leapfrog_q16 wherever the two
- *                       exponent rules agree, and by its own invariants (no
- *                       overflow, exponent never below the one-step rule's).
+ *   orc_leapfrog_q16_passes (and _pass2, its passes of two)  the same steps
+ *                       in passes: pinned by equality with orc_leapfrog_q16
+ *                       for passes of one step and wherever the exponent
+ *                       rules agree, and by its own invariants (no overflow,
+ *                       exponent never below the one-step rule's, the bound
+ *                       recursion checked against its closed form).
  *   orc_fill_uniform    SplitMix64 published output vectors.
  */
 #ifndef STENCIL_ORACLE_H
@@ -119,12 +121,17 @@ void orc_dequantize(int fmt, const orc_geom *g, const uint16_t *q, int e, float 
 int orc_leapfrog_q16(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
                      const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
                      const float *c2, int steps, int nthreads);
-/* The same update in passes of two steps (the GPU's FP16 two-step pass):
- * in a pass the first output's exponent is the one-step rule's (bound B1 of
- * the stored inputs' maxima), the second's comes from
- *   B2 = (|c0| + |c1| gain) B1 + M_u + |c2[n+1]| M_s
- * (B1 in place of the unknown maximum of u_{n+1}); a step left over runs
- * with the one-step rule. */
+/* The same update in passes of plen steps (1..8; the GPU's FP16 multi-step
+ * passes): step M of a pass takes its output exponent from the bound
+ *   B_M = (|c0| + |c1| gain) B_{M-1} + B_{M-2} + |c2[n+M-1]| M_s,
+ *   B_0 = M_u, B_{-1} = M_v
+ * (B_1: the one-step rule's bound; later bounds stand in for the maxima of
+ * outputs the pass does not know before it ends).  The steps left over run
+ * as one shorter pass.  plen = 1 is orc_leapfrog_q16. */
+int orc_leapfrog_q16_passes(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
+                            const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
+                            const float *c2, int steps, int plen, int nthreads);
+/* orc_leapfrog_q16_passes with plen = 2 (the GPU's FP16 two-step pass). */
 int orc_leapfrog_q16_pass2(int npts, int fmt, const orc_geom *g, uint16_t *a, uint16_t *b,
                            const uint16_t *s, int exps[3], float maxs[3], float c0, float c1,
                            const float *c2, int steps, int nthreads);

@@ -431,6 +431,87 @@ def test_q16_pass2_against_one_step_rule(orc, npts):
     assert np.array_equal(b3[I], b4[I]) and e3 == e4 and m3 == m4
 
 
+@pytest.mark.parametrize("npts", [7, 13, 25])
+def test_q16_passes_bound_recursion(orc, npts):
+    """Passes of L steps (orc_leapfrog_q16_passes).  A pass of one step is the
+    one-step rule, bit for bit.  The exponents of a pass's last two outputs
+    are scale_exp of B_{L-1} and B_L from the recursion
+    B_M = (|c0| + |c1| gain) B_{M-1} + B_{M-2} + |c2[M-1]| M_s, B_0 = M_u,
+    B_{-1} = M_v, recomputed here from the inputs' maxima (a wrong index or
+    coefficient in the recursion moves an exponent); each output's maximum
+    stays at or below its bound (no overflow); 2L + r steps are two passes of
+    L and one of r."""
+    import math
+    R = orc.radius(npts)
+    g = orc.Grid(8, 9, 10, R, R, R)
+    fmt = 1
+    u = orc.fill_gaussian(g, 1.0, 2.0, 1e-3, 2, 0)
+    v = orc.fill_gaussian(g, 0.7, 2.0, 1e-3, 2, 1)
+    s = orc.fill_uniform(g, 4, 2, -1e-3, 1e-3)
+    qs, ex, mx = [], [], []
+    for x in (u, v, s):
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, m = orc.quantize(fmt, g, x, e)
+        qs.append(q)
+        ex.append(e)
+        mx.append(m)
+    c1 = np.float32(0.16)
+    c0 = np.float32(2 - orc.lap_gain(npts) * 0.16)
+    c2 = np.array([0.3, -0.2, 0.5, -0.4, 0.25, 0.1, -0.3, 0.2, 0.6, -0.1, 0.35, 0.15, -0.25],
+                  np.float32)
+    I = g.interior
+
+    def run(steps, **kw):
+        a, b = qs[0].copy(), qs[1].copy()
+        e, m = list(ex), list(mx)
+        n = orc.leapfrog_q16(npts, fmt, g, a, b, qs[2], e, m, c0, c1, c2, steps, **kw)
+        return n, a, b, e, m
+
+    n1, a1, b1, e1, m1 = run(5)
+    n2, a2, b2, e2, m2 = run(5, plen=1)
+    assert n1 == n2 and np.array_equal(a1[I], a2[I]) and np.array_equal(b1[I], b2[I])
+    assert e1 == e2 and m1 == m2
+    gmul = abs(float(c0)) + abs(float(c1)) * orc.lap_gain(npts)
+    Ms = math.ldexp(float(mx[2]), ex[2])
+    for L in (2, 3, 4, 5):
+        bm2, bm1 = math.ldexp(float(mx[1]), ex[1]), math.ldexp(float(mx[0]), ex[0])
+        bounds = []
+        for M in range(1, L + 1):
+            B = gmul * bm1 + bm2 + abs(float(c2[M - 1])) * Ms
+            bounds.append(B)
+            bm2, bm1 = bm1, B
+        n, a, b, e, m = run(L, plen=L)
+        assert n == L % 2
+        last, prev = (a, b) if n == 0 else (b, a)
+        il, ip = n, 1 - n  # buffer indices of u_{n+L} and u_{n+L-1}
+        assert e[il] == orc.scale_exp(bounds[-1]) and e[ip] == orc.scale_exp(bounds[-2])
+        assert math.ldexp(float(m[il]), e[il]) <= bounds[-1]
+        assert math.ldexp(float(m[ip]), e[ip]) <= bounds[-2]
+        assert np.all(np.isfinite(orc.dequantize(fmt, g, last, e[il])[I]))
+        # 2L + r steps: two passes of L, then the remainder as one pass
+        for r in (0, 1, L - 1):
+            nA, aA, bA, eA, mA = run(2 * L + r, plen=L)
+            a, b = qs[0].copy(), qs[1].copy()
+            e, m = list(ex), list(mx)
+            cur = 0
+            for n0, ln in ((0, L), (L, L), (2 * L, r)):
+                if ln == 0:
+                    continue
+                x, y = (a, b) if cur == 0 else (b, a)
+                ee = e if cur == 0 else [e[1], e[0], e[2]]
+                mm = m if cur == 0 else [m[1], m[0], m[2]]
+                k = orc.leapfrog_q16(npts, fmt, g, x, y, qs[2], ee, mm, c0, c1, c2[n0:], ln,
+                                     plen=ln)
+                if cur == 1:
+                    e[:] = [ee[1], ee[0], ee[2]]
+                    m[:] = [mm[1], mm[0], mm[2]]
+                else:
+                    e[:], m[:] = ee, mm
+                cur = cur ^ k
+            assert nA == cur and eA == e and mA == m
+            assert np.array_equal(aA[I], a[I]) and np.array_equal(bA[I], b[I])
+
+
 # ------------------------------------------------------------------ misc
 def test_gaussian_init(orc):
     g = orc.Grid(7, 8, 9, 2, 2, 2)
