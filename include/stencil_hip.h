@@ -110,10 +110,12 @@ typedef struct {
                              the grid and the device's CU count) */
   uint32_t xcd_remap;     /* 1: keep neighbouring tiles on one XCD (default), 0: off */
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
-  uint32_t steps_per_pass; /* LEAPFROG, FP32: 2 = two steps per pass over memory
-                              (temporal blocking), 3 = three (7 and 13 points),
-                              4 or 5 = four or five (7 points); FP16: 2 (the second
-                              step's scale from the bound of the first's output);
+  uint32_t steps_per_pass; /* LEAPFROG, FP32 and FP16: 2 = two steps per pass over
+                              memory (temporal blocking), 3 = three (7 and 13
+                              points), 4 or 5 = four or five (7 points); FP16: each
+                              step's scale from the bound of the previous steps'
+                              outputs (FP16 passes: rows_per_wave 4 or 3 for two
+                              steps, 3 for three, 2 for four and five; 16 waves);
                               steps left over run as the longest shorter pass that
                               fits, then one-step; 1 = one; 0: default */
   uint32_t launch;        /* one-step sweep: 1 = one workgroup per tile, 2 = persistent

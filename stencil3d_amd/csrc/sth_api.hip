@@ -422,6 +422,42 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
   return STH_OK;
 }
 
+// FP16 passes (sweep2h_kernel.h): the default shape of each pass length
+// (rows per wave; 16 waves), and whether the stencil has a pass of that
+// length (two steps for every stencil, three for 7 and 13 points, four and
+// five for 7).
+int fp16_pass_mj(int npts, int s) { return s == 2 ? (npts == 25 ? 3 : 4) : s == 3 ? 3 : 2; }
+bool fp16_pass_exists(int npts, int s) {
+  return s >= 2 && s <= 5 && fused2h_supported(npts, fp16_pass_mj(npts, s), 16, s);
+}
+
+// The default FP16 pass length of a grid (0: the one-step sweep), from the
+// number of output tiles of each pass's default shape.  Measured on cubic
+// grids against the packed one-step sweep (one-step time / pass time per
+// step):
+//   7 points, three steps: 0.89-0.92 at 96^3 and 128^3 (3 and 6 output
+//     tiles), 2.7 at 192^3 (10), 1.8-2.0 from 256^3 to 640^3; four steps
+//     2.0-2.3 from 640^3, 2-6 % faster than three from 768^3 (240 tiles of
+//     its 2 x 16 shape) to 1040^3, 1 % slower at 640^3 (175); five steps
+//     slower than four at 1040^3 (0.98 against 0.88 ms per step); two steps
+//     slower than three everywhere;
+//   13 points, two steps 0.87-0.92 up to 512^3 (54 tiles of 4 x 16),
+//     1.1-1.25 from 640^3 (77); three steps 3-5 % faster than two from
+//     896^3 (230 tiles of 3 x 16) to 1040^3, slower at 768^3 (160);
+//   25 points: the two-step pass does 1.47 lane-updates per point-update
+//     and is instruction-bound where the sweep is held by the HBM
+//     read/write mix (0.63-0.98 from 256^3 to 1040^3), so it stays one-step.
+int fp16_default_pass(const sth_ctx *c) {
+  const int npts = (int)c->g.npts;
+  if (npts == 25 || (int)c->g.xs < 16 * c->R) return 0;
+  auto tiles = [&](int s) {
+    const Fused2Geom g = fused2h_geom(npts, fp16_pass_mj(npts, s), 16, s);
+    return (((int)c->g.zs + g.ok - 1) / g.ok) * (((int)c->g.ys + g.oj - 1) / g.oj);
+  };
+  if (npts == 7) return tiles(4) >= 200 ? 4 : tiles(3) >= 10 ? 3 : 0;
+  return tiles(3) >= 200 ? 3 : tiles(2) >= 70 ? 2 : 0;
+}
+
 // Several LEAPFROG steps per pass (FP32): the default on grids with enough
 // output tiles to fill the CUs (the thresholds below), five steps per pass
 // for 7 points, three for 13, two for 25; otherwise, or when the tuning asks
@@ -429,19 +465,8 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (c->no_fuse) return false;
   if (c->fmt == FMT_F16) {
-    if (t.steps_per_pass) return t.steps_per_pass == 2;
-    // FP16 two-step pass against the packed one-step sweep (cubic grids,
-    // one-step time / pass time per step): 7 points 0.76 at 128^3 (6
-    // output tiles), 1.40 at 256^3 (15), 1.32 at 512^3, 1.62 at 1040^3;
-    // 13 points 0.94 at 512^3 (54 output tiles of the 4 x 16 shape), 1.09 at
-    // 640^3 (77), 1.18 at 768^3, 1.35 at 1040^3; 25 points 0.63-0.98 from 256^3 to 1040^3 (the
-    // pass does 1.47 lane-updates per point-update and is instruction-bound
-    // there, the sweep is held by the HBM read/write mix), so it stays
-    // one-step.
-    const int npts = (int)c->g.npts;
-    if (npts == 25) return false;
-    const Plan2 p = make_plan2(c, t);
-    return p.ntk * p.ntj >= (npts == 7 ? 12 : 70) && (int)c->g.xs >= 16 * c->R;
+    if (t.steps_per_pass) return fp16_pass_exists((int)c->g.npts, (int)t.steps_per_pass);
+    return fp16_default_pass(c) >= 2;
   }
   if (c->fmt != FMT_F32) return false;
   if (t.steps_per_pass) return t.steps_per_pass >= 2;
@@ -469,18 +494,22 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   Plan2 p;
   Fused2Geom g;
   if (c->fmt == FMT_F16) {
-    // FP16 storage: two steps per pass (sweep2h_kernel.h)
-    p.s = 2;
-    if (fused2h_supported(npts, (int)t.rows_per_wave, (int)t.waves)) {
+    // FP16 storage (sweep2h_kernel.h): the tuning's pass length where the
+    // stencil has it, else two steps
+    p.s = fp16_pass_exists(npts, (int)t.steps_per_pass) ? (int)t.steps_per_pass
+                                                        : std::max(2, fp16_default_pass(c));
+    if (fused2h_supported(npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
       p.mj = (int)t.rows_per_wave;
       p.nw = (int)t.waves;
     } else {
-      // 7 and 13 points 4 x 16 (13 points at 1040^3: 1.42 ms/step against
-      // 1.48 for 3 x 16), 25 points 3 x 16 (4 x 16 exceeds 128 VGPRs)
-      p.mj = npts == 25 ? 3 : 4;
+      // two steps: 7 and 13 points 4 x 16 (13 points at 1040^3: 1.42 ms/step
+      // against 1.48 for 3 x 16), 25 points 3 x 16 (4 x 16 exceeds 128
+      // VGPRs); three steps 3 x 16, four and five 2 x 16 (the LDS planes of
+      // the longer passes)
+      p.mj = fp16_pass_mj(npts, p.s);
       p.nw = 16;
     }
-    g = fused2h_geom(npts, p.mj, p.nw);
+    g = fused2h_geom(npts, p.mj, p.nw, p.s);
   } else {
     if (npts == 25 || t.steps_per_pass == 2)
       p.s = 2;
@@ -612,7 +641,7 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
     a.gain = wts.gain;
   }
   const hipError_t e =
-      h16 ? launch_sweep2h((int)c->g.npts, p.mj, p.nw, p.grid, c->stream, a)
+      h16 ? launch_sweep2h((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a)
           : launch_sweep2((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (c->timing) c->launches++;
@@ -644,6 +673,7 @@ int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, cons
       // steps), then one-step sweeps
       for (int rs = p2.s - 1; rs >= 2; rs--) {
         if (n + (uint32_t)rs > steps) continue;
+        if (c->fmt == FMT_F16 && !fp16_pass_exists((int)c->g.npts, rs)) continue;
         sth_tuning t2 = c->tune;
         t2.steps_per_pass = (uint32_t)rs;
         t2.rows_per_wave = t2.waves = t2.chunks = 0;
@@ -1241,14 +1271,16 @@ int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
   if (!c || !t || t->steps_per_pass > 5 || t->launch > 2 || t->chunks > 64 ||
       t->ref_sweeps > 4096)
     return STH_ERR_ARG;
-  if (t->steps_per_pass >= 2 && c->fmt != FMT_F32 && !(c->fmt == FMT_F16 && t->steps_per_pass == 2))
+  if (t->steps_per_pass >= 2 && c->fmt != FMT_F32 &&
+      !(c->fmt == FMT_F16 && fp16_pass_exists((int)c->g.npts, (int)t->steps_per_pass)))
     return STH_ERR_UNSUPPORTED;
   if ((t->steps_per_pass == 3 && c->g.npts == 25) || (t->steps_per_pass >= 4 && c->g.npts != 7))
     return STH_ERR_UNSUPPORTED;
   const bool fused_shape =
       t->steps_per_pass >= 2 &&
       (c->fmt == FMT_F16
-           ? fused2h_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves)
+           ? fused2h_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves,
+                               (int)t->steps_per_pass)
            : fused2_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves,
                               (int)t->steps_per_pass));
   const TileShape ts = default_tile(c->fmt, c->g.ys, c->g.zs, *t);
@@ -1304,7 +1336,7 @@ bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning 
     const bool ok =
         spp >= 2 ? (!c->no_fuse &&
                     (c->fmt == FMT_F16
-                         ? spp == 2 && fused2h_supported((int)npts, (int)mj, (int)nw)
+                         ? fused2h_supported((int)npts, (int)mj, (int)nw, (int)spp)
                          : c->fmt == FMT_F32 &&
                                fused2_supported((int)npts, (int)mj, (int)nw, (int)spp)))
                  : shape_supported(c, (int)mj, (int)nw, (int)vec);
@@ -1372,21 +1404,22 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
       if (ch != 0 && p.chunks != ch) continue;
       cand.push_back(t);
     }
-  // FP16: the two-step pass's shapes
+  // FP16: the passes' shapes
   if (leapfrog && c->fmt == FMT_F16 && !c->no_fuse)
-    for (int mj : {3, 4})
-      for (int ch : {0, 4, 8, 16}) {
-        if (!fused2h_supported((int)c->g.npts, mj, 16)) continue;
-        sth_tuning t = saved;
-        t.rows_per_wave = (uint32_t)mj;
-        t.waves = 16;
-        t.vec = 2;
-        t.chunks = (uint32_t)ch;
-        t.steps_per_pass = 2;
-        const Plan2 p = make_plan2(c, t);
-        if (ch != 0 && p.chunks != ch) continue;
-        cand.push_back(t);
-      }
+    for (int sp = 2; sp <= 5; sp++)
+      for (int mj : {2, 3, 4})
+        for (int ch : {0, 4, 8, 16}) {
+          if (!fused2h_supported((int)c->g.npts, mj, 16, sp)) continue;
+          sth_tuning t = saved;
+          t.rows_per_wave = (uint32_t)mj;
+          t.waves = 16;
+          t.vec = 2;
+          t.chunks = (uint32_t)ch;
+          t.steps_per_pass = (uint32_t)sp;
+          const Plan2 p = make_plan2(c, t);
+          if (ch != 0 && p.chunks != ch) continue;
+          cand.push_back(t);
+        }
   if (leapfrog && c->fmt == FMT_F32 && !c->no_fuse)
     for (const auto &fs : fused_shapes) {
       if (!fused2_supported((int)c->g.npts, fs[0], fs[1], fs[2])) continue;

@@ -42,31 +42,34 @@ hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hip
 }
 
 // FP16 storage, two steps per pass: (stencil, rows per wave, waves)
-#define STH_FUSED2H_LIST(X) X(7, 4, 16) X(13, 4, 16) X(13, 3, 16) X(25, 3, 16)
+// (stencil, rows per wave, waves, steps per pass) of the FP16 passes
+#define STH_FUSED2H_LIST(X)                                                           \
+  X(7, 4, 16, 2) X(13, 4, 16, 2) X(13, 3, 16, 2) X(25, 3, 16, 2) X(7, 3, 16, 3)       \
+  X(13, 3, 16, 3) X(7, 2, 16, 4) X(7, 2, 16, 5)
 
-bool fused2h_supported(int npts, int mj, int nw) {
-#define STH_F2H_SUP(N, MJ, NW) \
-  if (npts == N && mj == MJ && nw == NW) return true;
+bool fused2h_supported(int npts, int mj, int nw, int steps) {
+#define STH_F2H_SUP(N, MJ, NW, S) \
+  if (npts == N && mj == MJ && nw == NW && steps == S) return true;
   STH_FUSED2H_LIST(STH_F2H_SUP)
 #undef STH_F2H_SUP
   return false;
 }
 
-Fused2Geom fused2h_geom(int npts, int mj, int nw) {
-#define STH_F2H_GEOM(N, MJ, NW)            \
-  if (npts == N && mj == MJ && nw == NW)   \
-    return {Fused2hShape<N, MJ, NW>::OJ, Fused2hShape<N, MJ, NW>::OK, 64 * NW};
+Fused2Geom fused2h_geom(int npts, int mj, int nw, int steps) {
+#define STH_F2H_GEOM(N, MJ, NW, S)                        \
+  if (npts == N && mj == MJ && nw == NW && steps == S)    \
+    return {Fused2hShape<N, MJ, NW, S>::OJ, Fused2hShape<N, MJ, NW, S>::OK, 64 * NW};
   STH_FUSED2H_LIST(STH_F2H_GEOM)
 #undef STH_F2H_GEOM
   return {0, 0, 0};
 }
 
-hipError_t launch_sweep2h(int npts, int mj, int nw, unsigned grid, hipStream_t st,
+hipError_t launch_sweep2h(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                           const Sweep2Args &a) {
-#define STH_F2H_LAUNCH(N, MJ, NW)                                                     \
-  if (npts == N && mj == MJ && nw == NW) {                                            \
-    hipLaunchKernelGGL((sweep2h_kernel<N, MJ, NW>), dim3(grid), dim3(64 * NW), 0, st, a); \
-    return hipGetLastError();                                                         \
+#define STH_F2H_LAUNCH(N, MJ, NW, S)                                                     \
+  if (npts == N && mj == MJ && nw == NW && steps == S) {                                 \
+    hipLaunchKernelGGL((sweep2h_kernel<N, MJ, NW, S>), dim3(grid), dim3(64 * NW), 0, st, a); \
+    return hipGetLastError();                                                            \
   }
   STH_FUSED2H_LIST(STH_F2H_LAUNCH)
 #undef STH_F2H_LAUNCH

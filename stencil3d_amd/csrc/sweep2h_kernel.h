@@ -1,29 +1,31 @@
-// sweep2h_kernel.h -- two LEAPFROG steps per pass over FP16 storage.
+// sweep2h_kernel.h -- two to five LEAPFROG steps per pass over FP16 storage.
 //
-// The FP32 two-step pass (sweep2_kernel.h) with the data kept in its stored
-// binary16 form, as in the packed one-step sweep (sweep16_kernel.h):
+// The FP32 multi-step pass (sweep2_kernel.h) with the data kept in its
+// stored binary16 form, as in the packed one-step sweep (sweep16_kernel.h):
 //
 //  * a lane holds two consecutive k points as one 32-bit word in every
 //    queue and LDS plane, so the 64 lanes of a row cover 128 columns (E1:
 //    EJ rows x 128 columns); the output tile is 96 columns (192 B, three
 //    whole 64-byte segments no other tile writes) centred with KO = 16
-//    columns on each side, enough for step one's columns widened by R and
-//    their own R-wide k-halo, so the pass loads no separate k-halo;
+//    columns on each side, enough for step one's columns widened by
+//    (S-1)R and their own R-wide k-halo (S R <= 16), so the pass loads no
+//    separate k-halo;
 //  * neighbour pairs are one v_fma_mix_f32 each, the centre, v and s terms
 //    take their halves inside the fused multiply-adds, and results round to
 //    binary16 in pairs (v_cvt_pk_f16_f32);
-//  * step two reads u_{n+1} as stored: step one's results are rounded to
+//  * step M+1 reads u_{n+M} as stored: each step's results are rounded to
 //    binary16 (with their output exponent) before they enter the queue and
-//    the LDS plane, so the pass is the one-step scheme's two steps;
-//  * the output exponents: u_{n+1}'s from the rigorous bound
-//    B1 = (|c0| + |c1| gain) M_u + M_v + |c2a| M_s of the maxima M of the
-//    stored inputs (as the one-step sweep); u_{n+2}'s from the same bound
-//    with B1 in place of u_{n+1}'s maximum, which the pass does not know
-//    before it ends: B2 = (|c0| + |c1| gain) B1 + M_u + |c2b| M_s.  B1 bounds
-//    every stored value of u_{n+1}, so no store overflows;
-//  * the maxima of both stored outputs are reduced in the pass (as unsigned
-//    maxima of the stored bits without the sign, which order like the
-//    magnitudes) and published for the next pass.
+//    the LDS plane, so a pass is S of the one-step scheme's steps;
+//  * the output exponents: step M's from the bound
+//      B_M = (|c0| + |c1| gain) B_{M-1} + B_{M-2} + |c2_M| M_s,
+//      B_0 = M_u, B_{-1} = M_v
+//    of the maxima M of the stored inputs: B_1 is the one-step sweep's
+//    bound, and each later bound stands in for the maximum of an output
+//    the pass does not know before it ends.  B_M bounds every stored value
+//    of u_{n+M}, so no store overflows;
+//  * the maxima of the two stored outputs are reduced in the pass (as
+//    unsigned maxima of the stored bits without the sign, which order like
+//    the magnitudes) and published for the next pass.
 #pragma once
 
 #include <utility>
@@ -33,42 +35,52 @@
 
 namespace sth {
 
-template <int NPTS, int MJ, int NW>
+// S steps per pass: step one runs on the output tile widened by WD = (S-1)R
+// on every side (E1: EJ rows x 128 columns).
+template <int NPTS, int MJ, int NW, int S = 2>
 struct Fused2hShape {
   static constexpr int R = Terms<NPTS>::R;
+  static constexpr int WD = (S - 1) * R;
   static constexpr int EJ = NW * MJ;       // E1 rows
-  static constexpr int OJ = EJ - 2 * R;    // output rows
+  static constexpr int OJ = EJ - 2 * WD;   // output rows
   static constexpr int OK = 96, KO = 16;   // output columns; lanes' columns on each side
   static constexpr int KH = (R + 1) / 2;   // padding words on each side of an LDS row
   static constexpr int LW = 64 + 2 * KH;   // words per LDS row
   static constexpr int LU = (EJ + 2 * R) * LW;  // u plane with the j-halo rows
-  static constexpr int LP = EJ * LW;            // u_{n+1} plane
+  static constexpr int LP = EJ * LW;            // one plane of u_{n+j}
   static constexpr int LS = EJ * 64;            // one slot of the s ring
-  static constexpr int SR = R + 1;              // s ring slots
-  static constexpr int LDS_WORDS = 2 * LU + 2 * LP + SR * LS;
+  static constexpr int SR = (S - 1) * (R + 1);  // s ring slots
+  static constexpr int LDS_WORDS = 2 * LU + 2 * (S - 1) * LP + SR * LS;
 };
 
-// The coefficients of both steps on the stored numbers (e1, e2: the output
-// exponents of the two steps).
+// The coefficients of the pass's steps on the stored numbers: step M
+// (index M-1) computes u_{n+M} with exponent e_M from u_{n+M-1} (exponent
+// e_{M-1}) and u_{n+M-2} (e_{M-2}); e_0 = e_u, e_{-1} = e_v:
+//   k0 = c0 2^(e_{M-1}-e_M), k1 = c1 2^(e_{M-1}-e_M), k2 = c2_M 2^(e_s-e_M),
+//   kv = 2^(e_{M-2}-e_M).
+template <int S>
 struct Coef2h {
-  float k0a, k1a, k2a, kva;  // step one: c0 2^(eu-e1), c1 2^(eu-e1), c2a 2^(es-e1), 2^(ev-e1)
-  float k0b, k1b, k2b, kvb;  // step two: c0 2^(e1-e2), c1 2^(e1-e2), c2b 2^(es-e2), 2^(eu-e2)
+  float k0[S], k1[S], k2[S], kv[S];
 };
 
-// The output exponents of the pass's two steps (see the file comment).
+// The output exponents e_1 .. e_S of a pass's steps (see the file comment).
+template <int S>
 __host__ __device__ inline void q16_pass_exps(const Q16Slot &cur, int ubuf, int vbuf, float c0,
-                                              float c1, float c2a, float c2b, double gain,
-                                              int *e1, int *e2) {
+                                              float c1, const float *c2, double gain, int *e) {
   auto M = [&](int b) {
     float m;
     memcpy(&m, &cur.maxbits[b], 4);
     return ldexp((double)m, cur.exp[b]);
   };
   const double gmul = fabs((double)c0) + fabs((double)c1) * gain;
-  const double B1 = gmul * M(ubuf) + M(vbuf) + fabs((double)c2a) * M(2);
-  const double B2 = gmul * B1 + M(ubuf) + fabs((double)c2b) * M(2);
-  *e1 = scale_exp(B1);
-  *e2 = scale_exp(B2);
+  const double Ms = M(2);
+  double b2 = M(vbuf), b1 = M(ubuf);  // B_{M-2}, B_{M-1}
+  for (int s = 0; s < S; s++) {
+    const double B = gmul * b1 + b2 + fabs((double)c2[s]) * Ms;
+    e[s] = scale_exp(B);
+    b2 = b1;
+    b1 = B;
+  }
 }
 
 // Packed unsigned maximum of two binary16 magnitudes per word.
@@ -114,24 +126,58 @@ __device__ __forceinline__ void lap_pair(const float *w, IW &&iw, JW &&jw, KWf &
   });
 }
 
+// One step's update of a lane's word: both halves in the one-step sweep's
+// order, each result rounded to FP32 and then (in pairs) to binary16.
+//   out = k2 s + (k1 L + (k0 c - kv v))
+__device__ __forceinline__ uint32_t update_pair(float k0, float k1, float k2, float kv,
+                                                uint32_t c, uint32_t v, uint32_t s, float L0,
+                                                float L1) {
+  float o[2];
+  static_for<2>([&](auto cc) __attribute__((always_inline)) {
+    constexpr int h = decltype(cc)::value;
+    const float t0 = __builtin_fmaf(k0, hwide<h>(c), hnegmul<h>(v, kv));
+    float x = __builtin_fmaf(k2, hwide<h>(s), __builtin_fmaf(k1, h ? L1 : L0, t0));
+    // keep the last fma apart from the conversion below (two roundings,
+    // FP32 then binary16, as the oracle rounds)
+    __asm__ volatile("" : "+v"(x));
+    o[h] = x;
+  });
+  using H2 = _Float16 __attribute__((ext_vector_type(2)));
+  H2 r;
+  r.x = (_Float16)o[0];
+  r.y = (_Float16)o[1];
+  return __builtin_bit_cast(uint32_t, r);
+}
+
 // One work item: output tile (tk, tj), i-chunk ch.  red: 2 NW floats.
-template <int NPTS, int MJ, int NW>
-__device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &K, const int tk,
-                                             const int tj, const int ch, uint32_t *SM,
-                                             float *red) {
+//
+// Step one runs at plane p on E1 from the u queue; step M (2 <= M <= S) at
+// plane p-(M-1)(R+1) on E1 less (M-1)R rows on each side, from the register
+// queue of step M-1's values (its i neighbours and in-wave j neighbours) and
+// an LDS plane of them (the other j and k neighbours), into a queue of its
+// own; the last step writes u_{n+S-1} and u_{n+S} of the output tile.
+template <int NPTS, int MJ, int NW, int S>
+__device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S> &K,
+                                             const int tk, const int tj, const int ch,
+                                             uint32_t *SM, float *red) {
   using Tm = Terms<NPTS>;
-  using F = Fused2hShape<NPTS, MJ, NW>;
-  constexpr int R = Tm::R, T = Tm::T;
+  using F = Fused2hShape<NPTS, MJ, NW, S>;
+  constexpr int R = Tm::R;
   constexpr int EJ = F::EJ, OJ = F::OJ, OK = F::OK, KO = F::KO, KH = F::KH, LW = F::LW,
-                SR = F::SR;
+                SR = F::SR, WD = F::WD;
   constexpr int HJ = (2 * R + NW - 1) / NW;  // u halo rows per wave
   constexpr int Q = 2 * R + 2;               // u queue: planes p-R-1 .. p+R
-  constexpr int GL = 2 * R + 2;              // u_{n+1} queue: planes p-2R-1 .. p
-  constexpr int GC = R;                      // its entry at step two's plane p-R-1
-  constexpr int U = 2 * (R + 1);             // steps per loop iteration
-  static_assert(KO >= 2 * R, "step one and its k-halo inside the lanes' columns");
+  // Queue j (1 <= j < S) holds the lane's u_{n+j} words: glen(j) entries
+  // (2R+2 for the last one, one more for the others: step j+2 takes its
+  // -u_{n+j} term from the oldest entry).  GL: the array length.
+  constexpr int GL = 2 * R + 3;
+  // Steps per loop iteration: a multiple of the s ring and of 2 (the LDS
+  // buffers alternate).
+  constexpr int U = (S - 1) % 2 == 0 ? (S - 1) * (R + 1) : 2 * (S - 1) * (R + 1);
+  static_assert(S >= 2 && S <= 5, "two to five steps per pass");
+  static_assert(KO >= WD + R, "step one and its k-halo inside the lanes' columns");
   static_assert(KO % 2 == 0 && OK % 2 == 0, "a lane's pair is all output or none");
-  static_assert(F::LDS_WORDS * 4 <= 160 * 1024, "LDS");
+  static_assert(F::LDS_WORDS * 4 + 2 * NW * 4 <= 160 * 1024, "LDS");
   static_assert(U % SR == 0 && U % 2 == 0, "ring slots and buffers repeat with the loop");
 
   const int ib = ch * a.chunk_len;
@@ -140,15 +186,16 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
 
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int je = tj * OJ - R, ke = tk * OK - KO;  // E1's first row; the lanes' first column
+  const int je = tj * OJ - WD, ke = tk * OK - KO;  // E1's first row; the lanes' first column
   const int64_t P = a.plane, Pj = a.pitch;
 
   // The lane's pair starts at column kg (even).  Reads clamp to the pairs
   // that cover the allocated box [-R, zs+R): from the even column at or
   // below -R to the even column at or below zs+R-1 (its second element may
   // lie one past the box, inside the row padding or the next row's halo).
-  // E1 points outside the interior take v's value (not a stencil value), so
-  // what their clamped neighbours hold never reaches an output.
+  // E1 points outside the interior take the older field's value (not a
+  // stencil value), so what their clamped neighbours hold never reaches an
+  // output.
   const int kg = ke + 2 * lane;
   const int klo = -2 * KH, khi = (a.zs + R - 1) & ~1;
   const int kc = min(max(kg, klo), khi);
@@ -189,13 +236,19 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
   const uint32_t outabs = klane ? (kinmask & 0x7fff7fffu) : 0u;  // magnitudes of output halves
   const bool kout0 = klane && kin0, kout1 = klane && kin1;
   uint32_t off[MJ];
-  int rowin[MJ], rowout[MJ];  // wave-uniform
+  // wave-uniform: the row is inside the interior / an output row / one of
+  // step M's rows (2 <= M < S: E1 less (M-1)R on each side)
+  int rowin[MJ], rowout[MJ], rowst[S][MJ];
 #pragma unroll
   for (int m = 0; m < MJ; m++) {
     const int er = wv * MJ + m, jg = je + er;
     off[m] = boff(jg);
     rowin[m] = __builtin_amdgcn_readfirstlane((int)(jg >= 0 && jg < a.ys));
-    rowout[m] = __builtin_amdgcn_readfirstlane((int)(er >= R && er < R + OJ));
+    rowout[m] = __builtin_amdgcn_readfirstlane((int)(er >= WD && er < WD + OJ));
+#pragma unroll
+    for (int M = 0; M < S; M++)
+      rowst[M][m] =
+          __builtin_amdgcn_readfirstlane((int)(er >= (M - 1) * R && er < EJ - (M - 1) * R));
   }
   uint32_t hoff[HJ];
   int hrow[HJ];
@@ -210,14 +263,16 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
   // LDS: per-lane byte bases plus compile-time offsets (each below 64 KiB),
   // passed through an empty asm statement at every step.
   //   lbL: u plane, row wv*MJ, word lane (offsets: buffer, row m+R+dr, word KH+dp);
-  //   lbW: u_{n+1} plane, row wv*MJ-R, word lane (offsets: buffer, row m+R+dr, word KH+dp);
+  //   lbW: u_{n+1} plane, row wv*MJ-R, word lane (offsets: plane 2(j-1) +
+  //        buffer, row m+R+dr, word KH+dp);
   //   lbS: s ring slot 0, row wv*MJ, word lane (offsets: slot, row m).
   using LW32 = __attribute__((address_space(3))) uint32_t;
   constexpr uint32_t LUB = F::LU * 4, LPB = F::LP * 4, LSB = F::LS * 4;
-  static_assert(2 * LUB <= 65536 && 2 * LPB <= 65536 && (SR - 1) * LSB + MJ * 256 <= 65536,
+  static_assert(2 * LUB <= 65536 && (2 * S - 3) * LPB + (MJ + 2 * R) * LW * 4 <= 65536 &&
+                    (SR - 1) * LSB + MJ * 256 <= 65536,
                 "offsets");
   const uint32_t lu0 = (uint32_t)(uintptr_t)(LW32 *)&SM[0];
-  const uint32_t wbase = lu0 + 2 * LUB, sbase = wbase + 2 * LPB;
+  const uint32_t wbase = lu0 + 2 * LUB, sbase = wbase + 2 * (S - 1) * LPB;
   uint32_t lbL = lu0 + (uint32_t)((wv * MJ * LW + lane) * 4);
   uint32_t lbW = wbase + (uint32_t)(((wv * MJ - R) * LW + lane) * 4);
   uint32_t lbS = sbase + (uint32_t)((wv * MJ * 64 + lane) * 4);
@@ -227,14 +282,17 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
   auto lds = [](uint32_t adr) -> LW32 & { return *(LW32 *)(uintptr_t)adr; };
 
   // Register queues of packed words:
-  //   q[m][t] = u_n(p - R - 1 + t), t = 0 .. 2R+1;
-  //   G[m][t] = u_{n+1}(p - 2R - 1 + t), t = 0 .. 2R (t = 2R+1 takes step
-  //             one's result at p before the shift); step two reads G[GC -+ r].
-  uint32_t q[MJ][Q], G[MJ][GL];
+  //   q[m][t]    = u_n(p - R - 1 + t), t = 0 .. 2R+1;
+  //   G[j][m][t] = u_{n+j}(p_j - L_j + 1 + t), t = 0 .. L_j-2, with p_j =
+  //                p-(j-1)(R+1) the plane of step j and L_j = glen(j) (t =
+  //                L_j-1 takes step j's result at p_j before the shift).  Step
+  //                j+1 reads entries gcen(j) -+ r (its plane p_{j+1} and i
+  //                neighbours), step j+2 entry 0 (u_{n+j} at p_{j+2}).
+  uint32_t q[MJ][Q], G[S][MJ][GL];
   uint32_t hq[HJ][R + 1];
-  // Step one from p0 (its planes past ie-1+R are never used), step two
-  // (output plane p - R - 1) up to plast.
-  const int p0 = ib - R, plast = ie + R;
+  // Step one from p0 (its planes past ie-1+WD are never used), the last
+  // step (output plane p-(S-1)(R+1)) up to plast.
+  const int p0 = ib - WD, plast = ie + (S - 1) * (R + 1) - 1;
 #pragma unroll
   for (int t = 1; t <= 2 * R; t++) {
     GB *Up = pbase(a.u, p0 - R - 1 + t);
@@ -245,7 +303,9 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
   for (int m = 0; m < MJ; m++) {
     q[m][0] = 0u;
 #pragma unroll
-    for (int t = 0; t < GL; t++) G[m][t] = 0u;
+    for (int j = 1; j < S; j++)
+#pragma unroll
+      for (int t = 0; t < glen<S, R>(j); t++) G[j][m][t] = 0u;
   }
 #pragma unroll
   for (int t = 0; t < R; t++) {
@@ -273,9 +333,11 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
   prefetch(pbase(a.u, p0 + R), pbase(a.v, p0), pbase(a.s, p0), SA);
   prefetch(pbase(a.u, p0 + 1 + R), pbase(a.v, p0 + 1), pbase(a.s, p0 + 1), SB);
 
-  // Plane pointers advanced by one plane per step, each held at the last
-  // allocated plane: the prefetch of step p reads u(p+2+R), v(p+2), s(p+2);
-  // step two stores plane p-R-1, from plane ib on.
+  // Plane pointers advanced by one plane per step, each held inside
+  // [-R, xs+R): the prefetch of step p reads u(p+2+R), v(p+2), s(p+2); the
+  // last step stores plane p-(S-1)(R+1), from plane ib on.  The first step
+  // of a chunk is WD planes below it, so the lower bound only binds for
+  // WD > R+2 (v, s) and WD > 2R+2 (u); elsewhere the test folds away.
   const int64_t PB = P * 2;
   const int last = a.xs + R - 1;
   GB *Un = pbase(a.u, p0 + 2 + R);
@@ -287,7 +349,7 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
 
   auto step = [&](int p, Staged &n, auto kcst) __attribute__((always_inline)) {
     constexpr int k = decltype(kcst)::value;
-    uint32_t vv[MJ], s1[MJ], sc[MJ];
+    uint32_t vv[MJ], s1[MJ], sc[S + 1][MJ];
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       q[m][Q - 1] = n.u[m];
@@ -297,78 +359,94 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
 #pragma unroll
     for (int h = 0; h < HJ; h++) hq[h][R] = n.hj[h];
     prefetch(Un, Vn, Sn, n);
-    Un = p + 3 + R <= last ? Un + PB : Un;
-    Vn = p + 3 <= last ? Vn + PB : Vn;
-    Sn = p + 3 <= last ? Sn + PB : Sn;
+    constexpr bool low_v = WD > R + 2, low_u = WD > 2 * R + 2;
+    Un = ((!low_u || p + 2 + R >= -R) && p + 3 + R <= last) ? Un + PB : Un;
+    Vn = ((!low_v || p + 2 >= -R) && p + 3 <= last) ? Vn + PB : Vn;
+    Sn = ((!low_v || p + 2 >= -R) && p + 3 <= last) ? Sn + PB : Sn;
 
     asm volatile("" : "+v"(lbL), "+v"(lbW), "+v"(lbS));
-    // s ring of R+1 slots (the lane's own words): step two's s (plane
-    // p-R-1, stored R+1 steps ago) is read from slot k, then s(p) replaces it.
+    // The s ring of (S-1)(R+1) slots (the lane's own words, so no barrier is
+    // involved): step M of plane p-(M-1)(R+1) finds its s in slot
+    // k + (S-M)(R+1); the last step's s and s(p) share slot k, the old value
+    // read before the new one is stored.
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
-      sc[m] = lds(lbS + (uint32_t)((k % SR) * LSB) + m * 256);
+#pragma unroll
+      for (int M = S; M >= 2; M--)
+        sc[M][m] = lds(lbS + (uint32_t)(((k + (S - M) * (R + 1)) % SR) * LSB) + m * 256);
       lds(lbS + (uint32_t)((k % SR) * LSB) + m * 256) = s1[m];
     }
-    constexpr uint32_t LO = (k & 1) * LUB, PO = (k & 1) * LPB;
+    constexpr uint32_t LO = (k & 1) * LUB;
     auto Lat = [&](int m, int dr, int dp) -> LW32 & {
       return lds(lbL + LO + (uint32_t)(((m + R + dr) * LW + KH + dp) * 4));
     };
-    auto Wat = [&](int m, int dr, int dp) -> LW32 & {
-      return lds(lbW + PO + (uint32_t)(((m + R + dr) * LW + KH + dp) * 4));
+    // this step's buffer of u_{n+j}
+    auto Wat = [&](int j, int m, int dr, int dp) -> LW32 & {
+      return lds(lbW + (uint32_t)((2 * (j - 1) + (k & 1)) * LPB) +
+                 (uint32_t)(((m + R + dr) * LW + KH + dp) * 4));
     };
-    // u(p) into LDS (the lanes' rows and the j-halo rows); u_{n+1} at step
-    // two's plane p-R-1.
+    // u(p) into LDS (the lanes' rows and the j-halo rows); u_{n+j} at the
+    // plane of step j+1 (p-j(R+1)), j = 1 .. S-1.
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       Lat(m, 0, 0) = q[m][R + 1];
-      Wat(m, 0, 0) = G[m][GC];
+#pragma unroll
+      for (int j = 1; j < S; j++) Wat(j, m, 0, 0) = G[j][m][gcen<S, R>(j)];
     }
 #pragma unroll
     for (int h = 0; h < HJ; h++)
       if (hrow[h] >= 0) lds(hadr[h] + LO) = hq[h][0];
     __syncthreads();
 
-    // Step two writes plane io = p-R-1 of the output tile: u_{n+1} as stored
-    // and u_{n+2}.  Rows without output points are skipped (uniform).
-    const int io = p - R - 1;
+    // u_{n+M-2} at step M's plane: u_n from the u queue, else the oldest
+    // entry of queue M-2.
+    auto older = [&](auto jc, int m) __attribute__((always_inline)) -> uint32_t {
+      constexpr int J = decltype(jc)::value;
+      if constexpr (J == 0)
+        return q[m][0];
+      else
+        return G[J][m][0];
+    };
+    // The Laplacian of u_{n+J} at row m, both columns: queue J (i neighbours
+    // and in-wave j neighbours) and its LDS plane (the others).
+    auto lapq = [&](auto jc, int m, float &L0, float &L1) __attribute__((always_inline)) {
+      constexpr int J = decltype(jc)::value, C = gcen<S, R>(J);
+      using W2 = uint32_t __attribute__((ext_vector_type(2)));
+      const uint32_t cw = G[J][m][C];
+      lap_pair<NPTS, KH>(
+          a.w,
+          [&](auto rc) -> W2 {
+            constexpr int r = decltype(rc)::value;
+            return W2{G[J][m][C - r], G[J][m][C + r]};
+          },
+          [&](auto rc) -> W2 {
+            constexpr int r = decltype(rc)::value;
+            return W2{m - r >= 0 ? G[J][m - r >= 0 ? m - r : 0][C] : (uint32_t)Wat(J, m, -r, 0),
+                      m + r < MJ ? G[J][m + r < MJ ? m + r : 0][C] : (uint32_t)Wat(J, m, r, 0)};
+          },
+          [&](auto dc) -> uint32_t {
+            constexpr int d = decltype(dc)::value;
+            return d == 0 ? cw : (uint32_t)Wat(J, m, 0, d);
+          },
+          L0, L1);
+    };
+
+    // The last step writes plane io = p-(S-1)(R+1) of the output tile: the
+    // two newest fields there.  Rows without output points are skipped
+    // (uniform).
+    const int io = p - (S - 1) * (R + 1);
     const bool store = io >= ib && io < ie;
     const BufRes Wr = buf_res(Wo), Zr = buf_res(Zo);
 #pragma unroll
     for (int m = 0; m < MJ; m++) {
       if (!rowout[m]) continue;
-      const uint32_t cw = G[m][GC];
-      using W2 = uint32_t __attribute__((ext_vector_type(2)));
+      constexpr int J = S - 1;
+      const uint32_t cw = G[J][m][gcen<S, R>(J)];
       float L0, L1;
-      lap_pair<NPTS, KH>(
-          a.w,
-          [&](auto rc) -> W2 {
-            constexpr int r = decltype(rc)::value;
-            return W2{G[m][GC - r], G[m][GC + r]};
-          },
-          [&](auto rc) -> W2 {
-            constexpr int r = decltype(rc)::value;
-            return W2{m - r >= 0 ? G[m - r >= 0 ? m - r : 0][GC] : (uint32_t)Wat(m, -r, 0),
-                      m + r < MJ ? G[m + r < MJ ? m + r : 0][GC] : (uint32_t)Wat(m, r, 0)};
-          },
-          [&](auto dc) -> uint32_t {
-            constexpr int d = decltype(dc)::value;
-            return d == 0 ? cw : (uint32_t)Wat(m, 0, d);
-          },
-          L0, L1);
-      // -u_n at io: the oldest entry of the u queue
-      float z[2];
-      static_for<2>([&](auto cc) __attribute__((always_inline)) {
-        constexpr int c = decltype(cc)::value;
-        const float t0 = __builtin_fmaf(K.k0b, hwide<c>(cw), hnegmul<c>(q[m][0], K.kvb));
-        float o = __builtin_fmaf(K.k2b, hwide<c>(sc[m]), __builtin_fmaf(K.k1b, c ? L1 : L0, t0));
-        __asm__ volatile("" : "+v"(o));
-        z[c] = o;
-      });
-      using H2 = _Float16 __attribute__((ext_vector_type(2)));
-      H2 zh;
-      zh.x = (_Float16)z[0];
-      zh.y = (_Float16)z[1];
-      const uint32_t zw = __builtin_bit_cast(uint32_t, zh);
+      lapq(std::integral_constant<int, J>{}, m, L0, L1);
+      const uint32_t zw = update_pair(K.k0[S - 1], K.k1[S - 1], K.k2[S - 1], K.kv[S - 1], cw,
+                                      older(std::integral_constant<int, J - 1>{}, m), sc[S][m],
+                                      L0, L1);
       if (store && rowin[m]) {
         if (kout0 && kout1) {
           buf_store<uint32_t, 2>(cw, Wr, off[m]);
@@ -385,6 +463,32 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
     if (store) {
       Wo += PB;
       Zo += PB;
+    }
+
+    // Steps S-1 .. 2, each into the newest entry of its queue; halves
+    // outside the interior take u_{n+M-2}'s value there.  (Only
+    // instantiated for S >= 3.)
+    if constexpr (S >= 3) {
+      static_for<S - 2>([&](auto ic) __attribute__((always_inline)) {
+        constexpr int M = S - 1 - decltype(ic)::value;  // S-1 .. 2
+        const int pm = p - (M - 1) * (R + 1);
+        const bool pinm = pm >= 0 && pm < a.xs;
+#pragma unroll
+        for (int m = 0; m < MJ; m++) {
+          const uint32_t old = older(std::integral_constant<int, M - 2>{}, m);
+          if (!rowst[M][m]) {
+            G[M][m][glen<S, R>(M) - 1] = old;
+            continue;
+          }
+          float L0, L1;
+          lapq(std::integral_constant<int, M - 1>{}, m, L0, L1);
+          const uint32_t zw =
+              update_pair(K.k0[M - 1], K.k1[M - 1], K.k2[M - 1], K.kv[M - 1],
+                          G[M - 1][m][gcen<S, R>(M - 1)], old, sc[M][m], L0, L1);
+          G[M][m][glen<S, R>(M) - 1] =
+              (rowin[m] && pinm) ? ((zw & kinmask) | (old & ~kinmask)) : old;
+        }
+      });
     }
 
     // Step one at plane p: u_{n+1} on E1, rounded to binary16, into the
@@ -411,21 +515,10 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
             return d == 0 ? cw : (uint32_t)Lat(m, 0, d);
           },
           L0, L1);
-      float w1[2];
-      static_for<2>([&](auto cc) __attribute__((always_inline)) {
-        constexpr int c = decltype(cc)::value;
-        const float t0 = __builtin_fmaf(K.k0a, hwide<c>(cw), hnegmul<c>(vv[m], K.kva));
-        float o = __builtin_fmaf(K.k2a, hwide<c>(s1[m]), __builtin_fmaf(K.k1a, c ? L1 : L0, t0));
-        __asm__ volatile("" : "+v"(o));
-        w1[c] = o;
-      });
-      using H2 = _Float16 __attribute__((ext_vector_type(2)));
-      H2 wh;
-      wh.x = (_Float16)w1[0];
-      wh.y = (_Float16)w1[1];
-      const uint32_t ww = __builtin_bit_cast(uint32_t, wh);
+      const uint32_t ww = update_pair(K.k0[0], K.k1[0], K.k2[0], K.kv[0], cw, vv[m], s1[m], L0, L1);
       // halves inside the interior take the new value, the others v's
-      G[m][GL - 1] = (rowin[m] && pin) ? ((ww & kinmask) | (vv[m] & ~kinmask)) : vv[m];
+      G[1][m][glen<S, R>(1) - 1] =
+          (rowin[m] && pin) ? ((ww & kinmask) | (vv[m] & ~kinmask)) : vv[m];
     }
 
 #pragma unroll
@@ -433,7 +526,9 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
 #pragma unroll
       for (int t = 0; t < Q - 1; t++) q[m][t] = q[m][t + 1];
 #pragma unroll
-      for (int t = 0; t < GL - 1; t++) G[m][t] = G[m][t + 1];
+      for (int j = 1; j < S; j++)
+#pragma unroll
+        for (int t = 0; t < glen<S, R>(j) - 1; t++) G[j][m][t] = G[j][m][t + 1];
     }
 #pragma unroll
     for (int t = 0; t < R; t++) {
@@ -473,59 +568,60 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h &
       b1 = fmaxf(b1, red[t]);
       b2 = fmaxf(b2, red[NW + t]);
     }
+    // u_{n+S-1} (o1) and u_{n+S} (o2): the one-step scheme writes u_{n+j}
+    // into v's buffer for odd j, into u's for even j
     Q16Slot &nx = a.qs[(a.slot + 1) % 3];
-    atomicMax(&nx.maxbits[a.vbuf], __float_as_uint(b1));
-    atomicMax(&nx.maxbits[a.ubuf], __float_as_uint(b2));
+    atomicMax(&nx.maxbits[(S - 1) % 2 ? a.vbuf : a.ubuf], __float_as_uint(b1));
+    atomicMax(&nx.maxbits[S % 2 ? a.vbuf : a.ubuf], __float_as_uint(b2));
   }
 }
 
-template <int NPTS, int MJ, int NW>
+template <int NPTS, int MJ, int NW, int S>
 __global__ __launch_bounds__(64 * NW) void sweep2h_kernel(const Sweep2Args a) {
-  using F = Fused2hShape<NPTS, MJ, NW>;
+  using F = Fused2hShape<NPTS, MJ, NW, S>;
   __shared__ __attribute__((aligned(16))) uint32_t SM[F::LDS_WORDS];
   __shared__ float red[2 * NW];
   const Q16Slot &cur = a.qs[a.slot];
-  int e1, e2;
-  q16_pass_exps(cur, a.ubuf, a.vbuf, a.c0, a.c1, a.c2a, a.c2b, a.gain, &e1, &e2);
+  const float c2[5] = {a.c2a, a.c2b, a.c2c, a.c2d, a.c2e};
+  int e[S];
+  q16_pass_exps<S>(cur, a.ubuf, a.vbuf, a.c0, a.c1, c2, a.gain, e);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // The next slot: u_{n+1} lands in v's buffer, u_{n+2} in u's; their
+    // The next slot: u_{n+S-1} and u_{n+S} with their exponents; their
     // maxima are reduced by the blocks into it (cleared by the previous
     // launch); the slot after it is cleared for the next launch.
     Q16Slot &nxt = a.qs[(a.slot + 1) % 3];
     Q16Slot &clr = a.qs[(a.slot + 2) % 3];
-    nxt.exp[a.vbuf] = e1;
-    nxt.exp[a.ubuf] = e2;
+    nxt.exp[(S - 1) % 2 ? a.vbuf : a.ubuf] = e[S - 2];
+    nxt.exp[S % 2 ? a.vbuf : a.ubuf] = e[S - 1];
     nxt.exp[2] = cur.exp[2];
     nxt.maxbits[2] = cur.maxbits[2];
     clr.maxbits[0] = clr.maxbits[1] = clr.maxbits[2] = 0u;
   }
-  const int eu = cur.exp[a.ubuf], ev = cur.exp[a.vbuf], es = cur.exp[2];
-  Coef2h K;
-  K.k0a = ldexpf(a.c0, eu - e1);
-  K.k1a = ldexpf(a.c1, eu - e1);
-  K.k2a = ldexpf(a.c2a, es - e1);
-  K.kva = ldexpf(1.0f, ev - e1);
-  K.k0b = ldexpf(a.c0, e1 - e2);
-  K.k1b = ldexpf(a.c1, e1 - e2);
-  K.k2b = ldexpf(a.c2b, es - e2);
-  K.kvb = ldexpf(1.0f, eu - e2);
+  // step M's inputs: u_{n+M-1} (exponent ein[M]) and u_{n+M-2} (ein[M-1])
+  const int es = cur.exp[2];
+  int ein[S + 1];
+  ein[0] = cur.exp[a.vbuf];
+  ein[1] = cur.exp[a.ubuf];
+#pragma unroll
+  for (int M = 2; M <= S; M++) ein[M] = e[M - 2];
   // the coefficients are the same in every lane: keep them in scalar registers
   auto uf = [](float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
   };
-  K.k0a = uf(K.k0a);
-  K.k1a = uf(K.k1a);
-  K.k2a = uf(K.k2a);
-  K.kva = uf(K.kva);
-  K.k0b = uf(K.k0b);
-  K.k1b = uf(K.k1b);
-  K.k2b = uf(K.k2b);
-  K.kvb = uf(K.kvb);
+  Coef2h<S> K;
+#pragma unroll
+  for (int M = 1; M <= S; M++) {
+    const int eo = e[M - 1];
+    K.k0[M - 1] = uf(ldexpf(a.c0, ein[M] - eo));
+    K.k1[M - 1] = uf(ldexpf(a.c1, ein[M] - eo));
+    K.k2[M - 1] = uf(ldexpf(c2[M - 1], es - eo));
+    K.kv[M - 1] = uf(ldexpf(1.0f, ein[M - 1] - eo));
+  }
   int bid = blockIdx.x;
   if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
   const int tk = bid % a.ntk;
   const int tt = bid / a.ntk;
-  sweep2h_tile<NPTS, MJ, NW>(a, K, tk, tt % a.ntj, tt / a.ntj, SM, red);
+  sweep2h_tile<NPTS, MJ, NW, S>(a, K, tk, tt % a.ntj, tt / a.ntj, SM, red);
 }
 
 }  // namespace sth

@@ -349,16 +349,16 @@ def test_five_step_passes_equal_one_step_sweeps_non_cubic(gpu, orc):
 
 @pytest.mark.parametrize("spp", [4, 5])
 def test_four_and_five_steps_only_for_7_points(gpu, spp):
-    for npts in (13, 25):
-        with gpu.Stencil(16, 16, 64, npts=npts) as ctx:
-            with pytest.raises(gpu.SthError):
-                ctx.set_tuning(steps_per_pass=spp)
-    with gpu.Stencil(16, 16, 64, npts=7, dtype="fp16") as ctx:
-        with pytest.raises(gpu.SthError):
+    for dtype in ("fp32", "fp16"):
+        for npts in (13, 25):
+            with gpu.Stencil(16, 16, 64, npts=npts, dtype=dtype) as ctx:
+                with pytest.raises(gpu.SthError):
+                    ctx.set_tuning(steps_per_pass=spp)
+        with gpu.Stencil(16, 16, 64, npts=7, dtype=dtype) as ctx:
             ctx.set_tuning(steps_per_pass=spp)
-    with gpu.Stencil(16, 16, 64, npts=7) as ctx:
-        with pytest.raises(gpu.SthError):
-            ctx.set_tuning(steps_per_pass=6)
+            assert ctx.get_tuning()["steps_per_pass"] == spp
+            with pytest.raises(gpu.SthError):
+                ctx.set_tuning(steps_per_pass=6)
 
 
 def test_three_steps_rejected_for_25_points_and_16bit(gpu):
@@ -371,24 +371,28 @@ def test_three_steps_rejected_for_25_points_and_16bit(gpu):
 
 
 def test_16bit_pass_lengths(gpu):
-    """FP16 storage has the two-step pass and no longer one; BF16 has none."""
+    """FP16 storage has passes of two steps for every stencil, three for 7
+    and 13 points, four and five for 7 points; BF16 has none."""
     with gpu.Stencil(16, 16, 64, npts=25, dtype="bf16") as ctx:
         with pytest.raises(gpu.SthError):
             ctx.set_tuning(steps_per_pass=2)
-    with gpu.Stencil(16, 16, 64, npts=13, dtype="fp16") as ctx:
-        with pytest.raises(gpu.SthError):
-            ctx.set_tuning(steps_per_pass=3)
-        ctx.set_tuning(steps_per_pass=2)
-        assert ctx.get_tuning()["steps_per_pass"] == 2
+    for npts, longest in ((25, 2), (13, 3), (7, 5)):
+        with gpu.Stencil(16, 16, 64, npts=npts, dtype="fp16") as ctx:
+            for spp in range(2, longest + 1):
+                ctx.set_tuning(steps_per_pass=spp)
+                assert ctx.get_tuning()["steps_per_pass"] == spp
+            with pytest.raises(gpu.SthError):
+                ctx.set_tuning(steps_per_pass=longest + 1)
 
 
-@pytest.mark.parametrize("npts,shape,spp", [(7, (40, 130, 300), 2), (7, (40, 70, 300), 1),
+@pytest.mark.parametrize("npts,shape,spp", [(7, (40, 130, 300), 3), (7, (40, 70, 300), 1),
                                              (25, (40, 130, 300), 1)])
 def test_fp16_default_schedule(gpu, orc, npts, shape, spp):
-    """FP16 LEAPFROG takes the two-step pass by default for 7 points from 12
-    output tiles (and 13 points from 70), and stays one-step for 25 points;
-    five steps are then two passes and a step, bit-identical to the oracle's
-    emulation of the same schedule."""
+    """FP16 LEAPFROG takes the three-step pass by default for 7 points from
+    10 output tiles of its shape (four steps from 200; 13 points: two steps
+    from 70 tiles, three from 200), and stays one-step for 25 points; five
+    steps are then the passes and the steps left over, bit-identical to the
+    oracle's emulation of the same schedule."""
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     u, v, s = seeded(orc, g, 31, 0.0)  # the 16-bit form needs zero halos
@@ -411,7 +415,7 @@ def test_fp16_default_schedule(gpu, orc, npts, shape, spp):
         gu, eu, mu = ctx.download_raw16("u", R)
     a, b = qs[0].copy(), qs[1].copy()
     newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
-                              np.float32(c1), c2, 5, pass2=spp == 2)
+                              np.float32(c1), c2, 5, plen=spp)
     want = a if newest == 0 else b
     assert eu == exps[newest]
     assert np.array_equal(gu[g.interior], want[g.interior])
@@ -500,12 +504,13 @@ def test_more_tiles_than_cus(gpu, orc, npts, shape, steps):
         assert same_bits(gv, want_v), chunks
 
 
-@pytest.mark.parametrize("npts", [7, 13])
-def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts):
-    """FP16 LEAPFROG split over calls of 3, 2 and 1 steps with the two-step
-    pass (each call its own passes and remainder), then autotuned (the
-    tuner also times the FP16 pass and the taller packed tiles) and stepped
-    on: bit-identical to the oracle's emulation of the same schedule."""
+@pytest.mark.parametrize("npts,spp", [(7, 2), (7, 4), (13, 2), (13, 3)])
+def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts, spp):
+    """FP16 LEAPFROG split over calls of 3, 2 and 1 steps with passes of spp
+    steps (each call its own passes and the steps left over as one shorter
+    pass or a step), then autotuned (the tuner also times the FP16 passes
+    and the taller packed tiles) and stepped on: bit-identical to the
+    oracle's emulation of the same schedule."""
     shape = (24, 70, 200)
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
@@ -523,7 +528,7 @@ def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts):
     a, b = qs[0].copy(), qs[1].copy()
     bufs, cur = [a, b], 0
     with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
-        ctx.set_tuning(steps_per_pass=2)
+        ctx.set_tuning(steps_per_pass=spp)
         for f, q, ex in zip(("u", "v", "s"), qs, exps):
             ctx.upload_raw16(f, q, ex, R)
         done = 0
@@ -533,7 +538,7 @@ def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts):
             m3 = [maxs[cur], maxs[1 - cur], maxs[2]]
             n = orc.leapfrog_q16(npts, 1, g, bufs[cur], bufs[1 - cur], qs[2], e3, m3,
                                  np.float32(c0), np.float32(c1), c2[done:done + k], k,
-                                 pass2=True)
+                                 plen=spp)
             exps[cur], exps[1 - cur] = e3[0], e3[1]
             maxs[cur], maxs[1 - cur] = m3[0], m3[1]
             cur = cur if n == 0 else 1 - cur
@@ -544,7 +549,7 @@ def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts):
         # autotune (it steps the fields), then start again from the same fields
         t, ms = ctx.autotune(reps=1)
         assert ms > 0
-        spp = ctx.get_tuning()["steps_per_pass"]
+        tuned = ctx.get_tuning()["steps_per_pass"]
         for f, q, ex in zip(("u", "v", "s"), qs, (orc.scale_exp(float(np.abs(x).max()))
                                                  for x in (u, v, s))):
             ctx.upload_raw16(f, q, ex, R)
@@ -554,6 +559,6 @@ def test_fp16_pass_across_calls_and_autotune(gpu, orc, npts):
     m0 = [orc.quantize(1, g, x, e)[1] for x, e in zip((u, v, s), e0)]
     a, b = qs[0].copy(), qs[1].copy()
     n = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], e0, m0, np.float32(c0), np.float32(c1),
-                         c2[:5], 5, pass2=spp == 2)
+                         c2[:5], 5, plen=tuned)
     assert eu == e0[n]
     assert np.array_equal(gu[g.interior], (a if n == 0 else b)[g.interior])

@@ -281,17 +281,21 @@ def test_fp16_packed_folded_edge_bit_exact(gpu, orc, npts, tile, zs):
     assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
 
 
-FP16_PASS_SHAPES = {7: [(4, 16)], 13: [(3, 16), (4, 16)], 25: [(3, 16)]}
+# (stencil, rows per wave, waves, steps per pass): every compiled FP16 pass
+FP16_PASSES = [(7, 4, 16, 2), (13, 3, 16, 2), (13, 4, 16, 2), (25, 3, 16, 2), (7, 3, 16, 3),
+               (13, 3, 16, 3), (7, 2, 16, 4), (7, 2, 16, 5)]
 
 
-@pytest.mark.parametrize("npts,tile", [(n, t) for n, ts in FP16_PASS_SHAPES.items() for t in ts])
+@pytest.mark.parametrize("npts,mj,nw,sp", FP16_PASSES,
+                         ids=[f"{n}pt-{m}x{w}-s{s}" for n, m, w, s in FP16_PASSES])
 @pytest.mark.parametrize("shape,chunks", [((30, 61, 201), 2), ((44, 130, 97), 3),
                                           ((17, 12, 40), 1)])
-def test_fp16_two_step_pass_bit_exact(gpu, orc, npts, tile, shape, chunks):
-    """The FP16 two-step pass: three passes and one step left over, several
+def test_fp16_pass_bit_exact(gpu, orc, npts, mj, nw, sp, shape, chunks):
+    """The FP16 passes of two to five steps: seven steps (passes of sp and
+    the steps left over as one shorter pass or a single step), several
     i-chunks, output tiles cut by the grid's edges in j and k (an odd zs
     leaves a lane with one point inside), a source term; bits, exponents and
-    maxima equal to the oracle's binary16 emulation in passes of two steps."""
+    maxima equal to the oracle's binary16 emulation in the same passes."""
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     v, s, u = seeded(orc, g, seed=23 + npts)
@@ -306,8 +310,9 @@ def test_fp16_two_step_pass_bit_exact(gpu, orc, npts, tile, shape, chunks):
     c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
     c2 = np.linspace(0.5, -0.5, 7).astype(np.float32)
     with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
-        ctx.set_tuning(rows_per_wave=tile[0], waves=tile[1], chunks=chunks, steps_per_pass=2)
-        assert ctx.get_tuning()["steps_per_pass"] == 2
+        ctx.set_tuning(rows_per_wave=mj, waves=nw, chunks=chunks, steps_per_pass=sp)
+        t = ctx.get_tuning()
+        assert (t["steps_per_pass"], t["rows_per_wave"], t["waves"]) == (sp, mj, nw)
         ctx.upload_raw16("u", qs[0], exps[0], R)
         ctx.upload_raw16("v", qs[1], exps[1], R)
         ctx.upload_raw16("s", qs[2], exps[2], R)
@@ -316,7 +321,7 @@ def test_fp16_two_step_pass_bit_exact(gpu, orc, npts, tile, shape, chunks):
         gv, ev, mv = ctx.download_raw16("v", R)
     a, b = qs[0].copy(), qs[1].copy()
     newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
-                              np.float32(c1), c2, 7, pass2=True)
+                              np.float32(c1), c2, 7, plen=sp)
     (wu, weu, wmu), (wv, wev, wmv) = (
         ((a, exps[0], maxs[0]), (b, exps[1], maxs[1])) if newest == 0 else
         ((b, exps[1], maxs[1]), (a, exps[0], maxs[0])))

@@ -61,12 +61,12 @@ def test_place_fp64_and_rejections(gpu, orc):
 
 
 
-@pytest.mark.parametrize("npts,shape,spp", [(25, (20, 30, 200), 1), (7, (40, 130, 300), 2)])
+@pytest.mark.parametrize("npts,shape,spp", [(25, (20, 30, 200), 1), (7, (40, 130, 300), 3)])
 def test_place_fp16_then_leapfrog_bit_exact(gpu, orc, npts, shape, spp):
     """Placement of 16-bit fields: the kept buffers hold no scale state and
     no zero halo until the fields are set (a LEAPFROG call before that is
     refused); after an upload the steps (the packed sweep, or for 7 points the
-    two-step pass) are bit-identical to the oracle's emulation."""
+    three-step pass) are bit-identical to the oracle's emulation."""
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     u, v, s = seeded(orc, g, 47, 0.0)
@@ -93,6 +93,6 @@ def test_place_fp16_then_leapfrog_bit_exact(gpu, orc, npts, shape, spp):
         gu, eu, mu = ctx.download_raw16("u", R)
     a, b = qs[0].copy(), qs[1].copy()
     newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
-                              np.float32(c1), c2, 4, pass2=spp == 2)
+                              np.float32(c1), c2, 4, plen=spp)
     assert eu == exps[newest]
     assert np.array_equal(gu[g.interior], (a if newest == 0 else b)[g.interior])
