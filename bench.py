@@ -432,8 +432,9 @@ def measured_traffic(n, dtype_code, tune, npts=NPTS, mode=1):
     if t is None or t.get("grid") != [n, n, n]:
         return None, None
     if tune.get("steps_per_pass", 0) >= 2 and mode == 1 and dtype_code == 1:
-        # the FP16 two-step pass
-        prefix = f"void sth::sweep2h_kernel<{npts}, {tune['rows_per_wave']}, {tune['waves']}"
+        # the FP16 passes
+        prefix = (f"void sth::sweep2h_kernel<{npts}, {tune['rows_per_wave']}, {tune['waves']}, "
+                  f"{tune['steps_per_pass']}")
     elif tune.get("steps_per_pass", 0) >= 2 and mode == 1:
         prefix = (f"void sth::sweep2_kernel<{npts}, {tune['rows_per_wave']}, "
                   f"{tune['waves']}, {tune['steps_per_pass']}")
