@@ -1,6 +1,6 @@
 """Degenerate and tiny grids through every path: one-point rows, one-row
 planes, one plane, widths below one wave, for REF and LEAPFROG in FP32, the
-16-bit forms, the two- and three-steps-per-pass sweeps and both launch
+16-bit forms, the FP32 and FP16 passes of two to five steps and both launch
 forms.  Every
 result must equal the oracle bit for bit."""
 import numpy as np
@@ -79,3 +79,42 @@ def test_tiny_q16(gpu, orc, fmt, dtype, shape):
     assert eu == weu
     assert np.array_equal(gu[I], wu[I])
     assert np.float32(mu) == np.float32(wmu)
+
+
+@pytest.mark.parametrize("npts", [7, 13, 25])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_tiny_fp16_passes(gpu, orc, npts, shape):
+    """FP16 passes of every length the stencil has (two to five steps for 7
+    points, two and three for 13, two for 25) asked for on degenerate grids,
+    where the passes' warm-up planes and edge tiles lie mostly outside the
+    grid: five steps (passes and the steps left over), bits and exponents
+    equal to the oracle's emulation of the same passes."""
+    R = orc.radius(npts)
+    g = orc.Grid(*shape, R, R, R)
+    xs = [orc.fill_uniform(g, 47, k) for k in range(3)]
+    xs[2] = xs[2] * np.float32(1e-3)
+    q0, e0, m0 = [], [], []
+    for x in xs:
+        e = orc.scale_exp(float(np.abs(x).max()))
+        q, mx = orc.quantize(1, g, x, e)
+        q0.append(q)
+        e0.append(e)
+        m0.append(mx)
+    c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
+    c2 = np.linspace(0.5, -0.5, 5).astype(np.float32)
+    I = g.interior
+    for spp in range({7: 2, 13: 2, 25: 2}[npts], {7: 5, 13: 3, 25: 2}[npts] + 1):
+        with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+            ctx.set_tuning(steps_per_pass=spp)
+            for f, q, e in zip(("u", "v", "s"), q0, e0):
+                ctx.upload_raw16(f, q, e, R)
+            ctx.leapfrog(5, c0, c1, c2)
+            gu, eu, mu = ctx.download_raw16("u", R)
+        a, b = q0[0].copy(), q0[1].copy()
+        exps, maxs = list(e0), list(m0)
+        newest = orc.leapfrog_q16(npts, 1, g, a, b, q0[2], exps, maxs, np.float32(c0),
+                                  np.float32(c1), c2, 5, plen=spp)
+        wu, weu, wmu = (a, exps[0], maxs[0]) if newest == 0 else (b, exps[1], maxs[1])
+        assert eu == weu, spp
+        assert np.array_equal(gu[I], wu[I]), spp
+        assert np.float32(mu) == np.float32(wmu), spp
