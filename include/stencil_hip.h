@@ -251,9 +251,8 @@ int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
 int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
 /* Times the candidate schedules on this context's geometry (REF or LEAPFROG
  * sweeps, 2*reps steps rounded up to a multiple of 60 each, fields are
- * modified; FP32 LEAPFROG also tries two to five steps per pass, FP16 LEAPFROG
- * the two-step pass), keeps
- * the fastest and returns it with its mean time
+ * modified; FP32 and FP16 LEAPFROG also try the stencil's passes of two to
+ * five steps), keeps the fastest and returns it with its mean time
  * per step in ms.  If the environment variable
  * STH_TUNE_CACHE names a file, a result stored there for the same (stencil,
  * dtype, mode, xs, ys, zs) is applied without timing, and new results are
