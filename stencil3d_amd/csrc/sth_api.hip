@@ -422,19 +422,23 @@ int run_ref(sth_ctx *c, const Plan &p, uint32_t iterations) {
   return STH_OK;
 }
 
-// FP16 passes (sweep2h_kernel.h): the default shape of each pass length
+// 16-bit passes (sweep2h_kernel.h): the default shape of each pass length
 // (rows per wave; 16 waves), and whether the stencil has a pass of that
-// length (two steps for every stencil, three for 7 and 13 points, four and
-// five for 7).
-int fp16_pass_mj(int npts, int s) { return s == 2 ? (npts == 25 ? 3 : 4) : s == 3 ? 3 : 2; }
-bool fp16_pass_exists(int npts, int s) {
-  return s >= 2 && s <= 5 && fused2h_supported(npts, fp16_pass_mj(npts, s), 16, s);
+// length in the storage format (FP16: two steps for every stencil, three
+// for 7 and 13 points, four and five for 7; BF16: two and three steps for 7
+// points, two for 13).
+int q16_pass_mj(int fmt, int npts, int s) {
+  if (fmt == FMT_BF16) return npts == 7 && s == 2 ? 4 : 2;
+  return s == 2 ? (npts == 25 ? 3 : 4) : s == 3 ? 3 : 2;
+}
+bool q16_pass_exists(int fmt, int npts, int s) {
+  return s >= 2 && s <= 5 && fused2h_supported(fmt, npts, q16_pass_mj(fmt, npts, s), 16, s);
 }
 
-// The default FP16 pass length of a grid (0: the one-step sweep), from the
-// number of output tiles of each pass's default shape.  Measured on cubic
-// grids against the packed one-step sweep (one-step time / pass time per
-// step):
+// The default 16-bit pass length of a grid (0: the one-step sweep), from
+// the number of output tiles of each pass's default shape.  Measured on
+// cubic grids (one-step time / pass time per step).  FP16, against the
+// packed one-step sweep:
 //   7 points, three steps: 0.89-0.92 at 96^3 and 128^3 (3 and 6 output
 //     tiles), 2.7 at 192^3 (10), 1.8-2.0 from 256^3 to 640^3; four steps
 //     2.0-2.3 from 640^3, 2-6 % faster than three from 768^3 (240 tiles of
@@ -447,13 +451,24 @@ bool fp16_pass_exists(int npts, int s) {
 //   25 points: the two-step pass does 1.47 lane-updates per point-update
 //     and is instruction-bound where the sweep is held by the HBM
 //     read/write mix (0.63-0.98 from 256^3 to 1040^3), so it stays one-step.
-int fp16_default_pass(const sth_ctx *c) {
+// BF16 (against its one-step sweep):
+//   7 points, two steps (4 x 16): 0.67 at 128^3 (6 tiles), 1.20 at 192^3
+//     (8), 1.30 at 256^3; three steps (2 x 16): 1.15 at 256^3 (30 tiles of
+//     its shape), 2.2 at 384^3 (56), 1.7-1.8 from 512^3 to 1040^3;
+//   13 points, two steps (2 x 16): 1.04 at 256^3 (30 tiles), 1.59 at 384^3,
+//     1.14-1.21 from 512^3 to 1040^3;
+//   25 points: no pass fits 128 VGPRs (see the pass list).
+int q16_default_pass(const sth_ctx *c) {
   const int npts = (int)c->g.npts;
   if (npts == 25 || (int)c->g.xs < 16 * c->R) return 0;
   auto tiles = [&](int s) {
-    const Fused2Geom g = fused2h_geom(npts, fp16_pass_mj(npts, s), 16, s);
+    const Fused2Geom g = fused2h_geom(c->fmt, npts, q16_pass_mj(c->fmt, npts, s), 16, s);
     return (((int)c->g.zs + g.ok - 1) / g.ok) * (((int)c->g.ys + g.oj - 1) / g.oj);
   };
+  if (c->fmt == FMT_BF16) {
+    if (npts == 7) return tiles(3) >= 50 ? 3 : tiles(2) >= 8 ? 2 : 0;
+    return tiles(2) >= 30 ? 2 : 0;
+  }
   if (npts == 7) return tiles(4) >= 200 ? 4 : tiles(3) >= 10 ? 3 : 0;
   return tiles(3) >= 200 ? 3 : tiles(2) >= 70 ? 2 : 0;
 }
@@ -464,9 +479,10 @@ int fp16_default_pass(const sth_ctx *c) {
 // for one step per pass, the one-step sweep.
 bool fused_applies(const sth_ctx *c, const sth_tuning &t) {
   if (c->no_fuse) return false;
-  if (c->fmt == FMT_F16) {
-    if (t.steps_per_pass) return fp16_pass_exists((int)c->g.npts, (int)t.steps_per_pass);
-    return fp16_default_pass(c) >= 2;
+  if (is16(c)) {
+    if (t.steps_per_pass)
+      return q16_pass_exists(c->fmt, (int)c->g.npts, (int)t.steps_per_pass);
+    return q16_default_pass(c) >= 2;
   }
   if (c->fmt != FMT_F32) return false;
   if (t.steps_per_pass) return t.steps_per_pass >= 2;
@@ -493,12 +509,13 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
   const int npts = (int)c->g.npts;
   Plan2 p;
   Fused2Geom g;
-  if (c->fmt == FMT_F16) {
-    // FP16 storage (sweep2h_kernel.h): the tuning's pass length where the
-    // stencil has it, else two steps
-    p.s = fp16_pass_exists(npts, (int)t.steps_per_pass) ? (int)t.steps_per_pass
-                                                        : std::max(2, fp16_default_pass(c));
-    if (fused2h_supported(npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
+  if (is16(c)) {
+    // 16-bit storage (sweep2h_kernel.h): the tuning's pass length where the
+    // stencil has it, else the grid's default
+    p.s = q16_pass_exists(c->fmt, npts, (int)t.steps_per_pass)
+              ? (int)t.steps_per_pass
+              : std::max(2, q16_default_pass(c));
+    if (fused2h_supported(c->fmt, npts, (int)t.rows_per_wave, (int)t.waves, p.s)) {
       p.mj = (int)t.rows_per_wave;
       p.nw = (int)t.waves;
     } else {
@@ -506,10 +523,10 @@ Plan2 make_plan2(const sth_ctx *c, const sth_tuning &t) {
       // against 1.48 for 3 x 16), 25 points 3 x 16 (4 x 16 exceeds 128
       // VGPRs); three steps 3 x 16, four and five 2 x 16 (the LDS planes of
       // the longer passes)
-      p.mj = fp16_pass_mj(npts, p.s);
+      p.mj = q16_pass_mj(c->fmt, npts, p.s);
       p.nw = 16;
     }
-    g = fused2h_geom(npts, p.mj, p.nw, p.s);
+    g = fused2h_geom(c->fmt, npts, p.mj, p.nw, p.s);
   } else {
     if (npts == 25 || t.steps_per_pass == 2)
       p.s = 2;
@@ -632,7 +649,7 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
   a.c2d = c2n && p.s >= 4 ? c2n[3] : 0.0f;
   a.c2e = c2n && p.s >= 5 ? c2n[4] : 0.0f;
   memcpy(a.w, wts.w, sizeof(a.w));
-  const bool h16 = c->fmt == FMT_F16;
+  const bool h16 = is16(c);
   if (h16) {
     a.qs = c->qs_dev;
     a.slot = c->slot;
@@ -641,7 +658,7 @@ int run_pass2(sth_ctx *c, const Plan2 &p, const SweepArgs &wts, float c0, float 
     a.gain = wts.gain;
   }
   const hipError_t e =
-      h16 ? launch_sweep2h((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a)
+      h16 ? launch_sweep2h(c->fmt, (int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a)
           : launch_sweep2((int)c->g.npts, p.mj, p.nw, p.s, p.grid, c->stream, a);
   if (e != hipSuccess) return hip_status(e);
   if (c->timing) c->launches++;
@@ -673,7 +690,7 @@ int run_leap(sth_ctx *c, const Plan &p, uint32_t steps, float c0, float c1, cons
       // steps), then one-step sweeps
       for (int rs = p2.s - 1; rs >= 2; rs--) {
         if (n + (uint32_t)rs > steps) continue;
-        if (c->fmt == FMT_F16 && !fp16_pass_exists((int)c->g.npts, rs)) continue;
+        if (is16(c) && !q16_pass_exists(c->fmt, (int)c->g.npts, rs)) continue;
         sth_tuning t2 = c->tune;
         t2.steps_per_pass = (uint32_t)rs;
         t2.rows_per_wave = t2.waves = t2.chunks = 0;
@@ -1272,14 +1289,14 @@ int sth_set_tuning(sth_ctx *c, const sth_tuning *t) {
       t->ref_sweeps > 4096)
     return STH_ERR_ARG;
   if (t->steps_per_pass >= 2 && c->fmt != FMT_F32 &&
-      !(c->fmt == FMT_F16 && fp16_pass_exists((int)c->g.npts, (int)t->steps_per_pass)))
+      !(is16(c) && q16_pass_exists(c->fmt, (int)c->g.npts, (int)t->steps_per_pass)))
     return STH_ERR_UNSUPPORTED;
   if ((t->steps_per_pass == 3 && c->g.npts == 25) || (t->steps_per_pass >= 4 && c->g.npts != 7))
     return STH_ERR_UNSUPPORTED;
   const bool fused_shape =
       t->steps_per_pass >= 2 &&
-      (c->fmt == FMT_F16
-           ? fused2h_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves,
+      (is16(c)
+           ? fused2h_supported(c->fmt, (int)c->g.npts, (int)t->rows_per_wave, (int)t->waves,
                                (int)t->steps_per_pass)
            : fused2_supported((int)c->g.npts, (int)t->rows_per_wave, (int)t->waves,
                               (int)t->steps_per_pass));
@@ -1335,8 +1352,8 @@ bool tune_cache_lookup(const char *path, const sth_ctx *c, int leap, sth_tuning 
       continue;
     const bool ok =
         spp >= 2 ? (!c->no_fuse &&
-                    (c->fmt == FMT_F16
-                         ? fused2h_supported((int)npts, (int)mj, (int)nw, (int)spp)
+                    (is16(c)
+                         ? fused2h_supported(c->fmt, (int)npts, (int)mj, (int)nw, (int)spp)
                          : c->fmt == FMT_F32 &&
                                fused2_supported((int)npts, (int)mj, (int)nw, (int)spp)))
                  : shape_supported(c, (int)mj, (int)nw, (int)vec);
@@ -1404,12 +1421,12 @@ int sth_autotune(sth_ctx *c, int leapfrog, uint32_t reps, sth_tuning *best, floa
       if (ch != 0 && p.chunks != ch) continue;
       cand.push_back(t);
     }
-  // FP16: the passes' shapes
-  if (leapfrog && c->fmt == FMT_F16 && !c->no_fuse)
+  // 16-bit storage: the passes' shapes
+  if (leapfrog && is16(c) && !c->no_fuse)
     for (int sp = 2; sp <= 5; sp++)
       for (int mj : {2, 3, 4})
         for (int ch : {0, 4, 8, 16}) {
-          if (!fused2h_supported((int)c->g.npts, mj, 16, sp)) continue;
+          if (!fused2h_supported(c->fmt, (int)c->g.npts, mj, 16, sp)) continue;
           sth_tuning t = saved;
           t.rows_per_wave = (uint32_t)mj;
           t.waves = 16;

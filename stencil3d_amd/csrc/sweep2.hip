@@ -42,34 +42,40 @@ hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hip
 }
 
 // FP16 storage, two steps per pass: (stencil, rows per wave, waves)
-// (stencil, rows per wave, waves, steps per pass) of the FP16 passes
-#define STH_FUSED2H_LIST(X)                                                           \
-  X(7, 4, 16, 2) X(13, 4, 16, 2) X(13, 3, 16, 2) X(25, 3, 16, 2) X(7, 3, 16, 3)       \
-  X(13, 3, 16, 3) X(7, 2, 16, 4) X(7, 2, 16, 5)
+// (storage format, stencil, rows per wave, waves, steps per pass) of the
+// 16-bit passes.  BF16 has no mixed-precision FMA on gfx950: its halves are
+// widened by a shift or a mask, which keeps more values live, and only the
+// shapes listed fit 128 VGPRs without spilling (25 points, 13 points' three
+// steps and 7 points' four and five spill at every shape with 16 waves).
+#define STH_FUSED2H_LIST(X)                                                                 \
+  X(FMT_F16, 7, 4, 16, 2) X(FMT_F16, 13, 4, 16, 2) X(FMT_F16, 13, 3, 16, 2)                \
+  X(FMT_F16, 25, 3, 16, 2) X(FMT_F16, 7, 3, 16, 3) X(FMT_F16, 13, 3, 16, 3)                 \
+  X(FMT_F16, 7, 2, 16, 4) X(FMT_F16, 7, 2, 16, 5) X(FMT_BF16, 7, 4, 16, 2)                  \
+  X(FMT_BF16, 7, 2, 16, 3) X(FMT_BF16, 13, 2, 16, 2)
 
-bool fused2h_supported(int npts, int mj, int nw, int steps) {
-#define STH_F2H_SUP(N, MJ, NW, S) \
-  if (npts == N && mj == MJ && nw == NW && steps == S) return true;
+bool fused2h_supported(int fmt, int npts, int mj, int nw, int steps) {
+#define STH_F2H_SUP(F, N, MJ, NW, S) \
+  if (fmt == F && npts == N && mj == MJ && nw == NW && steps == S) return true;
   STH_FUSED2H_LIST(STH_F2H_SUP)
 #undef STH_F2H_SUP
   return false;
 }
 
-Fused2Geom fused2h_geom(int npts, int mj, int nw, int steps) {
-#define STH_F2H_GEOM(N, MJ, NW, S)                        \
-  if (npts == N && mj == MJ && nw == NW && steps == S)    \
+Fused2Geom fused2h_geom(int fmt, int npts, int mj, int nw, int steps) {
+#define STH_F2H_GEOM(F, N, MJ, NW, S)                                \
+  if (fmt == F && npts == N && mj == MJ && nw == NW && steps == S)   \
     return {Fused2hShape<N, MJ, NW, S>::OJ, Fused2hShape<N, MJ, NW, S>::OK, 64 * NW};
   STH_FUSED2H_LIST(STH_F2H_GEOM)
 #undef STH_F2H_GEOM
   return {0, 0, 0};
 }
 
-hipError_t launch_sweep2h(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
-                          const Sweep2Args &a) {
-#define STH_F2H_LAUNCH(N, MJ, NW, S)                                                     \
-  if (npts == N && mj == MJ && nw == NW && steps == S) {                                 \
-    hipLaunchKernelGGL((sweep2h_kernel<N, MJ, NW, S>), dim3(grid), dim3(64 * NW), 0, st, a); \
-    return hipGetLastError();                                                            \
+hipError_t launch_sweep2h(int fmt, int npts, int mj, int nw, int steps, unsigned grid,
+                          hipStream_t st, const Sweep2Args &a) {
+#define STH_F2H_LAUNCH(F, N, MJ, NW, S)                                                       \
+  if (fmt == F && npts == N && mj == MJ && nw == NW && steps == S) {                         \
+    hipLaunchKernelGGL((sweep2h_kernel<N, MJ, NW, S, F>), dim3(grid), dim3(64 * NW), 0, st, a); \
+    return hipGetLastError();                                                                \
   }
   STH_FUSED2H_LIST(STH_F2H_LAUNCH)
 #undef STH_F2H_LAUNCH

@@ -18,13 +18,13 @@ bool fused2_supported(int npts, int mj, int nw, int steps);
 Fused2Geom fused2_geom(int npts, int mj, int nw, int steps);
 hipError_t launch_sweep2(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
                          const Sweep2Args &a);
-// The FP16 passes of two to five steps (sweep2h_kernel.h): their compiled
-// shapes, their output tiles, and the launch (one workgroup per output tile
-// and i-chunk).
-bool fused2h_supported(int npts, int mj, int nw, int steps);
-Fused2Geom fused2h_geom(int npts, int mj, int nw, int steps);
-hipError_t launch_sweep2h(int npts, int mj, int nw, int steps, unsigned grid, hipStream_t st,
-                          const Sweep2Args &a);
+// The 16-bit (FP16, BF16) passes of two to five steps (sweep2h_kernel.h):
+// their compiled shapes, their output tiles, and the launch (one workgroup
+// per output tile and i-chunk).
+bool fused2h_supported(int fmt, int npts, int mj, int nw, int steps);
+Fused2Geom fused2h_geom(int fmt, int npts, int mj, int nw, int steps);
+hipError_t launch_sweep2h(int fmt, int npts, int mj, int nw, int steps, unsigned grid,
+                          hipStream_t st, const Sweep2Args &a);
 // The batch form (persistent, items of many shots from a queue) exists for
 // each stencil's default shape of each pass length.
 bool batch2_supported(int npts, int mj, int nw, int steps);

@@ -1,7 +1,8 @@
-// sweep2h_kernel.h -- two to five LEAPFROG steps per pass over FP16 storage.
+// sweep2h_kernel.h -- two to five LEAPFROG steps per pass over 16-bit storage.
 //
 // The FP32 multi-step pass (sweep2_kernel.h) with the data kept in its
-// stored binary16 form, as in the packed one-step sweep (sweep16_kernel.h):
+// stored 16-bit form (FP16, as in the packed one-step sweep of
+// sweep16_kernel.h, or BF16):
 //
 //  * a lane holds two consecutive k points as one 32-bit word in every
 //    queue and LDS plane, so the 64 lanes of a row cover 128 columns (E1:
@@ -10,12 +11,14 @@
 //    columns on each side, enough for step one's columns widened by
 //    (S-1)R and their own R-wide k-halo (S R <= 16), so the pass loads no
 //    separate k-halo;
-//  * neighbour pairs are one v_fma_mix_f32 each, the centre, v and s terms
-//    take their halves inside the fused multiply-adds, and results round to
-//    binary16 in pairs (v_cvt_pk_f16_f32);
+//  * FP16: neighbour pairs are one v_fma_mix_f32 each, the centre, v and s
+//    terms take their halves inside the fused multiply-adds, and results
+//    round to binary16 in pairs (v_cvt_pk_f16_f32); BF16: a half widens
+//    exactly by a shift or a mask, and results round to nearest even in
+//    integer arithmetic (the one-step sweep's encoding);
 //  * step M+1 reads u_{n+M} as stored: each step's results are rounded to
-//    binary16 (with their output exponent) before they enter the queue and
-//    the LDS plane, so a pass is S of the one-step scheme's steps;
+//    the 16-bit format (with their output exponent) before they enter the
+//    queue and the LDS plane, so a pass is S of the one-step scheme's steps;
 //  * the output exponents: step M's from the bound
 //      B_M = (|c0| + |c1| gain) B_{M-1} + B_{M-2} + |c2_M| M_s,
 //      B_0 = M_u, B_{-1} = M_v
@@ -83,7 +86,44 @@ __host__ __device__ inline void q16_pass_exps(const Q16Slot &cur, int ubuf, int 
   }
 }
 
-// Packed unsigned maximum of two binary16 magnitudes per word.
+// A 16-bit pair per 32-bit word: half H widened to FP32, the FP32 sum of
+// half H of two words (one rounding: the sum of the widened values), -(half
+// H of x * k) (one rounding, as the oracle's -(v * kv)), and two FP32 results
+// rounded into one word.
+template <int FMT>
+struct Pk16;
+template <>
+struct Pk16<FMT_F16> {
+  template <int H>
+  __device__ static float wide(uint32_t x) { return hwide<H>(x); }
+  template <int H>
+  __device__ static float add(uint32_t x, uint32_t y) { return hadd<H>(x, y); }
+  template <int H>
+  __device__ static float negmul(uint32_t x, float k) { return hnegmul<H>(x, k); }
+  __device__ static uint32_t pack(float a, float b) {
+    using H2 = _Float16 __attribute__((ext_vector_type(2)));
+    H2 r;
+    r.x = (_Float16)a;
+    r.y = (_Float16)b;
+    return __builtin_bit_cast(uint32_t, r);
+  }
+};
+template <>
+struct Pk16<FMT_BF16> {
+  template <int H>
+  __device__ static float wide(uint32_t x) {
+    return __uint_as_float(H ? (x & 0xffff0000u) : (x << 16));
+  }
+  template <int H>
+  __device__ static float add(uint32_t x, uint32_t y) { return wide<H>(x) + wide<H>(y); }
+  template <int H>
+  __device__ static float negmul(uint32_t x, float k) { return -(wide<H>(x) * k); }
+  __device__ static uint32_t pack(float a, float b) {
+    return (uint32_t)Codec<FMT_BF16>::enc(a) | ((uint32_t)Codec<FMT_BF16>::enc(b) << 16);
+  }
+};
+
+// Packed unsigned maximum of two 16-bit magnitudes per word.
 __device__ __forceinline__ uint32_t pk_umax16(uint32_t a, uint32_t b) {
   using U2 = unsigned short __attribute__((ext_vector_type(2)));
   return __builtin_bit_cast(
@@ -95,9 +135,10 @@ __device__ __forceinline__ uint32_t pk_umax16(uint32_t a, uint32_t b) {
 // columns have used them.  iw(r): the words of the i-neighbours at -r, +r;
 // jw(r): the j-neighbours; kw(d): the word d to the right of the lane's
 // (d = -KH .. KH).
-template <int NPTS, int KH, class IW, class JW, class KWf>
+template <int NPTS, int KH, int FMT, class IW, class JW, class KWf>
 __device__ __forceinline__ void lap_pair(const float *w, IW &&iw, JW &&jw, KWf &&kw, float &L0,
                                          float &L1) {
+  using PK = Pk16<FMT>;
   using Tm = Terms<NPTS>;
   static_for<Tm::T>([&](auto tc) __attribute__((always_inline)) {
     constexpr int t = decltype(tc)::value;
@@ -106,15 +147,15 @@ __device__ __forceinline__ void lap_pair(const float *w, IW &&iw, JW &&jw, KWf &
     if constexpr (ax == 2) {
       // halves 2KH + c -/+ r of the words kw(-KH) .. kw(KH)
       constexpr int l0 = 2 * KH - r, h0 = 2 * KH + r, l1 = l0 + 1, h1 = h0 + 1;
-      p0 = hadd<l0 & 1>(kw(std::integral_constant<int, (l0 >> 1) - KH>{}),
-                        kw(std::integral_constant<int, (h0 >> 1) - KH>{}));
-      p1 = hadd<l1 & 1>(kw(std::integral_constant<int, (l1 >> 1) - KH>{}),
-                        kw(std::integral_constant<int, (h1 >> 1) - KH>{}));
+      p0 = PK::template add<l0 & 1>(kw(std::integral_constant<int, (l0 >> 1) - KH>{}),
+                                    kw(std::integral_constant<int, (h0 >> 1) - KH>{}));
+      p1 = PK::template add<l1 & 1>(kw(std::integral_constant<int, (l1 >> 1) - KH>{}),
+                                    kw(std::integral_constant<int, (h1 >> 1) - KH>{}));
     } else {
       const auto xy = ax == 0 ? iw(std::integral_constant<int, r>{})
                               : jw(std::integral_constant<int, r>{});
-      p0 = hadd<0>(xy.x, xy.y);
-      p1 = hadd<1>(xy.x, xy.y);
+      p0 = PK::template add<0>(xy.x, xy.y);
+      p1 = PK::template add<1>(xy.x, xy.y);
     }
     if constexpr (t == 0) {
       L0 = w[0] * p0;
@@ -127,26 +168,25 @@ __device__ __forceinline__ void lap_pair(const float *w, IW &&iw, JW &&jw, KWf &
 }
 
 // One step's update of a lane's word: both halves in the one-step sweep's
-// order, each result rounded to FP32 and then (in pairs) to binary16.
+// order, each result rounded to FP32 and then (in pairs) to the 16-bit
+// format.
 //   out = k2 s + (k1 L + (k0 c - kv v))
+template <int FMT>
 __device__ __forceinline__ uint32_t update_pair(float k0, float k1, float k2, float kv,
                                                 uint32_t c, uint32_t v, uint32_t s, float L0,
                                                 float L1) {
+  using PK = Pk16<FMT>;
   float o[2];
   static_for<2>([&](auto cc) __attribute__((always_inline)) {
     constexpr int h = decltype(cc)::value;
-    const float t0 = __builtin_fmaf(k0, hwide<h>(c), hnegmul<h>(v, kv));
-    float x = __builtin_fmaf(k2, hwide<h>(s), __builtin_fmaf(k1, h ? L1 : L0, t0));
+    const float t0 = __builtin_fmaf(k0, PK::template wide<h>(c), PK::template negmul<h>(v, kv));
+    float x = __builtin_fmaf(k2, PK::template wide<h>(s), __builtin_fmaf(k1, h ? L1 : L0, t0));
     // keep the last fma apart from the conversion below (two roundings,
-    // FP32 then binary16, as the oracle rounds)
+    // FP32 then 16-bit, as the oracle rounds)
     __asm__ volatile("" : "+v"(x));
     o[h] = x;
   });
-  using H2 = _Float16 __attribute__((ext_vector_type(2)));
-  H2 r;
-  r.x = (_Float16)o[0];
-  r.y = (_Float16)o[1];
-  return __builtin_bit_cast(uint32_t, r);
+  return PK::pack(o[0], o[1]);
 }
 
 // One work item: output tile (tk, tj), i-chunk ch.  red: 2 NW floats.
@@ -156,7 +196,7 @@ __device__ __forceinline__ uint32_t update_pair(float k0, float k1, float k2, fl
 // queue of step M-1's values (its i neighbours and in-wave j neighbours) and
 // an LDS plane of them (the other j and k neighbours), into a queue of its
 // own; the last step writes u_{n+S-1} and u_{n+S} of the output tile.
-template <int NPTS, int MJ, int NW, int S>
+template <int NPTS, int MJ, int NW, int S, int FMT>
 __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S> &K,
                                              const int tk, const int tj, const int ch,
                                              uint32_t *SM, float *red) {
@@ -413,7 +453,7 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
       constexpr int J = decltype(jc)::value, C = gcen<S, R>(J);
       using W2 = uint32_t __attribute__((ext_vector_type(2)));
       const uint32_t cw = G[J][m][C];
-      lap_pair<NPTS, KH>(
+      lap_pair<NPTS, KH, FMT>(
           a.w,
           [&](auto rc) -> W2 {
             constexpr int r = decltype(rc)::value;
@@ -444,9 +484,9 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
       const uint32_t cw = G[J][m][gcen<S, R>(J)];
       float L0, L1;
       lapq(std::integral_constant<int, J>{}, m, L0, L1);
-      const uint32_t zw = update_pair(K.k0[S - 1], K.k1[S - 1], K.k2[S - 1], K.kv[S - 1], cw,
-                                      older(std::integral_constant<int, J - 1>{}, m), sc[S][m],
-                                      L0, L1);
+      const uint32_t zw =
+          update_pair<FMT>(K.k0[S - 1], K.k1[S - 1], K.k2[S - 1], K.kv[S - 1], cw,
+                           older(std::integral_constant<int, J - 1>{}, m), sc[S][m], L0, L1);
       if (store && rowin[m]) {
         if (kout0 && kout1) {
           buf_store<uint32_t, 2>(cw, Wr, off[m]);
@@ -483,8 +523,8 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
           float L0, L1;
           lapq(std::integral_constant<int, M - 1>{}, m, L0, L1);
           const uint32_t zw =
-              update_pair(K.k0[M - 1], K.k1[M - 1], K.k2[M - 1], K.kv[M - 1],
-                          G[M - 1][m][gcen<S, R>(M - 1)], old, sc[M][m], L0, L1);
+              update_pair<FMT>(K.k0[M - 1], K.k1[M - 1], K.k2[M - 1], K.kv[M - 1],
+                               G[M - 1][m][gcen<S, R>(M - 1)], old, sc[M][m], L0, L1);
           G[M][m][glen<S, R>(M) - 1] =
               (rowin[m] && pinm) ? ((zw & kinmask) | (old & ~kinmask)) : old;
         }
@@ -499,7 +539,7 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
       const uint32_t cw = q[m][R + 1];
       using W2 = uint32_t __attribute__((ext_vector_type(2)));
       float L0, L1;
-      lap_pair<NPTS, KH>(
+      lap_pair<NPTS, KH, FMT>(
           a.w,
           [&](auto rc) -> W2 {
             constexpr int r = decltype(rc)::value;
@@ -515,7 +555,8 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
             return d == 0 ? cw : (uint32_t)Lat(m, 0, d);
           },
           L0, L1);
-      const uint32_t ww = update_pair(K.k0[0], K.k1[0], K.k2[0], K.kv[0], cw, vv[m], s1[m], L0, L1);
+      const uint32_t ww =
+          update_pair<FMT>(K.k0[0], K.k1[0], K.k2[0], K.kv[0], cw, vv[m], s1[m], L0, L1);
       // halves inside the interior take the new value, the others v's
       G[1][m][glen<S, R>(1) - 1] =
           (rowin[m] && pin) ? ((ww & kinmask) | (vv[m] & ~kinmask)) : vv[m];
@@ -545,11 +586,7 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
 
   // Maxima of the item: the larger half, widened, reduced over the
   // workgroup, one atomicMax per output field.
-  using H2 = _Float16 __attribute__((ext_vector_type(2)));
-  auto hmax = [](uint32_t x) {
-    const uint32_t b = max(x & 0xffffu, x >> 16);
-    return (float)__builtin_bit_cast(H2, b).x;
-  };
+  auto hmax = [](uint32_t x) { return Pk16<FMT>::template wide<0>(max(x & 0xffffu, x >> 16)); };
   float m1 = hmax(mx1), m2 = hmax(mx2);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -576,7 +613,7 @@ __device__ __forceinline__ void sweep2h_tile(const Sweep2Args &a, const Coef2h<S
   }
 }
 
-template <int NPTS, int MJ, int NW, int S>
+template <int NPTS, int MJ, int NW, int S, int FMT>
 __global__ __launch_bounds__(64 * NW) void sweep2h_kernel(const Sweep2Args a) {
   using F = Fused2hShape<NPTS, MJ, NW, S>;
   __shared__ __attribute__((aligned(16))) uint32_t SM[F::LDS_WORDS];
@@ -621,7 +658,7 @@ __global__ __launch_bounds__(64 * NW) void sweep2h_kernel(const Sweep2Args a) {
   if (a.remap) bid = xcd_remap(bid, (int)gridDim.x);
   const int tk = bid % a.ntk;
   const int tt = bid / a.ntk;
-  sweep2h_tile<NPTS, MJ, NW, S>(a, K, tk, tt % a.ntj, tt / a.ntj, SM, red);
+  sweep2h_tile<NPTS, MJ, NW, S, FMT>(a, K, tk, tt % a.ntj, tt / a.ntj, SM, red);
 }
 
 }  // namespace sth

@@ -372,27 +372,33 @@ def test_three_steps_rejected_for_25_points_and_16bit(gpu):
 
 def test_16bit_pass_lengths(gpu):
     """FP16 storage has passes of two steps for every stencil, three for 7
-    and 13 points, four and five for 7 points; BF16 has none."""
+    and 13 points, four and five for 7 points; BF16 two and three steps for
+    7 points and two for 13, none for 25."""
     with gpu.Stencil(16, 16, 64, npts=25, dtype="bf16") as ctx:
         with pytest.raises(gpu.SthError):
             ctx.set_tuning(steps_per_pass=2)
-    for npts, longest in ((25, 2), (13, 3), (7, 5)):
-        with gpu.Stencil(16, 16, 64, npts=npts, dtype="fp16") as ctx:
-            for spp in range(2, longest + 1):
-                ctx.set_tuning(steps_per_pass=spp)
-                assert ctx.get_tuning()["steps_per_pass"] == spp
-            with pytest.raises(gpu.SthError):
-                ctx.set_tuning(steps_per_pass=longest + 1)
+    for dtype, lengths in (("fp16", ((25, 2), (13, 3), (7, 5))),
+                           ("bf16", ((13, 2), (7, 3)))):
+        for npts, longest in lengths:
+            with gpu.Stencil(16, 16, 64, npts=npts, dtype=dtype) as ctx:
+                for spp in range(2, longest + 1):
+                    ctx.set_tuning(steps_per_pass=spp)
+                    assert ctx.get_tuning()["steps_per_pass"] == spp
+                with pytest.raises(gpu.SthError):
+                    ctx.set_tuning(steps_per_pass=longest + 1)
 
 
-@pytest.mark.parametrize("npts,shape,spp", [(7, (40, 130, 300), 3), (7, (40, 70, 300), 1),
-                                             (25, (40, 130, 300), 1)])
-def test_fp16_default_schedule(gpu, orc, npts, shape, spp):
-    """FP16 LEAPFROG takes the three-step pass by default for 7 points from
-    10 output tiles of its shape (four steps from 200; 13 points: two steps
-    from 70 tiles, three from 200), and stays one-step for 25 points; five
-    steps are then the passes and the steps left over, bit-identical to the
-    oracle's emulation of the same schedule."""
+@pytest.mark.parametrize("dtype,npts,shape,spp", [
+    ("fp16", 7, (40, 130, 300), 3), ("fp16", 7, (40, 70, 300), 1), ("fp16", 25, (40, 130, 300), 1),
+    ("bf16", 7, (40, 130, 300), 2), ("bf16", 13, (40, 130, 300), 1)])
+def test_16bit_default_schedule(gpu, orc, dtype, npts, shape, spp):
+    """16-bit LEAPFROG's default pass length follows the grid's output tiles
+    of each pass's shape: FP16 7 points three steps from 10 tiles (four from
+    200; 13 points two steps from 70 tiles, three from 200), 25 points one
+    step; BF16 7 points two steps from 8 tiles (three from 50), 13 points two
+    from 30.  Five steps are then the passes and the steps left over,
+    bit-identical to the oracle's emulation of the same schedule."""
+    fmt = 1 if dtype == "fp16" else 2
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     u, v, s = seeded(orc, g, 31, 0.0)  # the 16-bit form needs zero halos
@@ -400,13 +406,13 @@ def test_fp16_default_schedule(gpu, orc, npts, shape, spp):
     qs, exps, maxs = [], [], []
     for x in (u, v, s):
         e = orc.scale_exp(float(np.abs(x).max()))
-        q, mx = orc.quantize(1, g, x, e)
+        q, mx = orc.quantize(fmt, g, x, e)
         qs.append(q)
         exps.append(e)
         maxs.append(mx)
     c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
     c2 = np.linspace(0.25, -0.25, 5).astype(np.float32)
-    with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+    with gpu.Stencil(*shape, npts=npts, dtype=dtype) as ctx:
         assert ctx.get_tuning()["steps_per_pass"] == spp
         ctx.upload_raw16("u", qs[0], exps[0], R)
         ctx.upload_raw16("v", qs[1], exps[1], R)
@@ -414,7 +420,7 @@ def test_fp16_default_schedule(gpu, orc, npts, shape, spp):
         ctx.leapfrog(5, c0, c1, c2)
         gu, eu, mu = ctx.download_raw16("u", R)
     a, b = qs[0].copy(), qs[1].copy()
-    newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
+    newest = orc.leapfrog_q16(npts, fmt, g, a, b, qs[2], exps, maxs, np.float32(c0),
                               np.float32(c1), c2, 5, plen=spp)
     want = a if newest == 0 else b
     assert eu == exps[newest]

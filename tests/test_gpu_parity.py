@@ -281,21 +281,26 @@ def test_fp16_packed_folded_edge_bit_exact(gpu, orc, npts, tile, zs):
     assert np.float32(mu) == np.float32(wmu) and np.float32(mv) == np.float32(wmv)
 
 
-# (stencil, rows per wave, waves, steps per pass): every compiled FP16 pass
-FP16_PASSES = [(7, 4, 16, 2), (13, 3, 16, 2), (13, 4, 16, 2), (25, 3, 16, 2), (7, 3, 16, 3),
-               (13, 3, 16, 3), (7, 2, 16, 4), (7, 2, 16, 5)]
+# (storage format, stencil, rows per wave, waves, steps per pass): every
+# compiled 16-bit pass
+Q16_PASSES = [(1, 7, 4, 16, 2), (1, 13, 3, 16, 2), (1, 13, 4, 16, 2), (1, 25, 3, 16, 2),
+              (1, 7, 3, 16, 3), (1, 13, 3, 16, 3), (1, 7, 2, 16, 4), (1, 7, 2, 16, 5),
+              (2, 7, 4, 16, 2), (2, 7, 2, 16, 3), (2, 13, 2, 16, 2)]
 
 
-@pytest.mark.parametrize("npts,mj,nw,sp", FP16_PASSES,
-                         ids=[f"{n}pt-{m}x{w}-s{s}" for n, m, w, s in FP16_PASSES])
+@pytest.mark.parametrize("fmt,npts,mj,nw,sp", Q16_PASSES,
+                         ids=[f"{'fp16' if f == 1 else 'bf16'}-{n}pt-{m}x{w}-s{s}"
+                              for f, n, m, w, s in Q16_PASSES])
 @pytest.mark.parametrize("shape,chunks", [((30, 61, 201), 2), ((44, 130, 97), 3),
                                           ((17, 12, 40), 1)])
-def test_fp16_pass_bit_exact(gpu, orc, npts, mj, nw, sp, shape, chunks):
-    """The FP16 passes of two to five steps: seven steps (passes of sp and
-    the steps left over as one shorter pass or a single step), several
-    i-chunks, output tiles cut by the grid's edges in j and k (an odd zs
-    leaves a lane with one point inside), a source term; bits, exponents and
-    maxima equal to the oracle's binary16 emulation in the same passes."""
+def test_q16_pass_bit_exact(gpu, orc, fmt, npts, mj, nw, sp, shape, chunks):
+    """The 16-bit passes of two to five steps (FP16, BF16): seven steps
+    (passes of sp and the steps left over as one shorter pass or a single
+    step), several i-chunks, output tiles cut by the grid's edges in j and k
+    (an odd zs leaves a lane with one point inside), a source term; bits,
+    exponents and maxima equal to the oracle's 16-bit emulation in the same
+    passes."""
+    dtype = "fp16" if fmt == 1 else "bf16"
     R = orc.radius(npts)
     g = orc.Grid(*shape, R, R, R)
     v, s, u = seeded(orc, g, seed=23 + npts)
@@ -303,13 +308,13 @@ def test_fp16_pass_bit_exact(gpu, orc, npts, mj, nw, sp, shape, chunks):
     qs, exps, maxs = [], [], []
     for x in (u, v, s):
         e = orc.scale_exp(float(np.abs(x).max()))
-        q, mx = orc.quantize(1, g, x, e)
+        q, mx = orc.quantize(fmt, g, x, e)
         qs.append(q)
         exps.append(e)
         maxs.append(mx)
     c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
     c2 = np.linspace(0.5, -0.5, 7).astype(np.float32)
-    with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+    with gpu.Stencil(*shape, npts=npts, dtype=dtype) as ctx:
         ctx.set_tuning(rows_per_wave=mj, waves=nw, chunks=chunks, steps_per_pass=sp)
         t = ctx.get_tuning()
         assert (t["steps_per_pass"], t["rows_per_wave"], t["waves"]) == (sp, mj, nw)
@@ -320,7 +325,7 @@ def test_fp16_pass_bit_exact(gpu, orc, npts, mj, nw, sp, shape, chunks):
         gu, eu, mu = ctx.download_raw16("u", R)
         gv, ev, mv = ctx.download_raw16("v", R)
     a, b = qs[0].copy(), qs[1].copy()
-    newest = orc.leapfrog_q16(npts, 1, g, a, b, qs[2], exps, maxs, np.float32(c0),
+    newest = orc.leapfrog_q16(npts, fmt, g, a, b, qs[2], exps, maxs, np.float32(c0),
                               np.float32(c1), c2, 7, plen=sp)
     (wu, weu, wmu), (wv, wev, wmv) = (
         ((a, exps[0], maxs[0]), (b, exps[1], maxs[1])) if newest == 0 else

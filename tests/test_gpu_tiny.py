@@ -81,11 +81,13 @@ def test_tiny_q16(gpu, orc, fmt, dtype, shape):
     assert np.float32(mu) == np.float32(wmu)
 
 
+@pytest.mark.parametrize("fmt,dtype", [(1, "fp16"), (2, "bf16")])
 @pytest.mark.parametrize("npts", [7, 13, 25])
 @pytest.mark.parametrize("shape", SHAPES)
-def test_tiny_fp16_passes(gpu, orc, npts, shape):
-    """FP16 passes of every length the stencil has (two to five steps for 7
-    points, two and three for 13, two for 25) asked for on degenerate grids,
+def test_tiny_16bit_passes(gpu, orc, fmt, dtype, npts, shape):
+    """16-bit passes of every length the stencil has (FP16: two to five steps
+    for 7 points, two and three for 13, two for 25; BF16: two and three for
+    7 points, two for 13) asked for on degenerate grids,
     where the passes' warm-up planes and edge tiles lie mostly outside the
     grid: five steps (passes and the steps left over), bits and exponents
     equal to the oracle's emulation of the same passes."""
@@ -96,15 +98,16 @@ def test_tiny_fp16_passes(gpu, orc, npts, shape):
     q0, e0, m0 = [], [], []
     for x in xs:
         e = orc.scale_exp(float(np.abs(x).max()))
-        q, mx = orc.quantize(1, g, x, e)
+        q, mx = orc.quantize(fmt, g, x, e)
         q0.append(q)
         e0.append(e)
         m0.append(mx)
     c1, c0 = 0.16, 2.0 - orc.lap_gain(npts) * 0.16
     c2 = np.linspace(0.5, -0.5, 5).astype(np.float32)
     I = g.interior
-    for spp in range({7: 2, 13: 2, 25: 2}[npts], {7: 5, 13: 3, 25: 2}[npts] + 1):
-        with gpu.Stencil(*shape, npts=npts, dtype="fp16") as ctx:
+    longest = {1: {7: 5, 13: 3, 25: 2}, 2: {7: 3, 13: 2, 25: 1}}[fmt][npts]
+    for spp in range(2, longest + 1):
+        with gpu.Stencil(*shape, npts=npts, dtype=dtype) as ctx:
             ctx.set_tuning(steps_per_pass=spp)
             for f, q, e in zip(("u", "v", "s"), q0, e0):
                 ctx.upload_raw16(f, q, e, R)
@@ -112,7 +115,7 @@ def test_tiny_fp16_passes(gpu, orc, npts, shape):
             gu, eu, mu = ctx.download_raw16("u", R)
         a, b = q0[0].copy(), q0[1].copy()
         exps, maxs = list(e0), list(m0)
-        newest = orc.leapfrog_q16(npts, 1, g, a, b, q0[2], exps, maxs, np.float32(c0),
+        newest = orc.leapfrog_q16(npts, fmt, g, a, b, q0[2], exps, maxs, np.float32(c0),
                                   np.float32(c1), c2, 5, plen=spp)
         wu, weu, wmu = (a, exps[0], maxs[0]) if newest == 0 else (b, exps[1], maxs[1])
         assert eu == weu, spp
