@@ -264,8 +264,12 @@ def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
         if npts == 25:
             # the FP16 two-step pass, which is not 25 points' default
             modes += ("leapfrog_fp16_two_step",)
+        else:
+            # BF16 (25 points' BF16 is the headline's BF16 line)
+            modes += ("leapfrog_bf16",)
         for mode in modes:
-            dt = "fp16" if mode.startswith("leapfrog_fp16") else "fp32"
+            dt = ("fp16" if mode.startswith("leapfrog_fp16") else
+                  "bf16" if mode.startswith("leapfrog_bf16") else "fp32")
             with S.Stencil(n, n, n, npts=npts, dtype=dt) as ctx:
                 leap = mode.startswith("leapfrog")
                 twin = Twin(S, n, npts, "leapfrog" if leap else "ref")
@@ -297,11 +301,11 @@ def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
             per = ms / steps  # per step (a pass counts as its steps)
             g = float(n) ** 3 / (per * 1e-3) / 1e9
             r = {"kernel_ms_per_step": round(per, 4), "gpts": round(g, 2),
-                 "alg_equiv_gbs": round((8.0 if dt == "fp16" else 16.0) * g, 1),
+                 "alg_equiv_gbs": round((16.0 if dt == "fp32" else 8.0) * g, 1),
                  "gflops_nominal": round(FLOPS_PT[npts] * g, 1),
                  "steps_per_launch": round(steps / nl, 3),
                  "err_vs_fp64": {"maxrel": mr, "l2rel": lr}}
-            tb = traffic_of(n, npts, 1 if dt == "fp16" else 0, 1 if leap else 0, tune)
+            tb = traffic_of(n, npts, {"fp16": 1, "bf16": 2}.get(dt, 0), 1 if leap else 0, tune)
             if tb is not None and mode != "ref_10_per_launch":
                 r["traffic_gbs"] = round(tb / (ms / nl * 1e-3) / 1e9, 1)
             if leap:
@@ -314,7 +318,9 @@ def run_other_stencils(S, n, steps, warmup, traffic_of, place=0):
         log(f"[bench] {npts}-pt: REF {row['ref']['gpts']} Gpts/s, "
             f"LEAPFROG {row['leapfrog']['gpts']} Gpts/s, "
             f"REF 10 sweeps per launch {row['ref_10_per_launch']['gpts']} Gpts/s, "
-            f"LEAPFROG FP16 {row['leapfrog_fp16']['gpts']} Gpts/s")
+            f"LEAPFROG FP16 {row['leapfrog_fp16']['gpts']} Gpts/s" +
+            (f", LEAPFROG BF16 {row['leapfrog_bf16']['gpts']} Gpts/s" if "leapfrog_bf16" in row
+             else ""))
     return out
 
 
@@ -431,10 +437,10 @@ def measured_traffic(n, dtype_code, tune, npts=NPTS, mode=1):
     t = load_traffic()
     if t is None or t.get("grid") != [n, n, n]:
         return None, None
-    if tune.get("steps_per_pass", 0) >= 2 and mode == 1 and dtype_code == 1:
-        # the FP16 passes
+    if tune.get("steps_per_pass", 0) >= 2 and mode == 1 and dtype_code in (1, 2):
+        # the 16-bit passes (the last template argument: the storage format)
         prefix = (f"void sth::sweep2h_kernel<{npts}, {tune['rows_per_wave']}, {tune['waves']}, "
-                  f"{tune['steps_per_pass']}")
+                  f"{tune['steps_per_pass']}, {dtype_code}")
     elif tune.get("steps_per_pass", 0) >= 2 and mode == 1:
         prefix = (f"void sth::sweep2_kernel<{npts}, {tune['rows_per_wave']}, "
                   f"{tune['waves']}, {tune['steps_per_pass']}")
