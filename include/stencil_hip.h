@@ -112,12 +112,14 @@ typedef struct {
   uint32_t vec;           /* consecutive k points per lane: 1 or 2 (0: default) */
   uint32_t steps_per_pass; /* LEAPFROG, FP32 and FP16: 2 = two steps per pass over
                               memory (temporal blocking), 3 = three (7 and 13
-                              points), 4 or 5 = four or five (7 points); FP16: each
-                              step's scale from the bound of the previous steps'
-                              outputs (FP16 passes: rows_per_wave 4 or 3 for two
-                              steps, 3 for three, 2 for four and five; 16 waves);
-                              steps left over run as the longest shorter pass that
-                              fits, then one-step; 1 = one; 0: default */
+                              points), 4 or 5 = four or five (7 points); BF16: 2 (7
+                              and 13 points) or 3 (7 points).  16-bit: each step's
+                              scale from the bound of the previous steps' outputs
+                              (rows_per_wave, 16 waves: FP16 4 or 3 for two steps,
+                              3 for three, 2 for four and five; BF16 4 for 7
+                              points' two steps, else 2); steps left over run as
+                              the longest shorter pass that fits, then one-step;
+                              1 = one; 0: default */
   uint32_t launch;        /* one-step sweep: 1 = one workgroup per tile, 2 = persistent
                              workgroups drawing tiles from a work queue; 0: default (2) */
   uint32_t ref_sweeps;    /* REF (FP32, FP64): sweeps per launch, 1..4096 (0: one).  u and s
@@ -251,8 +253,8 @@ int sth_set_tuning(sth_ctx *ctx, const sth_tuning *t);
 int sth_get_tuning(const sth_ctx *ctx, sth_tuning *t);
 /* Times the candidate schedules on this context's geometry (REF or LEAPFROG
  * sweeps, 2*reps steps rounded up to a multiple of 60 each, fields are
- * modified; FP32 and FP16 LEAPFROG also try the stencil's passes of two to
- * five steps), keeps the fastest and returns it with its mean time
+ * modified; LEAPFROG also tries the passes of two to five steps the stencil
+ * has in the storage format), keeps the fastest and returns it with its mean time
  * per step in ms.  If the environment variable
  * STH_TUNE_CACHE names a file, a result stored there for the same (stencil,
  * dtype, mode, xs, ys, zs) is applied without timing, and new results are
