@@ -104,20 +104,24 @@ def run_throughput(S, n, dtype, steps, warmup, twin=None, tuning=None, copy_byte
             placed = {"tries": place, "ms_per_step": [round(x, 4) for x in ms], "chosen": chosen}
         init_pulse(ctx)
         ctx.leapfrog(warmup, c0, c1)
-        ctx.sync()
+        q0 = ctx.energy(c0, c1)  # synchronises the stream
         t0 = time.perf_counter()
         ctx.timer_start()
         ctx.leapfrog(steps, c0, c1)
         kms, launches = ctx.timer_read()  # synchronises the stream
         t1 = time.perf_counter()
         mx, l2 = ctx.norms("u")
+        q1 = ctx.energy(c0, c1)
         used = ctx.get_tuning()
         err = None
         if twin is not None:
             twin.advance_to(warmup + steps)
             err = err_dict(ctx, twin.ctx)
         copy_ms = ctx.stream_copy(copy_bytes, 5) if copy_bytes else None
-    return t1 - t0, kms, launches, used, (mx, l2), copy_ms, err, placed
+    # s = 0 and zero halos: the scheme's discrete energy is conserved, so its
+    # change over the timed steps is the storage format's rounding
+    energy = {"before": q0, "after": q1, "rel_drift": abs(q1 - q0) / abs(q0) if q0 else None}
+    return t1 - t0, kms, launches, used, (mx, l2, energy), copy_ms, err, placed
 
 
 def run_error_growth(S, dtypes=("fp32", "fp16", "bf16"), n=256, checkpoints=(100, 250, 500, 1000)):
@@ -521,21 +525,21 @@ def main():
     one_step = None
     if tune.get("steps_per_pass", 0) >= 2:
         # the one-step sweep on the same fields: the baseline of the FP16 ratio
-        w1, k1, l1, t1, _, _, e1, p1 = run_throughput(S, n, "fp32", K, W, twin=twin,
+        w1, k1, l1, t1, n1, _, e1, p1 = run_throughput(S, n, "fp32", K, W, twin=twin,
                                                       tuning={"steps_per_pass": 1},
                                                       place=args.place)
         g1 = pts * K / w1 / 1e9
         one_step = {"gpts": round(g1, 3), "ms_per_step": round(w1 / K * 1e3, 4),
                     "kernel_ms": round(k1 / l1, 4),
                     "alg_gbs": round(16.0 * pts / (k1 / l1 * 1e-3) / 1e9, 1), "tuning": t1,
-                    "err_vs_fp64": e1, "placement": p1}
+                    "err_vs_fp64": e1, "energy": n1[2], "placement": p1}
         log(f"[bench] fp32 one step per pass: {g1:.2f} Gpts/s")
 
     extra = {}
     if not args.no_16bit:
         for dt in ("fp16", "bf16"):
             log(f"[bench] {dt}: {W} warm-up + {K} timed steps")
-            w2, k2, l2, t2, _, _, e2, p2 = run_throughput(S, n, dt, K, W, twin=twin,
+            w2, k2, l2, t2, n2, _, e2, p2 = run_throughput(S, n, dt, K, W, twin=twin,
                                                           place=args.place)
             g2 = pts * K / w2 / 1e9
             a2 = 8.0 * pts / (k2 / l2 * 1e-3) / 1e9
@@ -543,7 +547,7 @@ def main():
                          "kernel_ms": round(k2 / l2, 4), "hbm_gbs_alg": round(a2, 1),
                          "frac_of_peak": round(a2 / PEAK_GBS, 4),
                          "speedup_vs_fp32": round(g2 / gpts, 3), "tuning": t2,
-                         "err_vs_fp64_1040": e2, "placement": p2}
+                         "err_vs_fp64_1040": e2, "energy": n2[2], "placement": p2}
             tb2, _ = measured_traffic(n, 1 if dt == "fp16" else 2, t2)
             if tb2 is not None:
                 extra[dt]["traffic_gbs"] = round(tb2 / (k2 / l2 * 1e-3) / 1e9, 1)
@@ -621,6 +625,7 @@ def main():
         "cpu_baseline": cpu,
         "placement": placed,
         "norms_u": {"max": nrm[0], "l2": nrm[1]},
+        "energy": nrm[2],
         "err_vs_fp64_1040": err32,
     }
     if one_step:

@@ -232,6 +232,16 @@ int sth_check(sth_ctx *ctx, float expected, uint64_t *errors);
 /* Interior max |x| and L2 norm of a field, accumulated in FP64 on the GPU. */
 int sth_norms(sth_ctx *ctx, sth_field f, double *maxabs, double *l2);
 
+/* Discrete energy of the LEAPFROG scheme, accumulated in FP64 on the GPU:
+ *   Q = <u,u> + <v,v> - <u, (c0 + c1 L) v>   over the interior,
+ * u the newest field, v the one before, L the stencil's neighbour sum with the
+ * weights the context's LEAPFROG steps use (2w per pair; FP64 contexts: their
+ * sth_set_weights choice), v's halo included.  For steps with s = 0 and zero
+ * halos Q is conserved exactly in exact arithmetic (M = c0 + c1 L is
+ * symmetric), so its drift measures the storage format's rounding.  No
+ * counterpart in the reference (README.md:10-24 states the update only). */
+int sth_energy(sth_ctx *ctx, float c0, float c1, double *energy);
+
 /* Error of field fa of context a against field fb of context b (the same
  * geometry and device; storage types may differ, e.g. an FP32 or 16-bit run
  * against its FP64 twin), over the interior, accumulated in FP64 on the GPU:

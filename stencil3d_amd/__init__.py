@@ -96,6 +96,7 @@ SIGNATURES = {
     "sth_check": (C.c_int, [_VP, C.c_float, _P(C.c_uint64)]),
     "sth_norms": (C.c_int, [_VP, C.c_int, _P(C.c_double), _P(C.c_double)]),
     "sth_compare": (C.c_int, [_VP, C.c_int, _VP, C.c_int, _P(C.c_double), _P(C.c_double)]),
+    "sth_energy": (C.c_int, [_VP, C.c_float, C.c_float, _P(C.c_double)]),
     "sth_set_weights": (C.c_int, [_VP, C.c_int]),
     "sth_get_weights": (C.c_int, [_VP, _P(C.c_int)]),
     "sth_set_tuning": (C.c_int, [_VP, _P(Tuning)]),
@@ -336,6 +337,14 @@ class Stencil:
         mx, l2 = C.c_double(), C.c_double()
         _ok(lib().sth_norms(self.handle, FIELDS[field], C.byref(mx), C.byref(l2)), "sth_norms")
         return mx.value, l2.value
+
+    def energy(self, c0, c1):
+        """The LEAPFROG scheme's discrete energy <u,u> + <v,v> - <u,(c0 + c1 L) v>
+        over the interior (u newest, v the one before), in FP64 on the GPU;
+        conserved by steps with s = 0 and zero halos."""
+        q = C.c_double()
+        _ok(lib().sth_energy(self.handle, c0, c1, C.byref(q)), "sth_energy")
+        return q.value
 
     def compare(self, field, ref, ref_field=None):
         """(max-rel, L2-rel) of this context's field against `ref`'s (same

@@ -29,12 +29,24 @@ struct FillSpec {
   int pi, pj, pk;
 };
 
+// Terms of the LEAPFROG operator M = c0 + c1 L for energy_kernel: element
+// offsets of the neighbour pairs and their weights in FP64.
+struct EnergySpec {
+  int T;
+  int64_t off[12];
+  double w[12];
+  double c0, c1;
+};
+
 hipError_t launch_fill(const FieldDesc &d, const FillSpec &f, void *dst, int fmt, int e, int pass,
                        uint32_t *maxbits, hipStream_t st);
 hipError_t launch_check(const FieldDesc &d, const void *v, int fmt, int e, float expected,
                         unsigned long long *errors, hipStream_t st);
 hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, double *partial,
                         uint32_t *maxbits, double *out, hipStream_t st);
+hipError_t launch_energy(const FieldDesc &d, const void *u, const void *v, int fmt, int eu,
+                         int ev, const EnergySpec &sp, double *partial, double *out,
+                         hipStream_t st);
 hipError_t launch_compare(const FieldDesc &dx, const void *x, int fx, int ex,
                           const FieldDesc &dr, const void *r, int fr, int er, double *pdiff,
                           double *pref, uint32_t *maxdiff, uint32_t *maxref, double *sums,

@@ -222,6 +222,38 @@ __global__ void compare_kernel(FieldDesc dx, const void *x, int fx, int ex, Fiel
   }
 }
 
+// Discrete energy of the LEAPFROG scheme (s = 0, zero halos): with
+// M = c0 + c1 L symmetric, Q = <u,u> + <v,v> - <u, M v> is the same after
+// every step u' = M u - v, v' = u.  Per-block FP64 sums over the interior of
+// u^2 + v^2 - u (c0 v + c1 L v), L v = sum_t w_t (v[c - o_t] + v[c + o_t]),
+// the neighbours read from v's halo-padded box.
+__global__ void energy_kernel(FieldDesc d, const void *u, const void *v, int fmt, int eu, int ev,
+                              EnergySpec sp, double *partial) {
+  __shared__ double ss[16];
+  double s = 0.0;
+  for (uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < d.elems;
+       idx += (uint64_t)gridDim.x * blockDim.x) {
+    const Cell c = decode(d, idx);
+    if (c.interior) {
+      const double x = load_dec(u, idx, fmt, eu), y = load_dec(v, idx, fmt, ev);
+      double L = 0.0;
+      for (int t = 0; t < sp.T; t++)
+        L += sp.w[t] * (load_dec(v, idx - (uint64_t)sp.off[t], fmt, ev) +
+                        load_dec(v, idx + (uint64_t)sp.off[t], fmt, ev));
+      s += x * x + y * y - x * (sp.c0 * y + sp.c1 * L);
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) ss[w] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += ss[q];
+    partial[blockIdx.x] = t;
+  }
+}
+
 __global__ void norms_finish(const double *partial, int n, double *out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     double t = 0.0;
@@ -338,6 +370,16 @@ hipError_t launch_norms(const FieldDesc &d, const void *x, int fmt, int e, doubl
                         uint32_t *maxbits, double *out, hipStream_t st) {
   hipLaunchKernelGGL(norms_kernel, dim3(kReduceBlocks), dim3(256), 0, st, d, x, fmt, e, partial,
                      maxbits);
+  hipLaunchKernelGGL(norms_finish, dim3(1), dim3(64), 0, st, (const double *)partial,
+                     kReduceBlocks, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy(const FieldDesc &d, const void *u, const void *v, int fmt, int eu,
+                         int ev, const EnergySpec &sp, double *partial, double *out,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(energy_kernel, dim3(kReduceBlocks), dim3(256), 0, st, d, u, v, fmt, eu, ev,
+                     sp, partial);
   hipLaunchKernelGGL(norms_finish, dim3(1), dim3(64), 0, st, (const double *)partial,
                      kReduceBlocks, out);
   return hipGetLastError();

@@ -1239,6 +1239,33 @@ int sth_norms(sth_ctx *c, sth_field fld, double *maxabs, double *l2) {
   return STH_OK;
 }
 
+int sth_energy(sth_ctx *c, float c0, float c1, double *energy) {
+  if (!c || !energy) return STH_ERR_ARG;
+  int st = fetch_slot(c);
+  if (st) return st;
+  SweepArgs a{};
+  fill_weights(c, true, a);  // the LEAPFROG weights (2w) the context's steps use
+  int ax[12], rr[12];
+  EnergySpec sp{};
+  sp.T = term_order(c->g.npts, ax, rr);
+  const int64_t stride[3] = {(int64_t)c->L.plane, (int64_t)c->L.pitch, 1};
+  for (int t = 0; t < sp.T; t++) {
+    sp.off[t] = stride[ax[t]] * rr[t];
+    sp.w[t] = c->fmt == FMT_F64 ? a.wd[t] : (double)a.w[t];
+  }
+  sp.c0 = (double)c0;
+  sp.c1 = (double)c1;
+  const int bu = buf_of(c, STH_FIELD_U), bv = buf_of(c, STH_FIELD_V);
+  const int eu = is16(c) ? c->host_slot.exp[bu] : 0, ev = is16(c) ? c->host_slot.exp[bv] : 0;
+  double q = 0.0;
+  STH_TRY(launch_energy(desc_of(c), c->base[bu], c->base[bv], c->fmt, eu, ev, sp, c->partial_dev,
+                        c->sum_dev, c->stream));
+  STH_TRY(hipMemcpyAsync(&q, c->sum_dev, 8, hipMemcpyDeviceToHost, c->stream));
+  STH_TRY(hipStreamSynchronize(c->stream));
+  *energy = q;
+  return STH_OK;
+}
+
 int sth_compare(sth_ctx *a, sth_field fa, sth_ctx *b, sth_field fb, double *max_rel,
                 double *l2_rel) {
   if (!a || !b || fa < 0 || fa > 2 || fb < 0 || fb > 2 || a == b) return STH_ERR_ARG;
