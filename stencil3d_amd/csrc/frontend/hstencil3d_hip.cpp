@@ -402,6 +402,10 @@ int main(int argc, char **argv) {
   t1 = seconds();
   printf("[rank0] Phase #%i: finish warmup in %f secs\n", ++phase, t1 - t0);
   report_check(1);
+  // LEAPFROG: the scheme's discrete energy before the timed steps (FP64 on
+  // the GPU, untimed); with s = 0 and zero halos the steps conserve it
+  double q0 = 0.0;
+  if (mode != 0 && (st = sth_energy(ctx, fc0, fc1, &q0))) die("energy", st);
 
   printf("[rank0] Phase #%i: start compute\n", ++phase);
   if ((st = sth_timer_start(ctx))) die("timer", st);
@@ -429,6 +433,10 @@ int main(int argc, char **argv) {
     double mx = 0, l2 = 0;
     if ((st = sth_norms(ctx, STH_FIELD_U, &mx, &l2))) die("norms", st);
     printf("norms: max |u| = %.9e, ||u||_2 = %.9e\n", mx, l2);
+    double q1 = 0.0;
+    if ((st = sth_energy(ctx, fc0, fc1, &q1))) die("energy", st);
+    printf("energy: before %.9e, after %.9e, relative change %.3e\n", q0, q1,
+           q0 != 0.0 ? fabs(q1 - q0) / fabs(q0) : fabs(q1 - q0));
     if (host_fields) {
       const int s2 = dtype == 3 ? download_host(ctx, hf64, STH_FIELD_U)
                                 : download_host(ctx, hf32, STH_FIELD_U);

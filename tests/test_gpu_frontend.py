@@ -250,6 +250,22 @@ def test_frontend_tune_keeps_the_physics(gpu, dtype):
     assert na and nb and na.group(0) == nb.group(0)
 
 
+@pytest.mark.parametrize("dtype,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("fp16", 1e-3)])
+def test_frontend_leapfrog_reports_conserved_energy(gpu, dtype, tol):
+    """A LEAPFROG run on the Gaussian pulse (s = 0, zero halos) prints the
+    scheme's discrete energy before and after the timed steps; it is
+    conserved up to the storage format's rounding."""
+    r = run_fe(gpu, "--xs", "40", "--ys", "44", "--zs", "72", "--iter", "20", "--stencil", "25",
+               "--mode", "leapfrog", "--init", "gauss", "--c1", "0.16", "--dtype", dtype)
+    assert r.returncode == 0, r.stderr
+    m = re.search(r"energy: before ([0-9.e+-]+), after ([0-9.e+-]+), relative change "
+                  r"([0-9.e+-]+)", r.stdout)
+    assert m, r.stdout
+    q0, q1, d = map(float, m.groups())
+    assert q0 > 0 and q1 > 0 and d <= tol
+    assert abs(abs(q1 - q0) / q0 - d) <= 1e-3 * d + 1e-8  # printed to 10 digits
+
+
 def test_frontend_default_c0_is_constant_preserving(gpu):
     """Without --c0 the 25-point LEAPFROG takes c0 = 2 - 12 c1 (the value
     sth_leapfrog_c0 gives), not 2 - 6 c1."""
