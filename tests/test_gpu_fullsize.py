@@ -11,7 +11,9 @@ oracle cannot follow: size-independent checks only.
     sweep count (iterators.cpp:304-325), the maximum equal to the exact FP32
     recurrence, and a wrong expectation flags every point.
   * 16-bit storage: the tracked maximum of each field equals the largest
-    stored value of its interior, inside the format's range."""
+    stored value of its interior, inside the format's range.
+  * The scheme's discrete energy (sth_energy) is conserved by the default
+    schedules, FP32 and FP16, on the headline workload."""
 from fractions import Fraction
 
 import numpy as np
@@ -176,3 +178,23 @@ def test_fp16_pass_tracks_one_step_at_headline_size(gpu, npts, spp):
     finally:
         for ctx in ctxs:
             ctx.close()
+
+
+@pytest.mark.parametrize("dtype,tol", [("fp32", 1e-6), ("fp16", 1e-4)])
+def test_energy_conserved_at_headline_size(gpu, dtype, tol):
+    """The headline workload (Gaussian pulse + noise, s = 0, zero halos) on
+    the default schedule conserves the scheme's discrete energy over 40 steps
+    within the storage format's rounding, while the same sum formed with a
+    centre weight 0.1 off (Q - 0.1 <u, v>, no invariant of these steps)
+    changes by far more (about a fifth at this size)."""
+    npts, c1 = 25, 0.16
+    c0 = gpu.leapfrog_c0(npts, c1)
+    with gpu.Stencil(N, N, N, npts=npts, dtype=dtype) as ctx:
+        ctx.init_gaussian("u", 1.0, 8.0, 1e-3, 1, 0)
+        ctx.init_gaussian("v", 1.0, 8.0, 1e-3, 1, 1)
+        ctx.init_point("s", 0, 0, 0, 0.0)
+        q0, b0 = ctx.energy(c0, c1), ctx.energy(c0 + 0.1, c1)
+        ctx.leapfrog(40, c0, c1)
+        q1, b1 = ctx.energy(c0, c1), ctx.energy(c0 + 0.1, c1)
+    assert q0 > 0 and abs(q1 - q0) / q0 <= tol, (q0, q1)
+    assert abs(b1 - b0) / abs(b0) > 1e2 * tol, (b0, b1)
