@@ -110,6 +110,26 @@ def test_null_handles_are_errors(sth):
     assert L.sth_timer_read(None, None, None) == sth.STH_ERR_STATE
 
 
+def test_every_entry_point_rejects_null_arguments(sth):
+    """Every entry point called with null handles and pointers (and zero
+    sizes) returns an error status, a null handle or zero, and touches no
+    memory: no call may crash a caller that passes a bad handle."""
+    import ctypes as C
+    L = sth.lib()
+    for name, (res, args) in sth.SIGNATURES.items():
+        if not args or name == "sth_strerror":
+            continue
+        vals = [0.0 if a in (C.c_float, C.c_double) else
+                0 if a in (C.c_int, C.c_uint32, C.c_uint64) else None for a in args]
+        r = getattr(L, name)(*vals)
+        if name == "sth_ricker":
+            assert r == 0  # zero steps: nothing to write
+        elif res is C.c_int:
+            assert r != 0, name
+        else:
+            assert not r, name
+
+
 def test_create_without_gpu_is_a_status(sth):
     if sth.device_count() > 0:
         pytest.skip("a GPU is visible")
