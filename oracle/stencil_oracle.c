@@ -5,8 +5,9 @@
  *
  * Built with -ffp-contract=off: every fused multiply-add below is an explicit
  * fmaf()/fma() call and every other product is rounded on its own, which is
- * the order the reference host build produces (Appendix B of the survey of
- * stencil_N.h under clang++/g++ -O3 -march with FMA).
+ * the order the reference host build produces (stencil_N.h compiled with
+ * clang++/g++ -O3 -march=... with FMA contracts each written a*b + c into one
+ * fused multiply-add; the golden vectors of that build pin it).
  */
 #include "stencil_oracle.h"
 

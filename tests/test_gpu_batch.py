@@ -56,7 +56,8 @@ def test_batch_matches_oracle(gpu, orc, method, graph, npts):
 
 
 def test_persistent_equals_sequential_config_shapes(gpu, orc):
-    """Eight shots with sides from the survey config's set, 12 steps: the
+    """Eight shots with sides from the batching workload's set
+    {96, 128, 192, 256, 320, 384}, 12 steps: the
     persistent, sequential, two- and three-steps-per-pass and graphed runs
     agree bit for bit."""
     rng = np.random.default_rng(2)
