@@ -6,8 +6,9 @@
  * Built with -ffp-contract=off: every fused multiply-add below is an explicit
  * fmaf()/fma() call and every other product is rounded on its own, which is
  * the order the reference host build produces (stencil_N.h compiled with
- * clang++/g++ -O3 -march=... with FMA contracts each written a*b + c into one
- * fused multiply-add; the golden vectors of that build pin it).
+ * clang++/g++ -O3 and an FMA-capable -march: the first product c*pair is
+ * rounded, coef0*u0 and every later term are fused, v0 - A*u0 is fused; the
+ * golden vectors of that build and the live comparison with it pin this).
  */
 #include "stencil_oracle.h"
 
