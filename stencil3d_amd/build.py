@@ -84,6 +84,20 @@ def build(force: bool = True, quiet: bool = True, jobs: int = 8) -> str:
     return LIB
 
 
+REFDIR = os.environ.get("STH_REFDIR", "/root/reference")
+
+
+def build_refbind(quiet: bool = True) -> bool:
+    """Where the reference's sources are present: its own driver (main.cpp)
+    with csrc/refbind/iterators_hip.cpp in place of iterators.cpp, linked with
+    the library, into stencil3d_amd/_refbind/.  Returns whether it was built."""
+    if not os.path.isfile(os.path.join(REFDIR, "main.cpp")):
+        return False
+    subprocess.run(["make", "-C", os.path.join(CSRC, "refbind"), "-j6", f"REFDIR={REFDIR}"],
+                   check=True, stdout=subprocess.DEVNULL if quiet else None)
+    return True
+
+
 if __name__ == "__main__":
     build(force="--force" in sys.argv or True, quiet=False)
     print(LIB)

@@ -13,6 +13,7 @@
 #include <array>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fields.h"
@@ -109,9 +110,11 @@ inline float dec16(int fmt, uint16_t q, int e) {
   return ldexpf(u2f((uint32_t)q << 16), e);
 }
 
+// Pinned staging for host transfers: two halves of at least one device
+// plane each, so the host fills (or reads) one while the other is in flight.
 int ensure_staging(sth_ctx *c) {
   if (c->staging) return STH_OK;
-  const size_t want = std::max<size_t>((size_t)64 << 20, c->L.plane * c->eb);
+  const size_t want = std::max<size_t>((size_t)128 << 20, 2 * c->L.plane * c->eb);
   STH_TRY(hipHostMalloc(&c->staging, want, hipHostMallocDefault));
   c->staging_bytes = want;
   return STH_OK;
@@ -134,38 +137,97 @@ inline HostIdx host_idx(const sth_ctx *c, const sth_host_layout *hl) {
   return h;
 }
 
+// Host threads for the row conversions of a transfer (memory-bound loops
+// over host arrays; no HIP call is made on them).
+int host_threads() {
+  static const int n = [] {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+  }();
+  return n;
+}
+
+// fn(i, j, t) for every cell row (i, j) of planes [ia, ib), rows [-R, ys+R),
+// split into contiguous runs over `host_threads()` threads (t: the thread).
+template <class Fn>
+void parallel_rows(int64_t ia, int64_t ib, int64_t R, int64_t ys, Fn &&fn) {
+  const int64_t nj = ys + 2 * R, rows = (ib - ia) * nj;
+  const int nt = (int)std::min<int64_t>(host_threads(), std::max<int64_t>(1, rows / 64));
+  auto run = [&](int t) {
+    const int64_t r0 = rows * t / nt, r1 = rows * (t + 1) / nt;
+    for (int64_t r = r0; r < r1; r++) fn(ia + r / nj, r % nj - R, t);
+  };
+  if (nt == 1) {
+    run(0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt - 1);
+  for (int t = 1; t < nt; t++) pool.emplace_back(run, t);
+  run(0);
+  for (auto &th : pool) th.join();
+}
+
 // Moves planes [-R, xs+R) between a host dense array and a device buffer via
-// the pinned staging buffer.  `row` is called per (i, j) cell row with the
-// host row pointer offset (k = -R) and the staging row (k = -R).
+// the pinned staging buffer, in batches of whole planes that alternate
+// between its two halves: uploads fill one half while the other's copy runs,
+// downloads convert one half while the next batch arrives in the other.
+// `row(i, j, srow, t)` is called per (i, j) cell row with the staging row
+// (k = -R) on host thread t (rows of one batch run concurrently).
 template <class RowFn>
 int transfer_planes(sth_ctx *c, int buf, bool to_device, RowFn row) {
   int st = ensure_staging(c);
   if (st) return st;
   const int64_t R = c->R, xs = c->g.xs, ys = c->g.ys;
   const size_t plane_bytes = c->L.plane * c->eb;
-  const int64_t per = std::max<int64_t>(1, (int64_t)(c->staging_bytes / plane_bytes));
+  const int64_t per = std::max<int64_t>(1, (int64_t)(c->staging_bytes / 2 / plane_bytes));
+  char *half[2] = {(char *)c->staging, (char *)c->staging + (size_t)per * plane_bytes};
   char *dev = (char *)c->base[buf];
   if (to_device) c->twins_ok = false;
-  for (int64_t ia = -R; ia < xs + R; ia += per) {
-    const int64_t ib = std::min<int64_t>(xs + R, ia + per);
-    const size_t bytes = (size_t)(ib - ia) * plane_bytes;
-    char *dptr = dev + plane_start(c, ia) * c->eb;
-    if (!to_device) {
-      STH_TRY(hipMemcpyAsync(c->staging, dptr, bytes, hipMemcpyDeviceToHost, c->stream));
-      STH_TRY(hipStreamSynchronize(c->stream));
+  const int64_t nb = (xs + 2 * R + per - 1) / per;  // batches
+  auto first = [&](int64_t b) { return -R + b * per; };
+  auto last = [&](int64_t b) { return std::min<int64_t>(xs + R, first(b) + per); };
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (int q = 0; q < 2 && !st; q++)
+    if (hipEventCreateWithFlags(&ev[q], hipEventDisableTiming) != hipSuccess) st = STH_ERR_HIP;
+  auto copy = [&](int64_t b) {  // batch b between the device and its half
+    const size_t bytes = (size_t)(last(b) - first(b)) * plane_bytes;
+    char *dptr = dev + plane_start(c, first(b)) * c->eb;
+    const hipError_t e =
+        to_device ? hipMemcpyAsync(dptr, half[b & 1], bytes, hipMemcpyHostToDevice, c->stream)
+                  : hipMemcpyAsync(half[b & 1], dptr, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) return hipEventRecord(ev[b & 1], c->stream);
+    return e;
+  };
+  auto fill = [&](int64_t b) {  // the host side of batch b
+    const int64_t ia = first(b);
+    char *base = half[b & 1];
+    parallel_rows(ia, last(b), R, ys, [&](int64_t i, int64_t j, int t) {
+      row(i, j, base + ((size_t)(i - ia) * c->L.plane + (size_t)(j + R) * c->L.pitch) * c->eb,
+          t);
+    });
+  };
+  hipError_t e = hipSuccess;
+  if (!st && to_device) {
+    for (int64_t b = 0; b < nb && e == hipSuccess; b++) {
+      if (b >= 2) e = hipEventSynchronize(ev[b & 1]);  // this half's previous copy is done
+      if (e != hipSuccess) break;
+      fill(b);
+      e = copy(b);
     }
-    for (int64_t i = ia; i < ib; i++)
-      for (int64_t j = -R; j < ys + R; j++) {
-        char *srow = (char *)c->staging +
-                     ((size_t)(i - ia) * c->L.plane + (size_t)(j + R) * c->L.pitch) * c->eb;
-        row(i, j, srow);
-      }
-    if (to_device) {
-      STH_TRY(hipMemcpyAsync(dptr, c->staging, bytes, hipMemcpyHostToDevice, c->stream));
-      STH_TRY(hipStreamSynchronize(c->stream));
+  } else if (!st) {
+    e = copy(0);
+    for (int64_t b = 0; b < nb && e == hipSuccess; b++) {
+      if (b + 1 < nb) e = copy(b + 1);  // its half held batch b-1, converted already
+      if (e == hipSuccess) e = hipEventSynchronize(ev[b & 1]);
+      if (e == hipSuccess) fill(b);
     }
   }
-  return STH_OK;
+  if (e == hipSuccess && !st) e = hipStreamSynchronize(c->stream);
+  for (int q = 0; q < 2; q++)
+    if (ev[q]) (void)hipEventDestroy(ev[q]);
+  if (st) return st;
+  return e == hipSuccess ? STH_OK : hip_status(e);
 }
 
 // Current 16-bit scale state of the context (from the device after sweeps).
@@ -887,6 +949,13 @@ int sth_init_point(sth_ctx *c, sth_field fld, uint32_t i, uint32_t j, uint32_t k
 
 namespace {
 
+// Per-thread partials of a transfer (one cache line each).
+struct alignas(64) RowAcc {
+  float mx = 0.0f;   // max |value| read
+  float qmax = 0.0f; // 16-bit: max |stored value|
+  bool hz = true;    // every halo cell read is zero
+};
+
 // Host (float or double) dense array -> device field of any storage type.
 template <class HT>
 int upload_any(sth_ctx *c, sth_field fld, const HT *host, const sth_host_layout *hl) {
@@ -894,47 +963,67 @@ int upload_any(sth_ctx *c, sth_field fld, const HT *host, const sth_host_layout 
   int st = check_host_layout(c, hl);
   if (st) return st;
   const HostIdx h = host_idx(c, hl);
-  const int64_t R = c->R, zs = c->g.zs, xs = c->g.xs, ys = c->g.ys;
+  const int64_t R = c->R, zs = c->g.zs, xs = c->g.xs, ys = c->g.ys, n = zs + 2 * R;
   const int buf = buf_of(c, fld);
-  // halo-zero flag and (16-bit) the max over the cells the sweep reads
-  bool hz = true;
-  float mx = 0.0f;
-  for (int64_t i = -R; i < xs + R; i++)
-    for (int64_t j = -R; j < ys + R; j++) {
-      const HT *src = host + h.ofs + i * h.im + j * h.jm - R;
-      const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
-      for (int64_t k = 0; k < zs + 2 * R; k++) {
-        const float x = (float)src[k];
-        mx = std::max(mx, fabsf(x));
-        if (src[k] != 0 && (!inrow || k < R || k >= zs + R)) hz = false;
+  const bool q16 = is16(c);
+  std::vector<RowAcc> acc(host_threads());
+  // a nonzero value at a halo cell the sweep reads clears the halo-zero flag
+  auto halo_scan = [&](int64_t i, int64_t j, const HT *src, RowAcc &a) {
+    if (!a.hz) return;
+    const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
+    for (int64_t k = 0; k < n; k++)
+      if (src[k] != 0 && (!inrow || k < R || k >= zs + R)) {
+        a.hz = false;
+        return;
       }
-    }
+  };
   int e = 0;
-  if (is16(c)) {
+  if (q16) {
+    // 16-bit: the max over the cells the sweep reads sets the exponent
+    // before any value is encoded
+    parallel_rows(-R, xs + R, R, ys, [&](int64_t i, int64_t j, int t) {
+      const HT *src = host + h.ofs + i * h.im + j * h.jm - R;
+      RowAcc &a = acc[t];
+      float m = a.mx;
+      for (int64_t k = 0; k < n; k++) m = std::max(m, fabsf((float)src[k]));
+      a.mx = m;
+      halo_scan(i, j, src, a);
+    });
+    float mx = 0.0f;
+    for (const RowAcc &a : acc) mx = std::max(mx, a.mx);
     st = fetch_slot(c);
     if (st) return st;
     e = scale_exp((double)mx);
   }
-  float qmax = 0.0f;
-  st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow) {
+  st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow, int t) {
     const HT *src = host + h.ofs + i * h.im + j * h.jm - R;
-    const int64_t n = zs + 2 * R;
+    RowAcc &a = acc[t];
     if (c->fmt == FMT_F32) {
       float *d = (float *)srow;
       for (int64_t k = 0; k < n; k++) d[k] = (float)src[k];
+      halo_scan(i, j, src, a);
     } else if (c->fmt == FMT_F64) {
       double *d = (double *)srow;
       for (int64_t k = 0; k < n; k++) d[k] = (double)src[k];
+      halo_scan(i, j, src, a);
     } else {
       uint16_t *d = (uint16_t *)srow;
+      float q = a.qmax;
       for (int64_t k = 0; k < n; k++) {
         d[k] = enc16(c->fmt, (float)src[k], e);
-        qmax = std::max(qmax, fabsf(dec16(c->fmt, d[k], 0)));
+        q = std::max(q, fabsf(dec16(c->fmt, d[k], 0)));
       }
+      a.qmax = q;
     }
   });
   if (st) return st;
-  if (is16(c)) {
+  bool hz = true;
+  float qmax = 0.0f;
+  for (const RowAcc &a : acc) {
+    hz = hz && a.hz;
+    qmax = std::max(qmax, a.qmax);
+  }
+  if (q16) {
     c->host_slot.exp[buf] = e;
     c->host_slot.maxbits[buf] = f2u(qmax);
   }
@@ -954,7 +1043,7 @@ int download_any(sth_ctx *c, sth_field fld, HT *host, const sth_host_layout *hl)
   const int64_t R = c->R, zs = c->g.zs;
   const int buf = buf_of(c, fld);
   const int e = is16(c) ? c->host_slot.exp[buf] : 0;
-  return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow) {
+  return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow, int) {
     HT *dst = host + h.ofs + i * h.im + j * h.jm - R;
     const int64_t n = zs + 2 * R;
     if (c->fmt == FMT_F32) {
@@ -1002,19 +1091,25 @@ int sth_upload_raw16(sth_ctx *c, sth_field fld, const uint16_t *host, int exp,
   const HostIdx h = host_idx(c, hl);
   const int64_t R = c->R, zs = c->g.zs, xs = c->g.xs, ys = c->g.ys;
   const int buf = buf_of(c, fld);
-  float qmax = 0.0f;
-  bool hz = true;
-  st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow) {
+  std::vector<RowAcc> acc(host_threads());
+  st = transfer_planes(c, buf, true, [&](int64_t i, int64_t j, char *srow, int t) {
     const uint16_t *src = host + h.ofs + i * h.im + j * h.jm - R;
     memcpy(srow, src, (size_t)(zs + 2 * R) * 2);
     const bool inrow = i >= 0 && i < xs && j >= 0 && j < ys;
+    RowAcc &a = acc[t];
     for (int64_t k = 0; k < zs + 2 * R; k++) {
       const float x = dec16(c->fmt, src[k], 0);
-      qmax = std::max(qmax, fabsf(x));
-      if (x != 0.0f && (!inrow || k < R || k >= zs + R)) hz = false;
+      a.qmax = std::max(a.qmax, fabsf(x));
+      if (x != 0.0f && (!inrow || k < R || k >= zs + R)) a.hz = false;
     }
   });
   if (st) return st;
+  float qmax = 0.0f;
+  bool hz = true;
+  for (const RowAcc &a : acc) {
+    hz = hz && a.hz;
+    qmax = std::max(qmax, a.qmax);
+  }
   c->host_slot.exp[buf] = exp;
   c->host_slot.maxbits[buf] = f2u(qmax);
   c->halo_zero[buf] = hz;
@@ -1034,7 +1129,7 @@ int sth_download_raw16(sth_ctx *c, sth_field fld, uint16_t *host, int *exp, floa
   const int buf = buf_of(c, fld);
   if (exp) *exp = c->host_slot.exp[buf];
   if (maxabs) *maxabs = u2f(c->host_slot.maxbits[buf]);
-  return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow) {
+  return transfer_planes(c, buf, false, [&](int64_t i, int64_t j, char *srow, int) {
     memcpy(host + h.ofs + i * h.im + j * h.jm - R, srow, (size_t)(zs + 2 * R) * 2);
   });
 }
