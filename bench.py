@@ -425,6 +425,30 @@ def cpu_baseline(n, sweeps, npts=NPTS):
             "sample": what, "secs": round(secs, 3), "cpu": cpu}
 
 
+def reference_driver_gpu(n, sweeps, npts=NPTS):
+    """The reference's own driver (main.cpp, unchanged) linked with the GPU
+    iterators of stencil3d_amd/csrc/refbind: its report of the timed ITERATE
+    (host arrays, so each call uploads u, s, v and downloads v), next to the
+    GPU sweeps alone.  None where the binary was not built."""
+    binary = os.path.join(ROOT, "stencil3d_amd", "_refbind", f"hstencil3d_gpu_{npts}")
+    if not os.path.exists(binary):
+        return None
+    cmd = [binary, "--xs", str(n), "--ys", str(n), "--zs", str(n), "--xh", "4", "--yh", "4",
+           "--zh", "4", "--iter", str(sweeps)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    m = re.search(r"finish compute in ([0-9.]+) secs \(([0-9.]+) GFlop/s", r.stdout)
+    checks = re.findall(r"check: (\d+) errors found", r.stdout)
+    if r.returncode != 0 or not m:
+        raise RuntimeError(f"reference driver on the GPU failed: {r.stderr[-500:]}")
+    secs = float(m.group(1))
+    return {"sweeps": sweeps, "secs": round(secs, 4), "gflops_report": float(m.group(2)),
+            "gpts_incl_transfers": round(n ** 3 * sweeps / secs / 1e9, 3),
+            "checks_errors": [int(x) for x in checks],
+            "what": (f"reference main.cpp + grid/gilbert2d/aux with csrc/refbind/iterators_hip.cpp "
+                     f"(hstencil3d_gpu_{npts}), {n}^3, {sweeps} REF sweeps in one ITERATE; host "
+                     f"arrays: the time includes uploading u, s, v and downloading v")}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
@@ -594,6 +618,11 @@ def main():
         cpu7 = cpu_baseline(n, args.cpu_sweeps, npts=7)
         log(f"[bench] cpu 7-pt: {cpu7['value']} Gpts/s")
         extra["cpu_baseline_7pt"] = cpu7
+        rd = reference_driver_gpu(n, 20)
+        if rd is not None:
+            extra["reference_driver_gpu"] = rd
+            log(f"[bench] reference driver on the GPU: {rd['secs']} s for 20 sweeps "
+                f"({rd['gpts_incl_transfers']} Gpts/s with transfers)")
 
     line = {
         "metric": METRIC,
