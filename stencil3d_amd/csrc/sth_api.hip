@@ -163,7 +163,12 @@ void parallel_rows(int64_t ia, int64_t ib, int64_t R, int64_t ys, Fn &&fn) {
   }
   std::vector<std::thread> pool;
   pool.reserve(nt - 1);
-  for (int t = 1; t < nt; t++) pool.emplace_back(run, t);
+  int t = 1;
+  try {
+    for (; t < nt; t++) pool.emplace_back(run, t);
+  } catch (...) {  // no thread to spare: the calling thread takes the rest
+  }
+  for (int u = t; u < nt; u++) run(u);
   run(0);
   for (auto &th : pool) th.join();
 }
@@ -223,7 +228,10 @@ int transfer_planes(sth_ctx *c, int buf, bool to_device, RowFn row) {
       if (e == hipSuccess) fill(b);
     }
   }
-  if (e == hipSuccess && !st) e = hipStreamSynchronize(c->stream);
+  // drain the stream on every path: no copy may still use the staging buffer
+  // when the call returns
+  const hipError_t es = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = es;
   for (int q = 0; q < 2; q++)
     if (ev[q]) (void)hipEventDestroy(ev[q]);
   if (st) return st;
