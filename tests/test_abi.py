@@ -130,6 +130,22 @@ def test_every_entry_point_rejects_null_arguments(sth):
             assert not r, name
 
 
+def test_reference_driver_without_gpu_reports_the_library_error(sth):
+    """The reference's main.cpp linked with the GPU iterators (built where the
+    reference's sources are present) loads libstencil_hip and, with no GPU,
+    stops at its first ITERATE with the library's message and exit code 3."""
+    if sth.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    exe = os.path.join(os.path.dirname(sth.LIB_PATH), "_refbind", "hstencil3d_gpu_7")
+    if not os.path.exists(exe):
+        pytest.skip("built only where the reference's sources are present")
+    r = subprocess.run([exe, "--xs", "16", "--ys", "16", "--zs", "32", "--iter", "1"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3
+    assert "libstencil_hip: sth_create:" in r.stderr
+    assert "Phase #3: initial warmup" in r.stdout  # init_arrays ran on the host arrays
+
+
 def test_create_without_gpu_is_a_status(sth):
     if sth.device_count() > 0:
         pytest.skip("a GPU is visible")
